@@ -1,0 +1,114 @@
+// common.h -- shared device helpers and internal host structures of libgbgpu.
+//
+// Arithmetic conventions (all exact, all integer):
+//   * residues are uint32 in [0, p), 2 <= p < 2^31;
+//   * accumulators are uint64 and only ever receive non-negative addends,
+//     so they can be summed with LDS atomics and reduced lazily;
+//   * for p < 2^17 an addend is the raw product (p - v) * c < 2^34 (the
+//     delayed reduction of la_ff_32.c:528-536, good for 2^29 additions);
+//   * for larger p the product is reduced first, (p - v) * c mod p < 2^31,
+//     which keeps 2^33 additions exact (la_ff_32.c:593-606 reaches the same
+//     residues by subtracting and lifting by p^2).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/gbgpu.h"
+
+namespace gbgpu {
+
+// Barrett reduction of a 64-bit value by p < 2^31 with m = floor((2^64-1)/p):
+// q = floor(x m / 2^64) is floor(x/p) or one less, so one correction step.
+struct Mod {
+    uint32_t p;
+    uint64_t m;
+
+    __host__ static Mod make(uint32_t p_) {
+        Mod r;
+        r.p = p_;
+        r.m = ~0ull / p_;
+        return r;
+    }
+    __device__ __forceinline__ uint32_t reduce(uint64_t x) const {
+        const uint64_t q = __umul64hi(x, m);
+        uint64_t r = x - q * (uint64_t)p;
+        return (uint32_t)(r >= p ? r - p : r);
+    }
+    __device__ __forceinline__ uint32_t mul(uint32_t a, uint32_t b) const {
+        return reduce((uint64_t)a * (uint64_t)b);
+    }
+};
+
+// addend for an accumulator: see the header comment
+template <bool BIG>
+__device__ __forceinline__ uint64_t addend(uint32_t mul, uint32_t c, const Mod &md) {
+    if (BIG) return (uint64_t)md.mul(mul, c);
+    return (uint64_t)mul * (uint64_t)c;
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+}  // namespace gbgpu
+
+// ------------------------------------------------------------- host side
+
+struct gbgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int num_cu = 256;
+    std::string err;
+    double phase_ms[3] = {-1, -1, -1};
+    hipEvent_t ev[6] = {};
+};
+
+namespace gbgpu {
+
+int set_error(gbgpu_ctx *ctx, int code, const std::string &msg);
+int check_hip(gbgpu_ctx *ctx, hipError_t e, const char *what);
+
+#define GBGPU_HIP(ctx, call)                                   \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return gbgpu::check_hip((ctx), e_, #call); \
+    } while (0)
+
+template <class T>
+int dalloc(gbgpu_ctx *ctx, T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return set_error(ctx, GBGPU_ENOMEM,
+                         "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+    }
+    return GBGPU_OK;
+}
+
+template <class T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+// exclusive prefix sum of n int64 values into out (may alias), returns the
+// total on the host; scratch from hipcub
+int exclusive_scan_i64(gbgpu_ctx *ctx, const int64_t *in, int64_t *out, int64_t n,
+                       int64_t *total);
+int exclusive_scan_i32(gbgpu_ctx *ctx, const int32_t *in, int32_t *out, int64_t n,
+                       int64_t *total);
+
+}  // namespace gbgpu

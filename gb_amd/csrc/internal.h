@@ -1,0 +1,83 @@
+// internal.h -- device-resident objects behind the opaque C-ABI handles.
+#pragma once
+
+#include "common.h"
+
+// One F4 matrix on the device plus everything derived from it.
+struct gbgpu_mat {
+    gbgpu_ctx *ctx = nullptr;
+    int32_t nr = 0, nc = 0, ncl = 0, ncf = 0;
+    uint32_t p = 0;
+    int64_t nnz = 0, npool = 0;
+    // wire format, as uploaded
+    int64_t *rp = nullptr;
+    int32_t *ci = nullptr;
+    int32_t *rcf = nullptr;
+    int64_t *cp = nullptr;
+    uint32_t *cf = nullptr;
+    // known-pivot split (first row per lead = pivot, la_ff_32.c:1106-1116)
+    int32_t split_lim = -1;     // leads were checked against [0, split_lim)
+    int32_t *first = nullptr;   // nc: first row with that lead, INT32_MAX if none
+    int32_t *upiv = nullptr;    // to-reduce rows in the reference's order
+    int32_t nrl = 0;
+    // pivot rows bucketed by column window (sparse pass)
+    int32_t wshift = 0, nwin = 0;
+    int64_t *seg = nullptr;     // (nwin + 1) x ncl, window-major
+    uint2 *bucket = nullptr;    // (column, coefficient), leads excluded
+    uint32_t *pbits = nullptr;  // has-pivot bit per left column
+    int64_t nbucket = 0;
+    // sparse pass output
+    uint32_t *dprime = nullptr; // nrl x ncr, row q = upiv[q]'s remainder
+    int32_t *dnz = nullptr;     // nrl: 1 if row q is nonzero
+    int32_t *dlist = nullptr;   // nd: nonzero rows q, in order
+    int32_t nd = 0;
+    // result rows (device)
+    int64_t *orp = nullptr;
+    int32_t *oci = nullptr;
+    uint32_t *ocf = nullptr;
+    int32_t onp = 0;
+    int64_t onnz = 0;
+    ~gbgpu_mat();
+};
+
+// A dense block and the state of its echelon form.
+struct gbgpu_dense {
+    gbgpu_ctx *ctx = nullptr;
+    int32_t m = 0, n = 0;
+    int64_t ld = 0;
+    uint32_t p = 0;
+    uint32_t *a = nullptr;      // m x ld, row-major
+    bool own_a = true;
+    // result: rank rows, decreasing lead
+    int32_t rank = 0;
+    int32_t *orow = nullptr;    // rank: row of a holding the k-th output row
+    int32_t *olead = nullptr;   // rank: its lead column
+    ~gbgpu_dense();
+};
+
+namespace gbgpu {
+
+// sparse.hip
+int prep_split(gbgpu_mat *m, int32_t lim);
+int prep_buckets(gbgpu_mat *m);
+int run_sparse_pass(gbgpu_mat *m);
+
+// dense.hip: reduced echelon form of d (in place); form = GBGPU_FORM_*
+int dense_rref(gbgpu_dense *d, int form);
+
+// rows.hip: rows of a dense RREF to CSR (columns shifted by col_shift);
+// with reverse the output order is increasing lead
+int dense_to_rows(gbgpu_dense *d, int32_t col_shift, int reverse, int64_t **orp, int32_t **oci,
+                  uint32_t **ocf, int64_t *onnz);
+
+// rows i with keep[i] of m scattered densely into row dst_row[i] of a
+// fresh nkeep x nc block
+int scatter_rows(gbgpu_mat *m, const int32_t *keep, const int32_t *dst_row, int32_t nkeep,
+                 gbgpu_dense **out);
+
+// gather rows `list` (n of them) of src (row stride ld_src, ncols wide) into
+// a fresh dense block
+int gather_dense(gbgpu_ctx *ctx, const uint32_t *src, int64_t ld_src, const int32_t *list,
+                 int32_t n, int32_t ncols, uint32_t p, gbgpu_dense **out);
+
+}  // namespace gbgpu

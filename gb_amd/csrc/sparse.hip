@@ -1,0 +1,515 @@
+// sparse.hip -- the plain sparse pass of F4 linear algebra on gfx950.
+//
+// Replaces sparse_AB_CD_linear_algebra_ff_{8,16,32} (la_ff_32.c:1081-1179)
+// with its row kernel reduce_dense_row_by_old_pivots_{17,31}_bit
+// (la_ff_32.c:496-630, la_ff_16.c:289-351).
+//
+// Layout in HBM:
+//   * the uploaded CSR (rp, ci, rcf, cp, cf) is read once per row;
+//   * the known pivot rows are re-laid out once per matrix as (column,
+//     coefficient) pairs -- 8 bytes, one coalesced load -- bucketed by column
+//     window: seg[w * ncl + j] .. seg[(w+1) * ncl + j] are pivot j's entries
+//     in window w (window-major so a wave loads 64 consecutive pivots'
+//     offsets with one coalesced load);
+//   * every to-reduce row q writes its dense remainder D'[q] (ncr uint32)
+//     and a nonzero flag.
+//
+// One wavefront owns one to-reduce row at a time (persistent grid, rows
+// taken from an atomic queue, longest rows first).  Its accumulator is a
+// window of W uint64 columns in LDS.  Per window:
+//   1. zero the window, scatter the row's own entries that fall in it;
+//   2. deferred part: add x_j * P_j[window] for every pivot j already
+//      applied in an earlier window (multipliers x_j kept in a per-wave
+//      scratch row); work is load-balanced over the lanes and summed with
+//      LDS atomics;
+//   3. chain part: walk the window's pivot columns left to right -- a ballot
+//      over 64 columns finds the next column whose value is nonzero mod p,
+//      its multiplier x = p - v is stored and the pivot's entries inside the
+//      window are added (entries right of the window are deferred);
+//   4. the window's right-hand columns are reduced mod p and written out.
+// The addends are non-negative, so the uint64 window is exact with lazy
+// reduction (see common.h).
+#include <hipcub/hipcub.hpp>
+
+#include <climits>
+
+#include "internal.h"
+
+namespace gbgpu {
+
+// ------------------------------------------------------------ split
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// lead of every row; first row per lead; contract errors into *err
+__global__ void k_leads(int32_t nr, const int64_t *rp, const int32_t *ci, int32_t lim,
+                        int32_t *first, int32_t *err) {
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += gridDim.x * blockDim.x) {
+        const int64_t b = rp[i], e = rp[i + 1];
+        if (e <= b) { atomicOr(err, 1); continue; }
+        const int32_t lead = ci[b];
+        if (lead < 0 || lead >= lim) { atomicOr(err, 2); continue; }
+        atomicMin(&first[lead], i);
+    }
+}
+
+__global__ void k_nonpiv(int32_t nr, const int64_t *rp, const int32_t *ci, const int32_t *first,
+                         int32_t *flag) {
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += gridDim.x * blockDim.x)
+        flag[i] = first[ci[rp[i]]] != i;
+}
+
+// the to-reduce list is filled from its end (upivs[nrl - j],
+// la_ff_32.c:1106-1116): reversed order of appearance
+__global__ void k_upiv(int32_t nr, const int32_t *flag, const int32_t *rank, int32_t nrl,
+                       int32_t *upiv) {
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += gridDim.x * blockDim.x)
+        if (flag[i]) upiv[nrl - 1 - rank[i]] = i;
+}
+
+static int grid_for(int64_t n, int bs) {
+    int64_t g = (n + bs - 1) / bs;
+    if (g < 1) g = 1;
+    if (g > 65535 * 4) g = 65535 * 4;
+    return (int)g;
+}
+
+int prep_split(gbgpu_mat *m, int32_t lim) {
+    gbgpu_ctx *ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    if (m->split_lim == lim) return GBGPU_OK;
+    dfree(m->first);
+    dfree(m->upiv);
+    int rc;
+    if ((rc = dalloc(ctx, &m->first, (size_t)m->nc))) return rc;
+    int32_t *err = nullptr, *flag = nullptr, *rank = nullptr;
+    if ((rc = dalloc(ctx, &err, 1))) return rc;
+    if ((rc = dalloc(ctx, &flag, (size_t)m->nr)) || (rc = dalloc(ctx, &rank, (size_t)m->nr))) {
+        dfree(err); dfree(flag); return rc;
+    }
+    k_fill_i32<<<grid_for(m->nc, 256), 256, 0, s>>>(m->first, m->nc, INT32_MAX);
+    k_fill_i32<<<1, 64, 0, s>>>(err, 1, 0);
+    k_leads<<<grid_for(m->nr, 256), 256, 0, s>>>(m->nr, m->rp, m->ci, lim, m->first, err);
+    int32_t herr = 0;
+    GBGPU_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GBGPU_HIP(ctx, hipStreamSynchronize(s));
+    if (herr) {
+        dfree(err); dfree(flag); dfree(rank);
+        return set_error(ctx, GBGPU_ECONTRACT,
+                         (herr & 1) ? "empty row in matrix"
+                                    : "row lead outside the known-pivot columns");
+    }
+    k_nonpiv<<<grid_for(m->nr, 256), 256, 0, s>>>(m->nr, m->rp, m->ci, m->first, flag);
+    int64_t nrl = 0;
+    if ((rc = exclusive_scan_i32(ctx, flag, rank, m->nr, &nrl))) {
+        dfree(err); dfree(flag); dfree(rank); return rc;
+    }
+    m->nrl = (int32_t)nrl;
+    if ((rc = dalloc(ctx, &m->upiv, (size_t)nrl))) {
+        dfree(err); dfree(flag); dfree(rank); return rc;
+    }
+    k_upiv<<<grid_for(m->nr, 256), 256, 0, s>>>(m->nr, flag, rank, m->nrl, m->upiv);
+    GBGPU_HIP(ctx, hipGetLastError());
+    GBGPU_HIP(ctx, hipStreamSynchronize(s));
+    dfree(err); dfree(flag); dfree(rank);
+    m->split_lim = lim;
+    return GBGPU_OK;
+}
+
+// ---------------------------------------------------------- buckets
+
+// pivot entry counts (lead excluded) and has-pivot bits per left column
+__global__ void k_pivlen(int32_t ncl, const int32_t *first, const int64_t *rp, int64_t *plen,
+                         uint32_t *pbits) {
+    for (int32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncl; c += gridDim.x * blockDim.x) {
+        const int32_t r = first[c];
+        const bool has = r != INT32_MAX;
+        plen[c] = has ? rp[r + 1] - rp[r] - 1 : 0;
+        const uint64_t bal = __ballot(has);
+        const int lane = threadIdx.x & 63;
+        // 64 consecutive columns per wave when blockDim is a multiple of 64
+        if (lane == 0) pbits[c >> 5] = (uint32_t)bal;
+        if (lane == 32) pbits[c >> 5] = (uint32_t)(bal >> 32);
+    }
+}
+
+// one wave per pivot column: histogram by window, scan, scatter
+__global__ __launch_bounds__(64) void k_bucket(int32_t ncl, int32_t nwin, int32_t wshift,
+                                               const int32_t *first, const int64_t *rp,
+                                               const int32_t *ci, const int32_t *rcf,
+                                               const int64_t *cp, const uint32_t *cf,
+                                               const int64_t *pbase, int64_t *seg, uint2 *bucket) {
+    __shared__ int32_t cnt[64];
+    __shared__ int32_t pos[64];
+    const int lane = threadIdx.x;
+    for (int32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
+        const int32_t r = first[c];
+        const int64_t base = pbase[c];
+        cnt[lane] = 0;
+        __syncthreads();
+        int64_t b = 0, e = 0;
+        const uint32_t *rc = nullptr;
+        if (r != INT32_MAX) {
+            b = rp[r] + 1;  // skip the lead
+            e = rp[r + 1];
+            rc = cf + cp[rcf[r]] - rp[r];
+        }
+        for (int64_t k = b + lane; k < e; k += 64) atomicAdd(&cnt[ci[k] >> wshift], 1);
+        __syncthreads();
+        const int v = lane < nwin ? cnt[lane] : 0;
+        const int incl = wave_incl_scan(v);
+        pos[lane] = incl - v;
+        if (lane <= nwin) seg[(int64_t)lane * ncl + c] = base + (incl - v);
+        __syncthreads();
+        for (int64_t k = b + lane; k < e; k += 64) {
+            const int32_t col = ci[k];
+            const int w = col >> wshift;
+            const int at = atomicAdd(&pos[w], 1);
+            bucket[base + at] = make_uint2((uint32_t)col, rc[k]);
+        }
+        __syncthreads();
+    }
+}
+
+int prep_buckets(gbgpu_mat *m) {
+    gbgpu_ctx *ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    if (m->seg) return GBGPU_OK;
+    // smallest window (2048 .. 8192 columns) that keeps nwin <= 63
+    int wshift = 11;
+    while (((int64_t)m->nc + (1 << wshift) - 1) >> wshift > 63 && wshift < 13) ++wshift;
+    const int nwin = (int)(((int64_t)m->nc + (1 << wshift) - 1) >> wshift);
+    if (nwin > 63) return set_error(ctx, GBGPU_ERANGE, "too many columns for the sparse pass");
+    m->wshift = wshift;
+    m->nwin = nwin;
+    const int32_t ncl = m->ncl;
+    int rc;
+    int64_t *plen = nullptr, *pbase = nullptr;
+    if ((rc = dalloc(ctx, &plen, (size_t)ncl)) || (rc = dalloc(ctx, &pbase, (size_t)ncl))) {
+        dfree(plen); return rc;
+    }
+    if ((rc = dalloc(ctx, &m->pbits, (size_t)(ncl + 63) / 32 + 2))) { dfree(plen); dfree(pbase); return rc; }
+    GBGPU_HIP(ctx, hipMemsetAsync(m->pbits, 0, ((size_t)(ncl + 63) / 32 + 2) * 4, s));
+    if (ncl > 0) {
+        // round the grid up to whole waves so k_pivlen's ballots cover 64 columns
+        const int blocks = (int)((ncl + 255) / 256);
+        k_pivlen<<<blocks, 256, 0, s>>>(ncl, m->first, m->rp, plen, m->pbits);
+    }
+    int64_t total = 0;
+    if ((rc = exclusive_scan_i64(ctx, plen, pbase, ncl, &total))) { dfree(plen); dfree(pbase); return rc; }
+    m->nbucket = total;
+    if ((rc = dalloc(ctx, &m->seg, (size_t)(nwin + 1) * (size_t)(ncl > 0 ? ncl : 1))) ||
+        (rc = dalloc(ctx, &m->bucket, (size_t)total))) {
+        dfree(plen); dfree(pbase); return rc;
+    }
+    if (ncl > 0) {
+        const int g = ncl < 65536 * 4 ? ncl : 65536 * 4;
+        k_bucket<<<g, 64, 0, s>>>(ncl, nwin, wshift, m->first, m->rp, m->ci, m->rcf, m->cp, m->cf,
+                                  pbase, m->seg, m->bucket);
+    }
+    GBGPU_HIP(ctx, hipGetLastError());
+    GBGPU_HIP(ctx, hipStreamSynchronize(s));
+    dfree(plen);
+    dfree(pbase);
+    return GBGPU_OK;
+}
+
+// ------------------------------------------------------ sparse pass
+
+struct SpArgs {
+    int32_t nc, ncl, ncr, nrl, nwin;
+    const int64_t *rp;
+    const int32_t *ci;
+    const int32_t *rcf;
+    const int64_t *cp;
+    const uint32_t *cf;
+    const int32_t *upiv;
+    const uint32_t *pbits;
+    const int64_t *seg;
+    const uint2 *bucket;
+    uint32_t *xs;       // gridDim.x x ncl multipliers
+    uint32_t *dprime;   // nrl x ncr
+    int32_t *dnz;       // nrl
+    int32_t *queue;     // work counter
+    Mod md;
+};
+
+// pivot entries staged per 64-column group (6 KB of LDS)
+constexpr int STAGE = 768;
+
+template <int WSHIFT, bool BIG>
+__global__ __launch_bounds__(64) void k_sparse_plain(SpArgs a) {
+    constexpr int W = 1 << WSHIFT;
+    __shared__ uint64_t acc[W];
+    __shared__ uint32_t pb[W / 32];
+    __shared__ int64_t sh_base[64];
+    __shared__ int32_t sh_start[64];
+    __shared__ uint32_t sh_x[64];
+    __shared__ int32_t sh_q;
+    __shared__ uint2 stage[STAGE];
+    const int lane = threadIdx.x;
+    const uint32_t p = a.md.p;
+    uint32_t *xs = a.xs + (size_t)blockIdx.x * (size_t)(a.ncl > 0 ? a.ncl : 1);
+
+    for (;;) {
+        if (lane == 0) sh_q = atomicAdd(a.queue, 1);
+        __syncthreads();
+        int q = sh_q;
+        __syncthreads();
+        if (q >= a.nrl) break;
+        q = a.nrl - 1 - q;  // upiv ends with the smallest leads: longest rows first
+        const int32_t row = a.upiv[q];
+        const int64_t rb = a.rp[row], re = a.rp[row + 1];
+        const uint32_t *rcf = a.cf + a.cp[a.rcf[row]] - rb;
+        const int32_t lead = a.ci[rb];
+        for (int32_t j = lead + lane; j < a.ncl; j += 64) xs[j] = 0;
+        bool nzrow = false;
+
+        for (int w = lead >> WSHIFT; w < a.nwin; ++w) {
+            const int32_t w0 = w << WSHIFT;
+            const int32_t w1 = min(w0 + W, a.nc);
+            for (int c = lane; c < W; c += 64) acc[c] = 0;
+            // has-pivot bits of the window's left columns
+            const int32_t lend = min(w1, a.ncl);
+            for (int k = lane; k < W / 32; k += 64)
+                pb[k] = (w0 + 32 * k < lend) ? a.pbits[(w0 >> 5) + k] : 0u;
+            __syncthreads();
+            // 1. the row's own entries in this window
+            for (int64_t k = rb + lane; k < re; k += 64) {
+                const int32_t c = a.ci[k];
+                if (c >= w0 && c < w1) acc[c - w0] = rcf[k];
+            }
+            __syncthreads();
+            // 2. deferred contributions of pivots applied in earlier windows
+            const int32_t jl = min(w0, a.ncl);
+            for (int32_t jb = lead; jb < jl; jb += 64) {
+                const int32_t j = jb + lane;
+                uint32_t x = 0;
+                int64_t s = 0;
+                int len = 0;
+                if (j < jl) {
+                    x = xs[j];
+                    if (x) {
+                        s = a.seg[(int64_t)w * a.ncl + j];
+                        len = (int)(a.seg[(int64_t)(w + 1) * a.ncl + j] - s);
+                    }
+                }
+                const int incl = wave_incl_scan(len);
+                const int total = __shfl(incl, 63, 64);
+                if (total == 0) continue;
+                sh_start[lane] = incl - len;
+                sh_base[lane] = s;
+                sh_x[lane] = x;
+                __syncthreads();
+                for (int f = lane; f < total; f += 64) {
+                    int lo = 0, hi = 63;  // last t with sh_start[t] <= f
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (sh_start[mid] <= f) lo = mid; else hi = mid - 1;
+                    }
+                    const uint2 e = a.bucket[sh_base[lo] + (f - sh_start[lo])];
+                    atomicAdd(&acc[e.x - w0], addend<BIG>(sh_x[lo], e.y, a.md));
+                }
+                __syncthreads();
+            }
+            // 3. chain over the window's pivot columns, one 64-column group
+            //    at a time; the group's in-window pivot entries are staged
+            //    in LDS first (one coalesced sweep) so that a chain step
+            //    touches LDS only
+            for (int32_t g = max(w0, lead) & ~63; g < lend; g += 64) {
+                const int32_t c = g + lane;
+                int64_t s = 0;
+                int len = 0;
+                if (c < lend && c >= lead) {
+                    s = a.seg[(int64_t)w * a.ncl + c];
+                    len = (int)(a.seg[(int64_t)(w + 1) * a.ncl + c] - s);
+                }
+                const int incl = wave_incl_scan(len);
+                const int total = __shfl(incl, 63, 64);
+                const bool staged = total <= STAGE;
+                sh_start[lane] = incl - len;
+                sh_base[lane] = s;
+                __syncthreads();
+                if (staged) {
+                    for (int f = lane; f < total; f += 64) {
+                        int lo = 0, hi = 63;
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (sh_start[mid] <= f) lo = mid; else hi = mid - 1;
+                        }
+                        stage[f] = a.bucket[sh_base[lo] + (f - sh_start[lo])];
+                    }
+                    __syncthreads();
+                }
+                int32_t i = max(g, lead);
+                const int32_t gend = min(g + 64, lend);
+                while (i < gend) {
+                    const int32_t cc = i + lane;
+                    uint32_t r = 0;
+                    if (cc < gend) {
+                        const uint64_t v = acc[cc - w0];
+                        if (v) r = a.md.reduce(v);
+                        if (r && !((pb[(cc - w0) >> 5] >> ((cc - w0) & 31)) & 1u)) r = 0;
+                    }
+                    const uint64_t mask = __ballot(r != 0);
+                    if (mask == 0) break;
+                    const int t = __ffsll((unsigned long long)mask) - 1;
+                    const int32_t col = i + t;
+                    const uint32_t x = p - __shfl(r, t, 64);
+                    const int tg = col - g;
+                    const int st = sh_start[tg];
+                    const int ln = (tg < 63 ? sh_start[tg + 1] : total) - st;
+                    if (lane == 0) { xs[col] = x; acc[col - w0] = 0; }
+                    if (staged) {
+                        for (int k = lane; k < ln; k += 64) {
+                            const uint2 ent = stage[st + k];
+                            acc[ent.x - w0] += addend<BIG>(x, ent.y, a.md);
+                        }
+                    } else {
+                        const int64_t sb = sh_base[tg];
+                        for (int k = lane; k < ln; k += 64) {
+                            const uint2 ent = a.bucket[sb + k];
+                            acc[ent.x - w0] += addend<BIG>(x, ent.y, a.md);
+                        }
+                    }
+                    __syncthreads();
+                    i = col + 1;
+                }
+                __syncthreads();
+            }
+            // 4. right-hand columns of this window
+            if (w1 > a.ncl) {
+                const int32_t cs = max(w0, a.ncl);
+                uint32_t *out = a.dprime + (size_t)q * (size_t)a.ncr - a.ncl;
+                for (int32_t c = cs + lane; c < w1; c += 64) {
+                    const uint32_t r = a.md.reduce(acc[c - w0]);
+                    out[c] = r;
+                    nzrow |= r != 0;
+                }
+            }
+            __syncthreads();
+        }
+        const bool anynz = __any(nzrow);
+        if (lane == 0) a.dnz[q] = anynz ? 1 : 0;
+    }
+}
+
+__global__ void k_compact(int32_t n, const int32_t *flag, const int32_t *rank, int32_t *list) {
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (flag[i]) list[rank[i]] = i;
+}
+
+template <int WS>
+static void launch_sparse(const SpArgs &a, bool big, int grid, hipStream_t s) {
+    if (big) k_sparse_plain<WS, true><<<grid, 64, 0, s>>>(a);
+    else k_sparse_plain<WS, false><<<grid, 64, 0, s>>>(a);
+}
+
+int run_sparse_pass(gbgpu_mat *m) {
+    gbgpu_ctx *ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    int rc;
+    if ((rc = prep_split(m, m->ncl))) return rc;
+    if ((rc = prep_buckets(m))) return rc;
+    const int32_t ncr = m->nc - m->ncl;
+    dfree(m->dprime); dfree(m->dnz); dfree(m->dlist);
+    m->nd = 0;
+    if ((rc = dalloc(ctx, &m->dprime, (size_t)m->nrl * (size_t)ncr)) ||
+        (rc = dalloc(ctx, &m->dnz, (size_t)m->nrl)))
+        return rc;
+    // waves in flight: LDS of one wave is W*8 bytes + ~1.5 KB, 160 KB per CU
+    const int W = 1 << m->wshift;
+    const int per_cu = (160 * 1024) / (W * 8 + STAGE * 8 + 2048);
+    int grid = ctx->num_cu * (per_cu > 0 ? per_cu : 1);
+    if (grid > m->nrl) grid = m->nrl > 0 ? m->nrl : 1;
+    uint32_t *xs = nullptr;
+    int32_t *queue = nullptr;
+    if ((rc = dalloc(ctx, &xs, (size_t)grid * (size_t)(m->ncl > 0 ? m->ncl : 1)))) return rc;
+    if ((rc = dalloc(ctx, &queue, 1))) { dfree(xs); return rc; }
+    GBGPU_HIP(ctx, hipMemsetAsync(queue, 0, sizeof(int32_t), s));
+    SpArgs a;
+    a.nc = m->nc; a.ncl = m->ncl; a.ncr = ncr; a.nrl = m->nrl; a.nwin = m->nwin;
+    a.rp = m->rp; a.ci = m->ci; a.rcf = m->rcf; a.cp = m->cp; a.cf = m->cf;
+    a.upiv = m->upiv; a.pbits = m->pbits; a.seg = m->seg; a.bucket = m->bucket;
+    a.xs = xs; a.dprime = m->dprime; a.dnz = m->dnz; a.queue = queue;
+    a.md = Mod::make(m->p);
+    const bool big = m->p >= (1u << 17);
+    GBGPU_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+    if (m->nrl > 0) {
+        switch (m->wshift) {
+            case 11: launch_sparse<11>(a, big, grid, s); break;
+            case 12: launch_sparse<12>(a, big, grid, s); break;
+            default: launch_sparse<13>(a, big, grid, s); break;
+        }
+    }
+    GBGPU_HIP(ctx, hipGetLastError());
+    GBGPU_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+    // compact the nonzero rows, keeping the order (la_ff_32.c:1166-1171)
+    int32_t *rank = nullptr;
+    if ((rc = dalloc(ctx, &rank, (size_t)m->nrl))) { dfree(xs); dfree(queue); return rc; }
+    int64_t nd = 0;
+    if ((rc = exclusive_scan_i32(ctx, m->dnz, rank, m->nrl, &nd))) {
+        dfree(xs); dfree(queue); dfree(rank); return rc;
+    }
+    m->nd = (int32_t)nd;
+    if ((rc = dalloc(ctx, &m->dlist, (size_t)nd))) { dfree(xs); dfree(queue); dfree(rank); return rc; }
+    if (m->nrl > 0) k_compact<<<grid_for(m->nrl, 256), 256, 0, s>>>(m->nrl, m->dnz, rank, m->dlist);
+    GBGPU_HIP(ctx, hipGetLastError());
+    GBGPU_HIP(ctx, hipStreamSynchronize(s));
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->phase_ms[0] = ms;
+    dfree(xs);
+    dfree(queue);
+    dfree(rank);
+    return GBGPU_OK;
+}
+
+// ----------------------------------------------------------- scans
+
+int exclusive_scan_i64(gbgpu_ctx *ctx, const int64_t *in, int64_t *out, int64_t n, int64_t *total) {
+    *total = 0;
+    if (n <= 0) return GBGPU_OK;
+    hipStream_t s = ctx->stream;
+    int64_t last_in = 0;
+    GBGPU_HIP(ctx, hipMemcpyAsync(&last_in, in + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    GBGPU_HIP(ctx, hipStreamSynchronize(s));
+    size_t tmp = 0;
+    GBGPU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, s));
+    void *d_tmp = nullptr;
+    int rc;
+    if ((rc = dalloc(ctx, (char **)&d_tmp, tmp))) return rc;
+    GBGPU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp, in, out, (int)n, s));
+    int64_t last_out = 0;
+    GBGPU_HIP(ctx, hipMemcpyAsync(&last_out, out + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    GBGPU_HIP(ctx, hipStreamSynchronize(s));
+    (void)hipFree(d_tmp);
+    *total = last_out + last_in;
+    return GBGPU_OK;
+}
+
+int exclusive_scan_i32(gbgpu_ctx *ctx, const int32_t *in, int32_t *out, int64_t n, int64_t *total) {
+    *total = 0;
+    if (n <= 0) return GBGPU_OK;
+    hipStream_t s = ctx->stream;
+    int32_t last_in = 0;
+    GBGPU_HIP(ctx, hipMemcpyAsync(&last_in, in + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GBGPU_HIP(ctx, hipStreamSynchronize(s));
+    size_t tmp = 0;
+    GBGPU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, s));
+    void *d_tmp = nullptr;
+    int rc;
+    if ((rc = dalloc(ctx, (char **)&d_tmp, tmp))) return rc;
+    GBGPU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp, in, out, (int)n, s));
+    int32_t last_out = 0;
+    GBGPU_HIP(ctx, hipMemcpyAsync(&last_out, out + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GBGPU_HIP(ctx, hipStreamSynchronize(s));
+    (void)hipFree(d_tmp);
+    *total = (int64_t)last_out + last_in;
+    return GBGPU_OK;
+}
+
+}  // namespace gbgpu

@@ -1,0 +1,251 @@
+"""ctypes binding of libgbgpu.so (the C-ABI declared in include/gbgpu.h).
+
+There is no CPU fallback: every entry point runs the HIP kernels, and
+loading fails loudly when the library or the GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from .matrix import F4Matrix, SparseRows
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgbgpu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "gbgpu.h")
+
+FORM_PLAIN = 0
+FORM_MFMA = 1
+
+_lib = None
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+class _Csr(C.Structure):
+    _fields_ = [("nr", C.c_int32), ("nc", C.c_int32), ("ncl", C.c_int32), ("ncf", C.c_int32),
+                ("p", C.c_uint32), ("rp", C.c_void_p), ("ci", C.c_void_p), ("rcf", C.c_void_p),
+                ("cp", C.c_void_p), ("cf", C.c_void_p)]
+
+
+class _Rows(C.Structure):
+    _fields_ = [("np", C.c_int32), ("rp", C.POINTER(C.c_int64)),
+                ("ci", C.POINTER(C.c_int32)), ("cf", C.POINTER(C.c_uint32))]
+
+
+def declared_symbols() -> list[str]:
+    """Every function declared in include/gbgpu.h."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gbgpu_[a-z0-9_]+)\s*\(", txt)))
+
+
+def lib():
+    """Load libgbgpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GpuError(f"{LIB_PATH} is missing: build it with `python -m gb_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, u32, i64 = C.c_void_p, C.c_int32, C.c_uint32, C.c_int64
+    pi32 = C.POINTER(C.c_int32)
+    sig = {
+        "gbgpu_init": ([C.c_int, C.POINTER(vp)], C.c_int),
+        "gbgpu_finalize": ([vp], None),
+        "gbgpu_set_stream": ([vp, vp], C.c_int),
+        "gbgpu_last_error": ([vp], C.c_char_p),
+        "gbgpu_arith_bits": ([u32], C.c_int),
+        "gbgpu_phase_ms": ([vp, C.c_int], C.c_double),
+        "gbgpu_mat_upload": ([vp, C.POINTER(_Csr), C.POINTER(vp)], C.c_int),
+        "gbgpu_mat_free": ([vp], None),
+        "gbgpu_sparse_pass": ([vp, vp, pi32], C.c_int),
+        "gbgpu_sparse_pass_download": ([vp, vp, vp, vp], C.c_int),
+        "gbgpu_la_sparse_dense": ([vp, vp, C.c_int, pi32], C.c_int),
+        "gbgpu_la_sparse": ([vp, vp, C.c_int, pi32], C.c_int),
+        "gbgpu_interreduce": ([vp, vp, C.c_int, pi32], C.c_int),
+        "gbgpu_rows_download": ([vp, vp, C.POINTER(_Rows)], C.c_int),
+        "gbgpu_rows_free": ([C.POINTER(_Rows)], None),
+        "gbgpu_la_ff": ([vp, C.POINTER(_Csr), C.c_int, C.c_int, C.POINTER(_Rows)], C.c_int),
+        "gbgpu_dense_upload": ([vp, i32, i32, u32, vp, C.POINTER(vp)], C.c_int),
+        "gbgpu_dense_free": ([vp], None),
+        "gbgpu_dense_rref": ([vp, vp, C.c_int, pi32], C.c_int),
+        "gbgpu_dense_download": ([vp, vp, vp, vp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _ = i64
+    _lib = L
+    return L
+
+
+def lib_checked_gpu() -> "Context":
+    """A context on device 0; raises when no GPU is usable."""
+    return Context(0)
+
+
+def _csr(m: F4Matrix):
+    arrs = (np.ascontiguousarray(m.rp, np.int64), np.ascontiguousarray(m.ci, np.int32),
+            np.ascontiguousarray(m.rcf, np.int32), np.ascontiguousarray(m.cp, np.int64),
+            np.ascontiguousarray(m.cf, np.uint32))
+    s = _Csr(m.nr, m.nc, m.ncl, len(m.cp) - 1, m.p, *(a.ctypes.data for a in arrs))
+    return s, arrs
+
+
+def _take_rows(r: _Rows) -> SparseRows:
+    n = r.np
+    rp = np.ctypeslib.as_array(r.rp, shape=(n + 1,)).copy()
+    nnz = int(rp[-1])
+    if nnz:
+        ci = np.ctypeslib.as_array(r.ci, shape=(nnz,)).copy()
+        cf = np.ctypeslib.as_array(r.cf, shape=(nnz,)).copy()
+    else:
+        ci = np.zeros(0, np.int32)
+        cf = np.zeros(0, np.uint32)
+    lib().gbgpu_rows_free(C.byref(r))
+    return SparseRows(rp=rp, ci=ci, cf=cf)
+
+
+class Context:
+    """One device, one stream (gbgpu_ctx)."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.gbgpu_init(device, C.byref(h))
+        if rc != 0:
+            raise GpuError(f"gbgpu_init({device}) failed with {rc}: no usable GPU")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().gbgpu_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = lib().gbgpu_last_error(self.h).decode()
+            raise GpuError(f"{what} failed ({rc}): {msg}")
+
+    def set_stream(self, stream_ptr: int):
+        self._check(lib().gbgpu_set_stream(self.h, C.c_void_p(stream_ptr)), "set_stream")
+
+    def phase_ms(self, phase: int) -> float:
+        return float(lib().gbgpu_phase_ms(self.h, phase))
+
+    def upload(self, m: F4Matrix) -> "DeviceMatrix":
+        s, keep = _csr(m)
+        h = C.c_void_p()
+        self._check(lib().gbgpu_mat_upload(self.h, C.byref(s), C.byref(h)), "upload")
+        return DeviceMatrix(self, h, m)
+
+    def la_ff(self, m: F4Matrix, la_option: int = 1, form: int = FORM_PLAIN) -> SparseRows:
+        s, keep = _csr(m)
+        r = _Rows()
+        self._check(lib().gbgpu_la_ff(self.h, C.byref(s), la_option, form, C.byref(r)), "la_ff")
+        return _take_rows(r)
+
+    def dense(self, a: np.ndarray, p: int) -> "DeviceDense":
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        h = C.c_void_p()
+        self._check(lib().gbgpu_dense_upload(self.h, a.shape[0], a.shape[1], p,
+                                             a.ctypes.data, C.byref(h)), "dense upload")
+        return DeviceDense(self, h, a.shape, p)
+
+
+class DeviceMatrix:
+    """An F4 matrix resident on the device (gbgpu_mat)."""
+
+    def __init__(self, ctx: Context, h, m: F4Matrix):
+        self.ctx, self.h, self.m = ctx, h, m
+
+    def free(self):
+        if self.h:
+            lib().gbgpu_mat_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def sparse_pass(self) -> int:
+        n = C.c_int32(0)
+        self.ctx._check(lib().gbgpu_sparse_pass(self.ctx.h, self.h, C.byref(n)), "sparse pass")
+        return int(n.value)
+
+    def sparse_pass_result(self, nd: int):
+        ncr = self.m.ncr
+        dense = np.zeros((max(nd, 1), max(ncr, 1)), dtype=np.uint32)
+        src = np.zeros(max(nd, 1), dtype=np.int32)
+        self.ctx._check(lib().gbgpu_sparse_pass_download(self.ctx.h, self.h, dense.ctypes.data,
+                                                         src.ctypes.data), "download")
+        return dense[:nd, :ncr], src[:nd]
+
+    def la_sparse_dense(self, form: int = FORM_PLAIN) -> int:
+        n = C.c_int32(0)
+        self.ctx._check(lib().gbgpu_la_sparse_dense(self.ctx.h, self.h, form, C.byref(n)), "la 1")
+        return int(n.value)
+
+    def la_sparse(self, form: int = FORM_PLAIN) -> int:
+        n = C.c_int32(0)
+        self.ctx._check(lib().gbgpu_la_sparse(self.ctx.h, self.h, form, C.byref(n)), "la 2")
+        return int(n.value)
+
+    def interreduce(self, form: int = FORM_PLAIN) -> int:
+        n = C.c_int32(0)
+        self.ctx._check(lib().gbgpu_interreduce(self.ctx.h, self.h, form, C.byref(n)), "interreduce")
+        return int(n.value)
+
+    def rows(self) -> SparseRows:
+        r = _Rows()
+        self.ctx._check(lib().gbgpu_rows_download(self.ctx.h, self.h, C.byref(r)), "rows download")
+        return _take_rows(r)
+
+
+class DeviceDense:
+    """A dense block resident on the device (gbgpu_dense)."""
+
+    def __init__(self, ctx: Context, h, shape, p: int):
+        self.ctx, self.h, self.shape, self.p = ctx, h, shape, p
+        self.rank = None
+
+    def free(self):
+        if self.h:
+            lib().gbgpu_dense_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def rref(self, form: int = FORM_PLAIN) -> int:
+        r = C.c_int32(0)
+        self.ctx._check(lib().gbgpu_dense_rref(self.ctx.h, self.h, form, C.byref(r)), "dense rref")
+        self.rank = int(r.value)
+        return self.rank
+
+    def result(self):
+        n = self.shape[1]
+        rk = self.rank or 0
+        rows = np.zeros((max(rk, 1), max(n, 1)), dtype=np.uint32)
+        lead = np.zeros(max(rk, 1), dtype=np.int32)
+        self.ctx._check(lib().gbgpu_dense_download(self.ctx.h, self.h, rows.ctypes.data,
+                                                   lead.ctypes.data), "dense download")
+        return rows[:rk, :n], lead[:rk]

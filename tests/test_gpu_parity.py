@@ -1,0 +1,202 @@
+"""GPU parity: every LA entry point of libgbgpu against the CPU oracle, bit
+for bit, on seeded inputs; full-size shapes through size-independent
+properties (sampled rows, Freivalds projections)."""
+import numpy as np
+import pytest
+
+from gb_amd import synth
+from gb_amd.matrix import F4Matrix, from_dense_rows, permute_rows, sort_rows
+from gb_amd.native import FORM_MFMA, FORM_PLAIN, GpuError
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+PRIMES = [127, 251, 32003, 65521, 2147483629, 2147483647]
+FORMS = [FORM_PLAIN, FORM_MFMA]
+
+
+def _f4(nr, nc, npiv, p, seed, **kw):
+    return synth.f4_shaped(nr, nc, npiv, p, seed=seed, **kw)
+
+
+@pytest.mark.parametrize("p", PRIMES)
+def test_sparse_pass_small(gpu_lib, p):
+    m = _f4(300, 500, 180, p, seed=p % 1000, piv_len=(3, 40), piv_mu=np.log(12), red_nnz=25)
+    d = gpu_lib.upload(m)
+    nd = d.sparse_pass()
+    got, src = d.sparse_pass_result(nd)
+    exp, esrc = oracle.sparse_pass(m)
+    assert nd == len(exp)
+    assert np.array_equal(src, esrc)
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("p", [251, 65521, 2147483647])
+def test_sparse_pass_many_windows(gpu_lib, p):
+    # 24000 columns: 12 column windows of the LDS accumulator
+    m = _f4(900, 24000, 600, p, seed=5, piv_len=(20, 200), piv_mu=np.log(80), red_nnz=120)
+    d = gpu_lib.upload(m)
+    nd = d.sparse_pass()
+    got, src = d.sparse_pass_result(nd)
+    exp, esrc = oracle.sparse_pass(m, nthreads=8)
+    assert np.array_equal(src, esrc)
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("p", PRIMES)
+@pytest.mark.parametrize("form", FORMS)
+def test_la_option1_medium(gpu_lib, p, form):
+    m = _f4(2000, 3000, 1200, p, seed=17, piv_len=(10, 120), piv_mu=np.log(40), red_nnz=40)
+    exp = oracle.la_sparse_dense(m, nthreads=8)
+    d = gpu_lib.upload(m)
+    n = d.la_sparse_dense(form)
+    got = d.rows()
+    assert n == exp.np
+    assert got.equals(exp)
+    # la_option 2 has the same canonical output
+    assert d.la_sparse(form) == exp.np
+    assert d.rows().equals(exp)
+
+
+@pytest.mark.parametrize("la_option", [1, 2, 42, 43, 44, 0, 7])
+def test_la_ff_one_call_all_options(gpu_lib, la_option):
+    m = _f4(400, 700, 250, 65521, seed=3, piv_len=(5, 50), piv_mu=np.log(15), red_nnz=20)
+    exp = oracle.la_sparse(m)
+    got = gpu_lib.la_ff(m, la_option=la_option, form=FORM_MFMA)
+    assert got.equals(exp)
+
+
+@pytest.mark.parametrize("p", [251, 65521, 2147483629])
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("shape,rank", [((64, 100), None), ((200, 330), None), ((330, 200), None),
+                                        ((256, 520), 100), ((130, 70), 69)])
+def test_dense_rref(gpu_lib, p, form, shape, rank):
+    a = synth.dense_block(shape[0], shape[1], p, seed=shape[0] + shape[1], rank=rank)
+    exp, elead = oracle.dense_rref(a, p, nthreads=8)
+    d = gpu_lib.dense(a, p)
+    r = d.rref(form)
+    got, lead = d.result()
+    assert r == len(exp)
+    assert np.array_equal(lead, elead)
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("p", [101, 65521, 2147483647])
+def test_interreduce(gpu_lib, p):
+    rng = np.random.default_rng(p % 97)
+    nc = 300
+    leads = np.sort(rng.choice(nc - 10, size=80, replace=False))
+    rows = []
+    for l in leads:
+        r = np.zeros(nc, np.int64)
+        r[l] = 1
+        nz = rng.choice(np.arange(l + 1, nc), size=min(12, nc - l - 1), replace=False)
+        r[nz] = rng.integers(1, p, size=len(nz))
+        rows.append(r)
+    m = from_dense_rows(rows, nc, 0, p)
+    exp = oracle.interreduce(m)
+    for form in FORMS:
+        d = gpu_lib.upload(m)
+        assert d.interreduce(form) == exp.np
+        assert d.rows().equals(exp)
+
+
+def test_edge_no_rows_to_reduce(gpu_lib):
+    # every row is a known pivot: nothing to reduce, no new pivots
+    m = _f4(50, 90, 50, 65521, seed=1, piv_len=(2, 10), piv_mu=np.log(5))
+    d = gpu_lib.upload(m)
+    assert d.sparse_pass() == 0
+    assert d.la_sparse_dense(FORM_MFMA) == 0
+    assert d.rows().np == 0
+
+
+def test_edge_everything_reduces_to_zero(gpu_lib):
+    p = 32003
+    rng = np.random.default_rng(9)
+    nc, ncl = 60, 25
+    piv = []
+    for c in range(ncl):
+        r = np.zeros(nc, np.int64)
+        r[c] = 1
+        nz = rng.choice(np.arange(c + 1, nc), size=5, replace=False)
+        r[nz] = rng.integers(1, p, size=5)
+        piv.append(r)
+    red = []
+    for k in range(30):
+        coef = rng.integers(0, p, size=ncl)
+        coef[k % ncl] = 1
+        coef[: k % ncl] = 0
+        red.append(sum(int(coef[j]) * piv[j] for j in range(ncl)) % p)
+    m = sort_rows(from_dense_rows(piv + red, nc, ncl, p))
+    d = gpu_lib.upload(m)
+    assert d.sparse_pass() == 0
+    assert d.la_sparse_dense(FORM_PLAIN) == 0
+    assert oracle.la_sparse_dense(m).np == 0
+
+
+def test_contract_violation_is_an_error(gpu_lib):
+    m = from_dense_rows([[1, 2, 0, 0], [0, 0, 1, 3]], 4, 1, 101)
+    d = gpu_lib.upload(m)
+    with pytest.raises(GpuError):
+        d.la_sparse_dense(FORM_PLAIN)
+    bad = F4Matrix(nr=1, nc=4, ncl=1, p=101, rp=np.array([0, 2], np.int64),
+                   ci=np.array([0, 9], np.int32), rcf=np.zeros(1, np.int32),
+                   cp=np.array([0, 2], np.int64), cf=np.array([1, 5], np.uint32))
+    with pytest.raises(GpuError):
+        gpu_lib.upload(bad)  # column out of range is rejected before any kernel
+
+
+def test_shared_coefficient_arrays(gpu_lib):
+    # rows that are multiples of one basis element share one coefficient
+    # array (hash.c:903-926): same result as with private copies
+    m = _f4(400, 600, 240, 65521, seed=21, piv_len=(4, 30), piv_mu=np.log(10), red_nnz=18)
+    lens = np.diff(m.rp)
+    # make rows with equal length share the first such row's array
+    first_of_len = {}
+    rcf = m.rcf.copy()
+    cf = m.cf.copy()
+    for i in range(m.nr):
+        L = int(lens[i])
+        if L in first_of_len:
+            rcf[i] = rcf[first_of_len[L]]
+        else:
+            first_of_len[L] = i
+    m2 = F4Matrix(nr=m.nr, nc=m.nc, ncl=m.ncl, p=m.p, rp=m.rp, ci=m.ci, rcf=rcf.astype(np.int32),
+                  cp=m.cp, cf=cf)
+    exp = oracle.la_sparse_dense(m2)
+    got = gpu_lib.la_ff(m2, la_option=1, form=FORM_PLAIN)
+    assert got.equals(exp)
+
+
+def _pivot_rows(m):
+    seen = set()
+    piv = []
+    for i, l in enumerate(m.leads()):
+        if int(l) not in seen:
+            seen.add(int(l))
+            piv.append(i)
+    return np.array(piv)
+
+
+@pytest.mark.parametrize("p", [65521, 2147483629])
+def test_full_size_sparse_pass_sampled(gpu_lib, p):
+    """20000 x 30000 F4 matrix (12000 known pivots): the GPU pass over all
+    8000 to-reduce rows, checked against the oracle on 48 sampled rows."""
+    m = _f4(20000, 30000, 12000, p, seed=2024)
+    d = gpu_lib.upload(m)
+    nd = d.sparse_pass()
+    got, src = d.sparse_pass_result(nd)
+    piv = _pivot_rows(m)
+    nonpiv = np.setdiff1d(np.arange(m.nr), piv)
+    rng = np.random.default_rng(0)
+    sample = np.sort(rng.choice(nonpiv, size=48, replace=False))
+    keep = np.sort(np.concatenate([piv, sample]))
+    sub = permute_rows(m, keep)
+    exp, esrc = oracle.sparse_pass(sub, nthreads=8)
+    by_src = {int(s): i for i, s in enumerate(src)}
+    for k, s in enumerate(esrc):
+        full = int(keep[s])
+        assert full in by_src, "oracle row nonzero but GPU row zero"
+        assert np.array_equal(got[by_src[full]], exp[k])
+    zero_sampled = set(int(x) for x in sample) - set(int(keep[s]) for s in esrc)
+    assert not (zero_sampled & set(by_src)), "GPU row nonzero but oracle row zero"
