@@ -183,6 +183,14 @@ int gbgpu_sparse_pass(gbgpu_ctx *ctx, gbgpu_mat *m, int32_t *nrows) {
     return GBGPU_OK;
 }
 
+int gbgpu_sparse_pass_work(gbgpu_ctx *ctx, gbgpu_mat *m, int32_t *nrows, uint64_t *work) {
+    if (!ctx || !m || m->ctx != ctx || !work) return GBGPU_EINVAL;
+    int rc = run_sparse_pass(m, work);
+    if (rc) return rc;
+    if (nrows) *nrows = m->nd;
+    return GBGPU_OK;
+}
+
 __global__ void k_src_rows(int32_t nd, const int32_t *dlist, const int32_t *upiv, int32_t *src) {
     for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nd; k += gridDim.x * blockDim.x)
         src[k] = upiv[dlist[k]];

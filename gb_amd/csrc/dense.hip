@@ -74,18 +74,23 @@ __global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t
     }
     const int bw = min(PB, n - c0);
     for (int i = tid; i < PB * PB; i += 256) { (&E[0][0])[i] = 0; (&T[0][0])[i] = 0; }
-    if (tid == 0) { sh_k = 0; sh_any = 0; }
+    // shared flags are written by every thread with the same value: no
+    // thread-specific branches inside the loops below (see sparse.hip)
+    sh_k = 0;
+    sh_any = 0;
     __syncthreads();
     int32_t cursor = 0;
     for (;;) {
-        if (tid == 0) sh_best = INT_MAX;
+        sh_best = INT_MAX;
         __syncthreads();
-        const int kk = sh_k;
+        // values every lane shares are made scalar: loops that contain
+        // cross-lane operations must have wave-uniform trip counts and exits
+        const int kk = __builtin_amdgcn_readfirstlane(sh_k);
         // parallel scan: first non-pivot row >= cursor not in span(E)
         for (int32_t r = cursor + wv; r < m; r += 4) {
-            if (r > *(volatile int32_t *)&sh_best) break;
-            if (rowpiv[r] >= 0) continue;
-            if (lane == 0) sh_any = 1;
+            if (r > __builtin_amdgcn_readfirstlane(*(volatile int32_t *)&sh_best)) break;
+            if (__builtin_amdgcn_readfirstlane(rowpiv[r]) >= 0) continue;
+            sh_any = 1;
             const uint32_t v = lane < bw ? a[(int64_t)r * ld + c0 + lane] : 0u;
             uint64_t s = 0;
             for (int t = 0; t < kk; ++t) {
@@ -94,12 +99,12 @@ __global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t
             }
             const uint32_t vr = sub_mod(v, md.reduce(s), p);
             if (__any(vr != 0)) {
-                if (lane == 0) atomicMin(&sh_best, r);
+                atomicMin(&sh_best, r);
                 break;
             }
         }
         __syncthreads();
-        const int32_t rs = sh_best;
+        const int32_t rs = __builtin_amdgcn_readfirstlane(sh_best);
         if (rs == INT_MAX) break;
         if (wv == 0) {
             const uint32_t v = lane < bw ? a[(int64_t)rs * ld + c0 + lane] : 0u;
@@ -111,23 +116,22 @@ __global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t
             }
             const uint32_t vr = sub_mod(v, md.reduce(s), p);
             const uint64_t nzm = __ballot(vr != 0);
-            const int cs = __ffsll((unsigned long long)nzm) - 1;
+            const int cs = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)nzm) - 1);
             const uint32_t inv = inv_mod(__shfl(vr, cs, 64), p);
             E[kk][lane] = md.mul(vr, inv);
             // T row of the new pivot: (e_new - sum_t ct T[t]) * inv
             uint32_t tn = sub_mod(0, md.reduce(ts), p);
             if (lane == kk) tn = 1;
             T[kk][lane] = md.mul(tn, inv);
-            if (lane == 0) {
-                ecol[kk] = c0 + cs;
-                erow[kk] = rs;
-                sh_cstar = cs;
-            }
+            ecol[kk] = c0 + cs;
+            erow[kk] = rs;
+            sh_cstar = cs;
         }
         __syncthreads();
         // keep E reduced: eliminate the new pivot column from older rows
         // (factors read into F first: the j == cs thread overwrites E[t][cs])
-        if (tid < kk) F[tid] = E[tid][sh_cstar];
+        const int cstar = __builtin_amdgcn_readfirstlane(sh_cstar);
+        if (tid < kk) F[tid] = E[tid][cstar];
         __syncthreads();
         for (int idx = tid; idx < kk * PB; idx += 256) {
             const int t = idx >> 6, j = idx & 63;
@@ -137,12 +141,12 @@ __global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t
             T[t][j] = sub_mod(T[t][j], md.mul(f, T[kk][j]), p);
         }
         __syncthreads();
-        if (tid == 0) sh_k = kk + 1;
+        sh_k = kk + 1;
         cursor = rs + 1;
         __syncthreads();
-        if (sh_k == bw) break;
+        if (__builtin_amdgcn_readfirstlane(sh_k) == bw) break;
     }
-    const int kk = sh_k;
+    const int kk = __builtin_amdgcn_readfirstlane(sh_k);
     if (tid < kk) {
         pan->erow[tid] = erow[tid];
         pan->ecol[tid] = ecol[tid];
@@ -263,7 +267,8 @@ __global__ __launch_bounds__(256) void k_update_plain(uint32_t *a, int64_t ld, i
 // Block: 4 waves, 128 rows x 64 columns; each wave a 32 x 64 tile held as
 // 2 x 4 accumulators of v_mfma_f64_16x16x4f64 (A 16x4, B 4x16, C 16x16).
 // Operand layout per lane l: A[l % 16][l / 16], B[l / 16][l % 16]; result
-// register e of lane l is C[4 (l / 16) + e][l % 16].
+// register e of lane l is C[4 e + l / 16][l % 16] (measured on gfx950 with
+// an in-bounds probe of the builtin).
 // Exactness: operands are integers below 2^17 (p < 2^17) or 16-bit halves
 // of residues (p >= 2^17), so every 64-term dot product is an integer below
 // 2^40 (2^36 for halves) and FP64 represents it and every partial sum
@@ -372,7 +377,7 @@ __global__ __launch_bounds__(256) void k_update_mfma(uint32_t *a, int64_t ld, in
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int rloc = 32 * wv + 16 * i + 4 * (lane >> 4) + e;
+            const int rloc = 32 * wv + 16 * i + 4 * e + (lane >> 4);
             const int32_t r = rb + rloc;
             if (r >= m || !act[rloc]) continue;
 #pragma unroll

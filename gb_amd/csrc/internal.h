@@ -60,7 +60,9 @@ namespace gbgpu {
 // sparse.hip
 int prep_split(gbgpu_mat *m, int32_t lim);
 int prep_buckets(gbgpu_mat *m);
-int run_sparse_pass(gbgpu_mat *m);
+// work (optional, 2 values): pivots applied and pivot entries streamed,
+// counted by an instrumented launch
+int run_sparse_pass(gbgpu_mat *m, uint64_t *work = nullptr);
 
 // dense.hip: reduced echelon form of d (in place); form = GBGPU_FORM_*
 int dense_rref(gbgpu_dense *d, int form);

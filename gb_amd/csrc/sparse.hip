@@ -81,7 +81,6 @@ static int grid_for(int64_t n, int bs) {
 int prep_split(gbgpu_mat *m, int32_t lim) {
     gbgpu_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
-    if (m->split_lim == lim) return GBGPU_OK;
     dfree(m->first);
     dfree(m->upiv);
     int rc;
@@ -178,7 +177,9 @@ __global__ __launch_bounds__(64) void k_bucket(int32_t ncl, int32_t nwin, int32_
 int prep_buckets(gbgpu_mat *m) {
     gbgpu_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
-    if (m->seg) return GBGPU_OK;
+    dfree(m->seg);
+    dfree(m->bucket);
+    dfree(m->pbits);
     // smallest window (2048 .. 8192 columns) that keeps nwin <= 63
     int wshift = 11;
     while (((int64_t)m->nc + (1 << wshift) - 1) >> wshift > 63 && wshift < 13) ++wshift;
@@ -235,13 +236,14 @@ struct SpArgs {
     uint32_t *dprime;   // nrl x ncr
     int32_t *dnz;       // nrl
     int32_t *queue;     // work counter
+    unsigned long long *work;  // COUNT only: [pivots applied, pivot entries]
     Mod md;
 };
 
 // pivot entries staged per 64-column group (6 KB of LDS)
 constexpr int STAGE = 768;
 
-template <int WSHIFT, bool BIG>
+template <int WSHIFT, bool BIG, bool COUNT>
 __global__ __launch_bounds__(64) void k_sparse_plain(SpArgs a) {
     constexpr int W = 1 << WSHIFT;
     __shared__ uint64_t acc[W];
@@ -249,23 +251,28 @@ __global__ __launch_bounds__(64) void k_sparse_plain(SpArgs a) {
     __shared__ int64_t sh_base[64];
     __shared__ int32_t sh_start[64];
     __shared__ uint32_t sh_x[64];
-    __shared__ int32_t sh_q;
     __shared__ uint2 stage[STAGE];
     const int lane = threadIdx.x;
     const uint32_t p = a.md.p;
     uint32_t *xs = a.xs + (size_t)blockIdx.x * (size_t)(a.ncl > 0 ? a.ncl : 1);
 
     for (;;) {
-        if (lane == 0) sh_q = atomicAdd(a.queue, 1);
-        __syncthreads();
-        int q = sh_q;
-        __syncthreads();
+        // No lane-specific code may sit in this loop: with a lane-0-only
+        // atomic here the compiler gives lanes 1..63 a back edge that skips
+        // it (in a one-wave workgroup __syncthreads() is only a wave
+        // barrier), and they re-read a stale row index forever.  Every lane
+        // issues the atomic (lane 0 adds 1, the others 0) and lane 0's
+        // result is broadcast as a scalar.
+        const int q0 = atomicAdd(a.queue, lane == 0 ? 1 : 0);
+        int q = __builtin_amdgcn_readfirstlane(q0);
         if (q >= a.nrl) break;
         q = a.nrl - 1 - q;  // upiv ends with the smallest leads: longest rows first
-        const int32_t row = a.upiv[q];
+        // per-row values are the same in every lane: keep them scalar so that
+        // the loops below (which hold ballots and shuffles) stay uniform
+        const int32_t row = __builtin_amdgcn_readfirstlane(a.upiv[q]);
         const int64_t rb = a.rp[row], re = a.rp[row + 1];
         const uint32_t *rcf = a.cf + a.cp[a.rcf[row]] - rb;
-        const int32_t lead = a.ci[rb];
+        const int32_t lead = __builtin_amdgcn_readfirstlane(a.ci[rb]);
         for (int32_t j = lead + lane; j < a.ncl; j += 64) xs[j] = 0;
         bool nzrow = false;
 
@@ -299,7 +306,7 @@ __global__ __launch_bounds__(64) void k_sparse_plain(SpArgs a) {
                     }
                 }
                 const int incl = wave_incl_scan(len);
-                const int total = __shfl(incl, 63, 64);
+                const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
                 if (total == 0) continue;
                 sh_start[lane] = incl - len;
                 sh_base[lane] = s;
@@ -329,7 +336,7 @@ __global__ __launch_bounds__(64) void k_sparse_plain(SpArgs a) {
                     len = (int)(a.seg[(int64_t)(w + 1) * a.ncl + c] - s);
                 }
                 const int incl = wave_incl_scan(len);
-                const int total = __shfl(incl, 63, 64);
+                const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
                 const bool staged = total <= STAGE;
                 sh_start[lane] = incl - len;
                 sh_base[lane] = s;
@@ -363,7 +370,16 @@ __global__ __launch_bounds__(64) void k_sparse_plain(SpArgs a) {
                     const int tg = col - g;
                     const int st = sh_start[tg];
                     const int ln = (tg < 63 ? sh_start[tg + 1] : total) - st;
-                    if (lane == 0) { xs[col] = x; acc[col - w0] = 0; }
+                    // every lane stores the same value (no lane-0-only branch)
+                    xs[col] = x;
+                    acc[col - w0] = 0;
+                    if (COUNT) {
+                        // algorithmic work: the whole pivot row (lead included);
+                        // lane 0 contributes, the others add zero (no branch)
+                        const uint64_t len = (uint64_t)(a.seg[(int64_t)a.nwin * a.ncl + col] - a.seg[col]) + 1;
+                        atomicAdd(&a.work[0], lane == 0 ? 1ull : 0ull);
+                        atomicAdd(&a.work[1], lane == 0 ? (unsigned long long)len : 0ull);
+                    }
                     if (staged) {
                         for (int k = lane; k < ln; k += 64) {
                             const uint2 ent = stage[st + k];
@@ -394,7 +410,7 @@ __global__ __launch_bounds__(64) void k_sparse_plain(SpArgs a) {
             __syncthreads();
         }
         const bool anynz = __any(nzrow);
-        if (lane == 0) a.dnz[q] = anynz ? 1 : 0;
+        a.dnz[q] = anynz ? 1 : 0;  // same value from every lane
     }
 }
 
@@ -404,12 +420,17 @@ __global__ void k_compact(int32_t n, const int32_t *flag, const int32_t *rank, i
 }
 
 template <int WS>
-static void launch_sparse(const SpArgs &a, bool big, int grid, hipStream_t s) {
-    if (big) k_sparse_plain<WS, true><<<grid, 64, 0, s>>>(a);
-    else k_sparse_plain<WS, false><<<grid, 64, 0, s>>>(a);
+static void launch_sparse(const SpArgs &a, bool big, bool count, int grid, hipStream_t s) {
+    if (count) {
+        if (big) k_sparse_plain<WS, true, true><<<grid, 64, 0, s>>>(a);
+        else k_sparse_plain<WS, false, true><<<grid, 64, 0, s>>>(a);
+    } else {
+        if (big) k_sparse_plain<WS, true, false><<<grid, 64, 0, s>>>(a);
+        else k_sparse_plain<WS, false, false><<<grid, 64, 0, s>>>(a);
+    }
 }
 
-int run_sparse_pass(gbgpu_mat *m) {
+int run_sparse_pass(gbgpu_mat *m, uint64_t *work) {
     gbgpu_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
     int rc;
@@ -429,21 +450,22 @@ int run_sparse_pass(gbgpu_mat *m) {
     uint32_t *xs = nullptr;
     int32_t *queue = nullptr;
     if ((rc = dalloc(ctx, &xs, (size_t)grid * (size_t)(m->ncl > 0 ? m->ncl : 1)))) return rc;
-    if ((rc = dalloc(ctx, &queue, 1))) { dfree(xs); return rc; }
-    GBGPU_HIP(ctx, hipMemsetAsync(queue, 0, sizeof(int32_t), s));
+    if ((rc = dalloc(ctx, &queue, 6))) { dfree(xs); return rc; }
+    GBGPU_HIP(ctx, hipMemsetAsync(queue, 0, 6 * sizeof(int32_t), s));
     SpArgs a;
     a.nc = m->nc; a.ncl = m->ncl; a.ncr = ncr; a.nrl = m->nrl; a.nwin = m->nwin;
     a.rp = m->rp; a.ci = m->ci; a.rcf = m->rcf; a.cp = m->cp; a.cf = m->cf;
     a.upiv = m->upiv; a.pbits = m->pbits; a.seg = m->seg; a.bucket = m->bucket;
     a.xs = xs; a.dprime = m->dprime; a.dnz = m->dnz; a.queue = queue;
+    a.work = (unsigned long long *)(queue + 2);  // 8-byte aligned inside the 24-byte block
     a.md = Mod::make(m->p);
     const bool big = m->p >= (1u << 17);
     GBGPU_HIP(ctx, hipEventRecord(ctx->ev[0], s));
     if (m->nrl > 0) {
         switch (m->wshift) {
-            case 11: launch_sparse<11>(a, big, grid, s); break;
-            case 12: launch_sparse<12>(a, big, grid, s); break;
-            default: launch_sparse<13>(a, big, grid, s); break;
+            case 11: launch_sparse<11>(a, big, work != nullptr, grid, s); break;
+            case 12: launch_sparse<12>(a, big, work != nullptr, grid, s); break;
+            default: launch_sparse<13>(a, big, work != nullptr, grid, s); break;
         }
     }
     GBGPU_HIP(ctx, hipGetLastError());
@@ -462,6 +484,9 @@ int run_sparse_pass(gbgpu_mat *m) {
     GBGPU_HIP(ctx, hipStreamSynchronize(s));
     float ms = 0;
     if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->phase_ms[0] = ms;
+    if (work) {
+        GBGPU_HIP(ctx, hipMemcpy(work, a.work, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    }
     dfree(xs);
     dfree(queue);
     dfree(rank);

@@ -132,11 +132,13 @@ def sort_rows(m: F4Matrix) -> F4Matrix:
 
 
 def permute_rows(m: F4Matrix, order: np.ndarray) -> F4Matrix:
+    """Rows order[0], order[1], ... of m (a permutation or a subset)."""
+    order = np.asarray(order, dtype=np.int64)
     lens = np.diff(m.rp)[order]
-    rp = np.zeros(m.nr + 1, dtype=np.int64)
+    rp = np.zeros(len(order) + 1, dtype=np.int64)
     np.cumsum(lens, out=rp[1:])
     starts = m.rp[:-1][order]
     idx = np.repeat(starts - rp[:-1], lens) + np.arange(rp[-1], dtype=np.int64)
-    return F4Matrix(nr=m.nr, nc=m.nc, ncl=m.ncl, p=m.p, rp=rp,
+    return F4Matrix(nr=len(order), nc=m.nc, ncl=m.ncl, p=m.p, rp=rp,
                     ci=m.ci[idx].astype(np.int32), rcf=m.rcf[order].astype(np.int32),
                     cp=m.cp, cf=m.cf)

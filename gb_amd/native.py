@@ -14,7 +14,8 @@ import numpy as np
 from .matrix import F4Matrix, SparseRows
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgbgpu.so")
+# GBGPU_LIB may point at another build of the same C-ABI
+LIB_PATH = os.environ.get("GBGPU_LIB", os.path.join(HERE, "libgbgpu.so"))
 HEADER = os.path.join(os.path.dirname(HERE), "include", "gbgpu.h")
 
 FORM_PLAIN = 0
@@ -66,6 +67,7 @@ def lib():
         "gbgpu_mat_free": ([vp], None),
         "gbgpu_sparse_pass": ([vp, vp, pi32], C.c_int),
         "gbgpu_sparse_pass_download": ([vp, vp, vp, vp], C.c_int),
+        "gbgpu_sparse_pass_work": ([vp, vp, pi32, C.POINTER(C.c_uint64)], C.c_int),
         "gbgpu_la_sparse_dense": ([vp, vp, C.c_int, pi32], C.c_int),
         "gbgpu_la_sparse": ([vp, vp, C.c_int, pi32], C.c_int),
         "gbgpu_interreduce": ([vp, vp, C.c_int, pi32], C.c_int),
@@ -187,6 +189,13 @@ class DeviceMatrix:
         n = C.c_int32(0)
         self.ctx._check(lib().gbgpu_sparse_pass(self.ctx.h, self.h, C.byref(n)), "sparse pass")
         return int(n.value)
+
+    def sparse_pass_work(self):
+        """(#nonzero D' rows, pivots applied, pivot entries streamed)."""
+        n = C.c_int32(0)
+        w = (C.c_uint64 * 2)()
+        self.ctx._check(lib().gbgpu_sparse_pass_work(self.ctx.h, self.h, C.byref(n), w), "work")
+        return int(n.value), int(w[0]), int(w[1])
 
     def sparse_pass_result(self, nd: int):
         ncr = self.m.ncr

@@ -118,6 +118,13 @@ void gbgpu_mat_free(gbgpu_mat *m);
  * receives their number (the reference's mat->np after this step). */
 int gbgpu_sparse_pass(gbgpu_ctx *ctx, gbgpu_mat *m, int32_t *nrows);
 
+/* The sparse pass once more with work counters: work[0] receives the
+ * number of pivot rows applied (sum over to-reduce rows r of |S(r)|, an
+ * implementation-independent count), work[1] the number of pivot entries
+ * those rows hold (sum of len(P_j)).  For sizing and roofline figures; the
+ * result is the same as gbgpu_sparse_pass. */
+int gbgpu_sparse_pass_work(gbgpu_ctx *ctx, gbgpu_mat *m, int32_t *nrows, uint64_t *work);
+
 /* Download the D' rows of the last sparse pass: `dense` receives nrows x
  * (nc - ncl) entries row-major in the reference's order (drs,
  * la_ff_32.c:1157-1176); src_row[k] the input row of D' row k.  Either
