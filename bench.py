@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the F4 linear-algebra step on one MI355X.
 
-Workload (one step): the plain sparse pass -- reduce every to-reduce row of
+Workload (one step): the sparse pass -- reduce every to-reduce row of
 one F4 matrix by its known pivots (gb's sparse_AB_CD_linear_algebra,
 la_ff_32.c:1081-1179) -- on a seeded synthetic F4 matrix of 20000 rows x
 30000 columns mod 65521: 12000 known pivots (staircase leads, lognormal
@@ -10,7 +10,10 @@ density).  Inputs are uploaded once; the timed region covers the whole pass
 on the device (pivot layout, the reduction kernel, compaction of the
 nonzero rows).
 
-`value` is reduced rows per second.  The JSON line also carries:
+`value` is reduced rows per second of the blocked form (64-column groups
+eliminated through precomputed inverses of the pivots' diagonal blocks);
+the plain form (one pivot column at a time, ballot over the accumulator)
+is timed beside it.  The JSON line also carries:
   * roofline: the reduction kernel's algorithmic bytes per launch (pivot
     entries streamed + row inputs + dense outputs) over its event-timed
     average duration, against 8 TB/s HBM;
@@ -95,19 +98,24 @@ def main(argv=None) -> int:
     red_nnz = int(lens[~first].sum())
     bytes_per_launch = 8.0 * red_nnz + 8.0 * nent + 4.0 * m.ncr * nrl
 
-    for _ in range(args.warmup):
-        d.sparse_pass()
-    _sync()
-    kernel_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        d.sparse_pass()
-        kernel_ms.append(ctx.phase_ms(0))
-    _sync()
-    t = time.perf_counter() - t0
-    ms_per_step = 1000.0 * t / args.steps
-    rows_per_s = nrl / (t / args.steps)
-    kavg = sum(kernel_ms) / len(kernel_ms)
+    def timed(form, steps, warmup):
+        for _ in range(warmup):
+            d.sparse_pass(form)
+        _sync()
+        kernel_ms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            d.sparse_pass(form)
+            kernel_ms.append(ctx.phase_ms(0))
+        _sync()
+        return (time.perf_counter() - t0) / steps, sum(kernel_ms) / len(kernel_ms)
+
+    # headline: the blocked form (MFMA build of the library); the plain
+    # ballot-chain form is timed beside it
+    t_step, kavg = timed(native.FORM_MFMA, args.steps, args.warmup)
+    t_plain, kavg_plain = timed(native.FORM_PLAIN, max(2, args.steps // 2), 1)
+    ms_per_step = 1000.0 * t_step
+    rows_per_s = nrl / t_step
     achieved = bytes_per_launch / (kavg / 1000.0) / 1e9
 
     out = {
@@ -123,15 +131,20 @@ def main(argv=None) -> int:
         "vs_baseline": None,
         "dtype": "uint64 accumulators over uint32 residues mod p",
         "data": "synthetic (seeded F4-shaped matrix)",
-        "config": {"workload": "plain sparse pass, F4 matrix 20000x30000 mod 65521",
+        "config": {"workload": "sparse pass (known-pivot reduction), F4 matrix 20000x30000 mod 65521",
+                   "form": "blocked (64-column group inverses)",
                    "rows": NR, "cols": NC, "known_pivots": NPIV, "to_reduce": nrl,
                    "p": P, "seed": SEED, "nonzero_remainders": nd},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_sparse_plain", "kernel_ms": kavg,
+                     "kernel": "k_sparse_blocked", "kernel_ms": kavg,
                      "bytes_per_launch": bytes_per_launch,
                      "pivots_applied": napp, "pivot_entries": nent},
     }
+
+    out["plain_form"] = {"rows_per_s": nrl / t_plain, "ms_per_step": 1000.0 * t_plain,
+                         "kernel": "k_sparse_plain", "kernel_ms": kavg_plain,
+                         "roofline_frac": bytes_per_launch / (kavg_plain / 1000.0) / 1e9 / HBM_PEAK_GBS}
 
     if not args.no_cpu:
         threads = int(os.environ.get("BENCH_CPU_THREADS", min(16, os.cpu_count() or 1)))

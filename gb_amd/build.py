@@ -52,7 +52,22 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_adapter_driver()
     return LIB
+
+
+def build_adapter_driver() -> str:
+    """tests/_bin/adapter_driver: calls the gb adapters like gb's F4 loop."""
+    out_dir = os.path.join(ROOT, "tests", "_bin")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "adapter_driver")
+    cmd = ["gcc", "-O2", "-std=gnu11", "-Wall", os.path.join(ROOT, "tests", "adapter_driver.c"),
+           "-o", exe, "-L", HERE, "-lgbgpu", "-Wl,-rpath,$ORIGIN/../../gb_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building the adapter driver failed")
+    return exe
 
 
 if __name__ == "__main__":

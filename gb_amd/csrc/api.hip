@@ -26,7 +26,7 @@ using namespace gbgpu;
 gbgpu_mat::~gbgpu_mat() {
     dfree(rp); dfree(ci); dfree(rcf); dfree(cp); dfree(cf);
     dfree(first); dfree(upiv);
-    dfree(seg); dfree(bucket); dfree(pbits);
+    dfree(seg); dfree(bucket); dfree(pbits); dfree(gseg); dfree(gbucket);
     dfree(dprime); dfree(dnz); dfree(dlist);
     dfree(orp); dfree(oci); dfree(ocf);
 }
@@ -175,9 +175,10 @@ int gbgpu_mat_upload(gbgpu_ctx *ctx, const gbgpu_csr_t *h, gbgpu_mat **out) {
 
 void gbgpu_mat_free(gbgpu_mat *m) { delete m; }
 
-int gbgpu_sparse_pass(gbgpu_ctx *ctx, gbgpu_mat *m, int32_t *nrows) {
+int gbgpu_sparse_pass(gbgpu_ctx *ctx, gbgpu_mat *m, int form, int32_t *nrows) {
     if (!ctx || !m || m->ctx != ctx) return GBGPU_EINVAL;
-    int rc = run_sparse_pass(m);
+    if (form != GBGPU_FORM_PLAIN && form != GBGPU_FORM_MFMA) return set_error(ctx, GBGPU_EINVAL, "bad form");
+    int rc = run_sparse_pass(m, nullptr, form);
     if (rc) return rc;
     if (nrows) *nrows = m->nd;
     return GBGPU_OK;
@@ -232,7 +233,7 @@ static int run_la1(gbgpu_ctx *ctx, gbgpu_mat *m, int form) {
     if (form != GBGPU_FORM_PLAIN && form != GBGPU_FORM_MFMA) return set_error(ctx, GBGPU_EINVAL, "bad form");
     GBGPU_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
     clear_rows(m);
-    int rc = run_sparse_pass(m);
+    int rc = run_sparse_pass(m, nullptr, form);
     if (rc) return rc;
     const int32_t ncr = m->nc - m->ncl;
     if (m->nd > 0) {

@@ -26,6 +26,11 @@ struct gbgpu_mat {
     uint2 *bucket = nullptr;    // (column, coefficient), leads excluded
     uint32_t *pbits = nullptr;  // has-pivot bit per left column
     int64_t nbucket = 0;
+    // the same entries grouped per (window, 64-pivot group) for the blocked
+    // pass: gseg[w * ngroups + g] .. gseg[w * ngroups + g + 1], each entry
+    // x = (pivot index in group << 24) | column, y = coefficient
+    int64_t *gseg = nullptr;
+    uint2 *gbucket = nullptr;
     // sparse pass output
     uint32_t *dprime = nullptr; // nrl x ncr, row q = upiv[q]'s remainder
     int32_t *dnz = nullptr;     // nrl: 1 if row q is nonzero
@@ -62,7 +67,9 @@ int prep_split(gbgpu_mat *m, int32_t lim);
 int prep_buckets(gbgpu_mat *m);
 // work (optional, 2 values): pivots applied and pivot entries streamed,
 // counted by an instrumented launch
-int run_sparse_pass(gbgpu_mat *m, uint64_t *work = nullptr);
+// form: GBGPU_FORM_PLAIN = ballot chain, GBGPU_FORM_MFMA = blocked (group
+// inverses); the work counters always use the plain chain
+int run_sparse_pass(gbgpu_mat *m, uint64_t *work = nullptr, int form = GBGPU_FORM_PLAIN);
 
 // dense.hip: reduced echelon form of d (in place); form = GBGPU_FORM_*
 int dense_rref(gbgpu_dense *d, int form);

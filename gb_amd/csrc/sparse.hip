@@ -219,6 +219,71 @@ int prep_buckets(gbgpu_mat *m) {
     return GBGPU_OK;
 }
 
+// pivot entries per (window, group): counts, then placement
+__global__ __launch_bounds__(64) void k_gbucket_count(int32_t ncl, int32_t ngroups, int32_t wshift,
+                                                      const int32_t *first, const int64_t *rp,
+                                                      const int32_t *ci, int64_t *gcnt) {
+    const int lane = threadIdx.x;
+    for (int32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
+        const int32_t r = first[c];
+        if (r == INT32_MAX) continue;
+        const int32_t g = c >> 6;
+        for (int64_t k = rp[r] + 1 + lane; k < rp[r + 1]; k += 64)
+            atomicAdd((unsigned long long *)&gcnt[(int64_t)(ci[k] >> wshift) * ngroups + g], 1ull);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_gbucket_fill(int32_t ncl, int32_t ngroups, int32_t wshift,
+                                                     const int32_t *first, const int64_t *rp,
+                                                     const int32_t *ci, const int32_t *rcf,
+                                                     const int64_t *cp, const uint32_t *cf,
+                                                     int64_t *gcur, uint2 *gbucket) {
+    const int lane = threadIdx.x;
+    for (int32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
+        const int32_t r = first[c];
+        if (r == INT32_MAX) continue;
+        const int32_t g = c >> 6;
+        const uint32_t tag = (uint32_t)(c & 63) << 24;
+        const uint32_t *rc = cf + cp[rcf[r]] - rp[r];
+        for (int64_t k = rp[r] + 1 + lane; k < rp[r + 1]; k += 64) {
+            const int32_t col = ci[k];
+            const int64_t at = (int64_t)atomicAdd(
+                (unsigned long long *)&gcur[(int64_t)(col >> wshift) * ngroups + g], 1ull);
+            gbucket[at] = make_uint2(tag | (uint32_t)col, rc[k]);
+        }
+    }
+}
+
+int prep_group_buckets(gbgpu_mat *m) {
+    gbgpu_ctx *ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    dfree(m->gseg);
+    dfree(m->gbucket);
+    const int32_t ngroups = (m->ncl + 63) / 64;
+    const int64_t nseg = (int64_t)m->nwin * ngroups;
+    if (m->nc >= (1 << 24)) return set_error(ctx, GBGPU_ERANGE, "too many columns for the blocked pass");
+    int rc;
+    int64_t *gcnt = nullptr;
+    if ((rc = dalloc(ctx, &gcnt, (size_t)nseg + 1))) return rc;
+    if ((rc = dalloc(ctx, &m->gseg, (size_t)nseg + 1))) { dfree(gcnt); return rc; }
+    GBGPU_HIP(ctx, hipMemsetAsync(gcnt, 0, ((size_t)nseg + 1) * sizeof(int64_t), s));
+    const int g = m->ncl < 65536 * 4 ? (m->ncl > 0 ? m->ncl : 1) : 65536 * 4;
+    if (m->ncl > 0) k_gbucket_count<<<g, 64, 0, s>>>(m->ncl, ngroups, m->wshift, m->first, m->rp, m->ci, gcnt);
+    int64_t total = 0;
+    if ((rc = exclusive_scan_i64(ctx, gcnt, m->gseg, nseg + 1, &total))) { dfree(gcnt); return rc; }
+    if ((rc = dalloc(ctx, &m->gbucket, (size_t)total))) { dfree(gcnt); return rc; }
+    // gcnt becomes the running cursor per (window, group)
+    GBGPU_HIP(ctx, hipMemcpyAsync(gcnt, m->gseg, ((size_t)nseg + 1) * sizeof(int64_t),
+                                  hipMemcpyDeviceToDevice, s));
+    if (m->ncl > 0)
+        k_gbucket_fill<<<g, 64, 0, s>>>(m->ncl, ngroups, m->wshift, m->first, m->rp, m->ci, m->rcf,
+                                        m->cp, m->cf, gcnt, m->gbucket);
+    GBGPU_HIP(ctx, hipGetLastError());
+    GBGPU_HIP(ctx, hipStreamSynchronize(s));
+    dfree(gcnt);
+    return GBGPU_OK;
+}
+
 // ------------------------------------------------------ sparse pass
 
 struct SpArgs {
@@ -414,6 +479,169 @@ __global__ __launch_bounds__(64) void k_sparse_plain(SpArgs a) {
     }
 }
 
+// ------------------------------------------------ blocked sparse pass
+//
+// The chain of the plain pass (one pivot column after the other) is what
+// limits it: ~5400 dependent steps per row on an F4 matrix of 30000
+// columns.  Inside one 64-column group the dependencies only run through
+// the pivots' entries that fall in the same group: with T the group's unit
+// upper-triangular block (T[i][j] = entry of pivot g+i in column g+j, both
+// pivot columns), the group's multipliers are x = -v T^-1 mod p, where v is
+// the group's accumulator before any of its pivots is applied.  T^-1 is
+// computed once per matrix, so a group costs one 64 x 64 vector-matrix
+// product plus a fully parallel scatter of the group pivots' remaining
+// entries.  Columns without a pivot get an identity row/column in T and
+// v = 0, hence x = 0 -- the reference skips them too (la_ff_32.c:517-519).
+
+// one wave per 64-column group: T from the pivots' in-group entries, then
+// T^-1 row by row (lane i owns row i: Y[i][j] = -sum_{i<=k<j} Y[i][k] T[k][j])
+__global__ __launch_bounds__(64) void k_group_inverse(int32_t ncl, int32_t wshift,
+                                                      const uint32_t *pbits, const int64_t *seg,
+                                                      const uint2 *bucket, Mod md, uint32_t *tinv) {
+    __shared__ uint32_t T[64][65];
+    __shared__ uint32_t Y[64][65];
+    const int lane = threadIdx.x;
+    const int32_t g0 = blockIdx.x * 64;
+    const int w = g0 >> wshift;  // a group never straddles a window
+    for (int k = 0; k < 64; ++k) {
+        T[k][lane] = (k == lane) ? 1u : 0u;
+        Y[k][lane] = (k == lane) ? 1u : 0u;
+    }
+    const int32_t c = g0 + lane;
+    const bool has = c < ncl && ((pbits[c >> 5] >> (c & 31)) & 1u);
+    const uint64_t hmask = __ballot(has);
+    __syncthreads();
+    if (has) {
+        const int64_t sb = seg[(int64_t)w * ncl + c], se = seg[(int64_t)(w + 1) * ncl + c];
+        for (int64_t k = sb; k < se; ++k) {
+            const uint2 ent = bucket[k];
+            const int j = (int)ent.x - g0;
+            if (j > lane && j < 64 && ((hmask >> j) & 1ull)) T[lane][j] = ent.y;
+        }
+    }
+    __syncthreads();
+    for (int j = lane + 1; j < 64; ++j) {
+        uint64_t acc = 0;
+        for (int k = lane; k < j; ++k) acc += (uint64_t)md.mul(Y[lane][k], T[k][j]);
+        const uint32_t r = md.reduce(acc);
+        Y[lane][j] = r ? md.p - r : 0u;
+    }
+    __syncthreads();
+    uint32_t *out = tinv + (int64_t)blockIdx.x * 4096;
+    for (int k = 0; k < 64; ++k) out[k * 64 + lane] = Y[k][lane];
+}
+
+// add x_t * cf for every entry (t, column, cf) of one (window, group)
+// range: a coalesced stream, the multipliers of the group's 64 pivots in LDS
+template <bool BIG>
+__device__ __forceinline__ void stream_group(uint64_t *acc, int32_t w0, uint32_t x, int64_t b,
+                                             int64_t e, const uint2 *gbucket, uint32_t *sh_x,
+                                             const Mod &md) {
+    const int lane = threadIdx.x & 63;
+    if (!__any(x != 0) || e <= b) return;
+    sh_x[lane] = x;
+    __syncthreads();
+    for (int64_t k = b + lane; k < e; k += 64) {
+        const uint2 ent = gbucket[k];
+        const uint32_t xt = sh_x[ent.x >> 24];
+        if (xt) atomicAdd(&acc[(ent.x & 0xFFFFFFu) - (uint32_t)w0], addend<BIG>(xt, ent.y, md));
+    }
+    __syncthreads();
+}
+
+template <int WSHIFT, bool BIG>
+__global__ __launch_bounds__(64) void k_sparse_blocked(SpArgs a, const uint32_t *tinv,
+                                                       const int64_t *gseg, const uint2 *gbucket,
+                                                       int32_t ngroups) {
+    constexpr int W = 1 << WSHIFT;
+    __shared__ uint64_t acc[W];
+    __shared__ uint32_t pb[W / 32];
+    __shared__ uint32_t sh_x[64];
+    const int lane = threadIdx.x;
+    const uint32_t p = a.md.p;
+    uint32_t *xs = a.xs + (size_t)blockIdx.x * (size_t)(a.ncl > 0 ? a.ncl : 1);
+
+    for (;;) {
+        // row index broadcast as a scalar (see k_sparse_plain)
+        const int q0 = atomicAdd(a.queue, lane == 0 ? 1 : 0);
+        int q = __builtin_amdgcn_readfirstlane(q0);
+        if (q >= a.nrl) break;
+        q = a.nrl - 1 - q;
+        const int32_t row = __builtin_amdgcn_readfirstlane(a.upiv[q]);
+        const int64_t rb = a.rp[row], re = a.rp[row + 1];
+        const uint32_t *rcf = a.cf + a.cp[a.rcf[row]] - rb;
+        const int32_t lead = __builtin_amdgcn_readfirstlane(a.ci[rb]);
+        const int32_t g_lead = lead >> 6;
+        bool nzrow = false;
+
+        for (int w = lead >> WSHIFT; w < a.nwin; ++w) {
+            const int32_t w0 = w << WSHIFT;
+            const int32_t w1 = min(w0 + W, a.nc);
+            const int32_t lend = min(w1, a.ncl);
+            const int64_t *gs = gseg + (int64_t)w * ngroups;
+            for (int c = lane; c < W; c += 64) acc[c] = 0;
+            for (int k = lane; k < W / 32; k += 64)
+                pb[k] = (w0 + 32 * k < lend) ? a.pbits[(w0 >> 5) + k] : 0u;
+            __syncthreads();
+            for (int64_t k = rb + lane; k < re; k += 64) {
+                const int32_t c = a.ci[k];
+                if (c >= w0 && c < w1) acc[c - w0] = rcf[k];
+            }
+            __syncthreads();
+            // groups of earlier windows (their multipliers are in xs); the
+            // last group of pivot columns may be partial
+            const int32_t jl = min(w0, a.ncl);
+            const int32_t g_end = (jl + 63) >> 6;
+            for (int32_t gi = g_lead; gi < g_end; ++gi) {
+                const int32_t j = gi * 64 + lane;
+                const uint32_t x = j < jl ? xs[j] : 0u;
+                stream_group<BIG>(acc, w0, x, gs[gi], gs[gi + 1], gbucket, sh_x, a.md);
+            }
+            // this window's groups
+            for (int32_t g = max(w0, g_lead << 6); g < lend; g += 64) {
+                const int32_t c = g + lane;
+                uint32_t u = 0;
+                if (c < lend) {
+                    const uint64_t v = acc[c - w0];
+                    const uint32_t r = v ? a.md.reduce(v) : 0u;
+                    if (r && ((pb[(c - w0) >> 5] >> ((c - w0) & 31)) & 1u)) u = p - r;
+                }
+                // x = u T^-1 (u = -v mod p); lane j owns column j
+                const uint32_t *tp = tinv + (int64_t)(g >> 6) * 4096 + lane;
+                uint32_t tcol[64];
+#pragma unroll
+                for (int i = 0; i < 64; ++i) tcol[i] = tp[i * 64];
+                uint64_t sx = 0;
+#pragma unroll
+                for (int i = 0; i < 64; ++i) {
+                    const uint32_t ui = __builtin_amdgcn_readlane(u, i);
+                    if (BIG) sx += (uint64_t)a.md.mul(ui, tcol[i]);
+                    else sx += (uint64_t)ui * (uint64_t)tcol[i];
+                }
+                uint32_t x = a.md.reduce(sx);
+                if (c < lend) xs[c] = x;
+                else x = 0;
+                // also adds the group's own in-group entries: harmless (its
+                // accumulator values are consumed) and needed for right-hand
+                // columns sharing the last partial group
+                stream_group<BIG>(acc, w0, x, gs[g >> 6], gs[(g >> 6) + 1], gbucket, sh_x, a.md);
+            }
+            if (w1 > a.ncl) {
+                const int32_t cs = max(w0, a.ncl);
+                uint32_t *out = a.dprime + (size_t)q * (size_t)a.ncr - a.ncl;
+                for (int32_t c = cs + lane; c < w1; c += 64) {
+                    const uint32_t r = a.md.reduce(acc[c - w0]);
+                    out[c] = r;
+                    nzrow |= r != 0;
+                }
+            }
+            __syncthreads();
+        }
+        const bool anynz = __any(nzrow);
+        a.dnz[q] = anynz ? 1 : 0;
+    }
+}
+
 __global__ void k_compact(int32_t n, const int32_t *flag, const int32_t *rank, int32_t *list) {
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         if (flag[i]) list[rank[i]] = i;
@@ -430,7 +658,14 @@ static void launch_sparse(const SpArgs &a, bool big, bool count, int grid, hipSt
     }
 }
 
-int run_sparse_pass(gbgpu_mat *m, uint64_t *work) {
+template <int WS>
+static void launch_blocked(const SpArgs &a, const uint32_t *tinv, const int64_t *gseg,
+                           const uint2 *gbucket, int32_t ngroups, bool big, int grid, hipStream_t s) {
+    if (big) k_sparse_blocked<WS, true><<<grid, 64, 0, s>>>(a, tinv, gseg, gbucket, ngroups);
+    else k_sparse_blocked<WS, false><<<grid, 64, 0, s>>>(a, tinv, gseg, gbucket, ngroups);
+}
+
+int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
     gbgpu_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
     int rc;
@@ -442,15 +677,26 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work) {
     if ((rc = dalloc(ctx, &m->dprime, (size_t)m->nrl * (size_t)ncr)) ||
         (rc = dalloc(ctx, &m->dnz, (size_t)m->nrl)))
         return rc;
-    // waves in flight: LDS of one wave is W*8 bytes + ~1.5 KB, 160 KB per CU
+    // the blocked form needs the inverses of the groups' pivot blocks
+    const bool blocked = form == GBGPU_FORM_MFMA && work == nullptr;
+    uint32_t *tinv = nullptr;
+    const int32_t ngroups = (m->ncl + 63) / 64;
+    if (blocked && (rc = prep_group_buckets(m))) return rc;
+    if (blocked && ngroups > 0) {
+        if ((rc = dalloc(ctx, &tinv, (size_t)ngroups * 4096))) return rc;
+        k_group_inverse<<<ngroups, 64, 0, s>>>(m->ncl, m->wshift, m->pbits, m->seg, m->bucket,
+                                               Mod::make(m->p), tinv);
+        GBGPU_HIP(ctx, hipGetLastError());
+    }
+    // waves in flight: LDS of one wave is W*8 bytes + staging, 160 KB per CU
     const int W = 1 << m->wshift;
-    const int per_cu = (160 * 1024) / (W * 8 + STAGE * 8 + 2048);
+    const int per_cu = (160 * 1024) / (W * 8 + (blocked ? 0 : STAGE * 8) + 2048);
     int grid = ctx->num_cu * (per_cu > 0 ? per_cu : 1);
     if (grid > m->nrl) grid = m->nrl > 0 ? m->nrl : 1;
     uint32_t *xs = nullptr;
     int32_t *queue = nullptr;
-    if ((rc = dalloc(ctx, &xs, (size_t)grid * (size_t)(m->ncl > 0 ? m->ncl : 1)))) return rc;
-    if ((rc = dalloc(ctx, &queue, 6))) { dfree(xs); return rc; }
+    if ((rc = dalloc(ctx, &xs, (size_t)grid * (size_t)(m->ncl > 0 ? m->ncl : 1)))) { dfree(tinv); return rc; }
+    if ((rc = dalloc(ctx, &queue, 6))) { dfree(xs); dfree(tinv); return rc; }
     GBGPU_HIP(ctx, hipMemsetAsync(queue, 0, 6 * sizeof(int32_t), s));
     SpArgs a;
     a.nc = m->nc; a.ncl = m->ncl; a.ncr = ncr; a.nrl = m->nrl; a.nwin = m->nwin;
@@ -461,7 +707,13 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work) {
     a.md = Mod::make(m->p);
     const bool big = m->p >= (1u << 17);
     GBGPU_HIP(ctx, hipEventRecord(ctx->ev[0], s));
-    if (m->nrl > 0) {
+    if (m->nrl > 0 && blocked) {
+        switch (m->wshift) {
+            case 11: launch_blocked<11>(a, tinv, m->gseg, m->gbucket, ngroups, big, grid, s); break;
+            case 12: launch_blocked<12>(a, tinv, m->gseg, m->gbucket, ngroups, big, grid, s); break;
+            default: launch_blocked<13>(a, tinv, m->gseg, m->gbucket, ngroups, big, grid, s); break;
+        }
+    } else if (m->nrl > 0) {
         switch (m->wshift) {
             case 11: launch_sparse<11>(a, big, work != nullptr, grid, s); break;
             case 12: launch_sparse<12>(a, big, work != nullptr, grid, s); break;
@@ -490,6 +742,7 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work) {
     dfree(xs);
     dfree(queue);
     dfree(rank);
+    dfree(tinv);
     return GBGPU_OK;
 }
 

@@ -16,7 +16,7 @@ from .matrix import F4Matrix, SparseRows
 HERE = os.path.dirname(os.path.abspath(__file__))
 # GBGPU_LIB may point at another build of the same C-ABI
 LIB_PATH = os.environ.get("GBGPU_LIB", os.path.join(HERE, "libgbgpu.so"))
-HEADER = os.path.join(os.path.dirname(HERE), "include", "gbgpu.h")
+HEADERS = [os.path.join(os.path.dirname(HERE), "include", h) for h in ("gbgpu.h", "gbgpu_gb.h")]
 
 FORM_PLAIN = 0
 FORM_MFMA = 1
@@ -40,10 +40,12 @@ class _Rows(C.Structure):
 
 
 def declared_symbols() -> list[str]:
-    """Every function declared in include/gbgpu.h."""
-    txt = open(HEADER).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(gbgpu_[a-z0-9_]+)\s*\(", txt)))
+    """Every function declared in include/gbgpu.h and include/gbgpu_gb.h."""
+    names = set()
+    for h in HEADERS:
+        txt = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names.update(re.findall(r"\b(gbgpu_[a-z0-9_]+)\s*\(", txt))
+    return sorted(names)
 
 
 def lib():
@@ -65,7 +67,7 @@ def lib():
         "gbgpu_phase_ms": ([vp, C.c_int], C.c_double),
         "gbgpu_mat_upload": ([vp, C.POINTER(_Csr), C.POINTER(vp)], C.c_int),
         "gbgpu_mat_free": ([vp], None),
-        "gbgpu_sparse_pass": ([vp, vp, pi32], C.c_int),
+        "gbgpu_sparse_pass": ([vp, vp, C.c_int, pi32], C.c_int),
         "gbgpu_sparse_pass_download": ([vp, vp, vp, vp], C.c_int),
         "gbgpu_sparse_pass_work": ([vp, vp, pi32, C.POINTER(C.c_uint64)], C.c_int),
         "gbgpu_la_sparse_dense": ([vp, vp, C.c_int, pi32], C.c_int),
@@ -185,9 +187,9 @@ class DeviceMatrix:
         except Exception:
             pass
 
-    def sparse_pass(self) -> int:
+    def sparse_pass(self, form: int = FORM_PLAIN) -> int:
         n = C.c_int32(0)
-        self.ctx._check(lib().gbgpu_sparse_pass(self.ctx.h, self.h, C.byref(n)), "sparse pass")
+        self.ctx._check(lib().gbgpu_sparse_pass(self.ctx.h, self.h, form, C.byref(n)), "sparse pass")
         return int(n.value)
 
     def sparse_pass_work(self):

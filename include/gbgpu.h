@@ -115,8 +115,11 @@ void gbgpu_mat_free(gbgpu_mat *m);
 /* Replaces sparse_AB_CD_linear_algebra_ff_{8,16,32}
  * (la_ff_32.c:1081-1179, la_ff_16.c:737-836): reduce every to-reduce row
  * by the known pivots; the nonzero D' rows stay on the device.  *nrows
- * receives their number (the reference's mat->np after this step). */
-int gbgpu_sparse_pass(gbgpu_ctx *ctx, gbgpu_mat *m, int32_t *nrows);
+ * receives their number (the reference's mat->np after this step).
+ * GBGPU_FORM_PLAIN walks the pivot columns one at a time (ballot over the
+ * accumulator); GBGPU_FORM_MFMA eliminates 64 columns at a time with the
+ * inverses of the pivots' diagonal blocks.  Same output. */
+int gbgpu_sparse_pass(gbgpu_ctx *ctx, gbgpu_mat *m, int form, int32_t *nrows);
 
 /* The sparse pass once more with work counters: work[0] receives the
  * number of pivot rows applied (sum over to-reduce rows r of |S(r)|, an

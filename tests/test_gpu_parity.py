@@ -20,10 +20,11 @@ def _f4(nr, nc, npiv, p, seed, **kw):
 
 
 @pytest.mark.parametrize("p", PRIMES)
-def test_sparse_pass_small(gpu_lib, p):
+@pytest.mark.parametrize("form", FORMS)
+def test_sparse_pass_small(gpu_lib, p, form):
     m = _f4(300, 500, 180, p, seed=p % 1000, piv_len=(3, 40), piv_mu=np.log(12), red_nnz=25)
     d = gpu_lib.upload(m)
-    nd = d.sparse_pass()
+    nd = d.sparse_pass(form)
     got, src = d.sparse_pass_result(nd)
     exp, esrc = oracle.sparse_pass(m)
     assert nd == len(exp)
@@ -32,11 +33,12 @@ def test_sparse_pass_small(gpu_lib, p):
 
 
 @pytest.mark.parametrize("p", [251, 65521, 2147483647])
-def test_sparse_pass_many_windows(gpu_lib, p):
+@pytest.mark.parametrize("form", FORMS)
+def test_sparse_pass_many_windows(gpu_lib, p, form):
     # 24000 columns: 12 column windows of the LDS accumulator
     m = _f4(900, 24000, 600, p, seed=5, piv_len=(20, 200), piv_mu=np.log(80), red_nnz=120)
     d = gpu_lib.upload(m)
-    nd = d.sparse_pass()
+    nd = d.sparse_pass(form)
     got, src = d.sparse_pass_result(nd)
     exp, esrc = oracle.sparse_pass(m, nthreads=8)
     assert np.array_equal(src, esrc)
@@ -105,7 +107,8 @@ def test_edge_no_rows_to_reduce(gpu_lib):
     # every row is a known pivot: nothing to reduce, no new pivots
     m = _f4(50, 90, 50, 65521, seed=1, piv_len=(2, 10), piv_mu=np.log(5))
     d = gpu_lib.upload(m)
-    assert d.sparse_pass() == 0
+    assert d.sparse_pass(FORM_PLAIN) == 0
+    assert d.sparse_pass(FORM_MFMA) == 0
     assert d.la_sparse_dense(FORM_MFMA) == 0
     assert d.rows().np == 0
 
@@ -129,7 +132,8 @@ def test_edge_everything_reduces_to_zero(gpu_lib):
         red.append(sum(int(coef[j]) * piv[j] for j in range(ncl)) % p)
     m = sort_rows(from_dense_rows(piv + red, nc, ncl, p))
     d = gpu_lib.upload(m)
-    assert d.sparse_pass() == 0
+    assert d.sparse_pass(FORM_PLAIN) == 0
+    assert d.sparse_pass(FORM_MFMA) == 0
     assert d.la_sparse_dense(FORM_PLAIN) == 0
     assert oracle.la_sparse_dense(m).np == 0
 
@@ -179,12 +183,13 @@ def _pivot_rows(m):
 
 
 @pytest.mark.parametrize("p", [65521, 2147483629])
-def test_full_size_sparse_pass_sampled(gpu_lib, p):
+@pytest.mark.parametrize("form", FORMS)
+def test_full_size_sparse_pass_sampled(gpu_lib, p, form):
     """20000 x 30000 F4 matrix (12000 known pivots): the GPU pass over all
     8000 to-reduce rows, checked against the oracle on 48 sampled rows."""
     m = _f4(20000, 30000, 12000, p, seed=2024)
     d = gpu_lib.upload(m)
-    nd = d.sparse_pass()
+    nd = d.sparse_pass(form)
     got, src = d.sparse_pass_result(nd)
     piv = _pivot_rows(m)
     nonpiv = np.setdiff1d(np.arange(m.nr), piv)
