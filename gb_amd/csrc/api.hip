@@ -5,6 +5,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 
 #include "internal.h"
 
@@ -17,6 +20,82 @@ int set_error(gbgpu_ctx *ctx, int code, const std::string &msg) {
 
 int check_hip(gbgpu_ctx *ctx, hipError_t e, const char *what) {
     return set_error(ctx, GBGPU_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------ caching pool
+//
+// Size classes: powers of two up to 2 MiB, multiples of 2 MiB above.  A
+// cached block is reused for a request of its class up to twice its size.
+// Blocks are tied to the device they were allocated on.
+namespace {
+struct Pool {
+    std::mutex mu;
+    std::multimap<size_t, std::pair<int, void *>> idle;   // class -> (device, ptr)
+    std::unordered_map<void *, std::pair<int, size_t>> live;
+    size_t idle_bytes = 0;
+};
+Pool &pool() {
+    static Pool *p = new Pool();  // never destroyed: freed blocks may outlive statics
+    return *p;
+}
+size_t size_class(size_t b) {
+    const size_t mb2 = (size_t)2 << 20;
+    if (b >= mb2) return (b + mb2 - 1) / mb2 * mb2;
+    size_t c = 256;
+    while (c < b) c <<= 1;
+    return c;
+}
+void release_idle(Pool &P) {
+    for (auto &kv : P.idle) {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(kv.second.first);
+        (void)hipFree(kv.second.second);
+        (void)hipSetDevice(cur);
+    }
+    P.idle.clear();
+    P.idle_bytes = 0;
+}
+}  // namespace
+
+void *pool_alloc(size_t bytes) {
+    Pool &P = pool();
+    const size_t cls = size_class(bytes);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(P.mu);
+    for (auto it = P.idle.lower_bound(cls); it != P.idle.end() && it->first <= 2 * cls; ++it) {
+        if (it->second.first != dev) continue;
+        void *ptr = it->second.second;
+        P.live[ptr] = {dev, it->first};
+        P.idle_bytes -= it->first;
+        P.idle.erase(it);
+        return ptr;
+    }
+    void *ptr = nullptr;
+    if (hipMalloc(&ptr, cls) != hipSuccess) {
+        (void)hipGetLastError();
+        release_idle(P);
+        ptr = nullptr;
+        if (hipMalloc(&ptr, cls) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    P.live[ptr] = {dev, cls};
+    return ptr;
+}
+
+void pool_free(void *ptr) {
+    Pool &P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto it = P.live.find(ptr);
+    if (it == P.live.end()) return;
+    P.idle.insert({it->second.second, {it->second.first, ptr}});
+    P.idle_bytes += it->second.second;
+    P.live.erase(it);
+    // keep at most 64 GiB idle
+    if (P.idle_bytes > ((size_t)64 << 30)) release_idle(P);
 }
 
 }  // namespace gbgpu

@@ -84,23 +84,26 @@ int check_hip(gbgpu_ctx *ctx, hipError_t e, const char *what);
         if (e_ != hipSuccess) return gbgpu::check_hip((ctx), e_, #call); \
     } while (0)
 
+// Device memory comes from a caching pool (api.hip): buffers freed by one
+// call are reused by the next one instead of going back to hipFree, so a
+// step of the hot path does not pay for hipMalloc.  Every entry point
+// synchronises its stream before it returns, so a freed buffer is idle.
+void *pool_alloc(size_t bytes);
+void pool_free(void *p);
+
 template <class T>
 int dalloc(gbgpu_ctx *ctx, T **p, size_t n) {
-    *p = nullptr;
     if (n == 0) n = 1;
-    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
-    if (e != hipSuccess) {
-        *p = nullptr;
-        (void)hipGetLastError();
+    *p = (T *)pool_alloc(n * sizeof(T));
+    if (!*p)
         return set_error(ctx, GBGPU_ENOMEM,
                          "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
-    }
     return GBGPU_OK;
 }
 
 template <class T>
 void dfree(T *&p) {
-    if (p) (void)hipFree((void *)p);
+    if (p) pool_free((void *)p);
     p = nullptr;
 }
 

@@ -124,16 +124,20 @@ int prep_split(gbgpu_mat *m, int32_t lim) {
 // pivot entry counts (lead excluded) and has-pivot bits per left column
 __global__ void k_pivlen(int32_t ncl, const int32_t *first, const int64_t *rp, int64_t *plen,
                          uint32_t *pbits) {
-    for (int32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncl; c += gridDim.x * blockDim.x) {
+    // one column per thread, no loop (the grid covers ncl): the ballot and
+    // the lane-specific stores below must not sit inside a loop
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    bool has = false;
+    if (c < ncl) {
         const int32_t r = first[c];
-        const bool has = r != INT32_MAX;
+        has = r != INT32_MAX;
         plen[c] = has ? rp[r + 1] - rp[r] - 1 : 0;
-        const uint64_t bal = __ballot(has);
-        const int lane = threadIdx.x & 63;
-        // 64 consecutive columns per wave when blockDim is a multiple of 64
-        if (lane == 0) pbits[c >> 5] = (uint32_t)bal;
-        if (lane == 32) pbits[c >> 5] = (uint32_t)(bal >> 32);
     }
+    const uint64_t bal = __ballot(has);
+    const int lane = threadIdx.x & 63;
+    // 64 consecutive columns per wave (blockDim is a multiple of 64)
+    if (c < ncl && lane == 0) pbits[c >> 5] = (uint32_t)bal;
+    if (c < ncl && lane == 32) pbits[c >> 5] = (uint32_t)(bal >> 32);
 }
 
 // one wave per pivot column: histogram by window, scan, scatter
@@ -219,37 +223,40 @@ int prep_buckets(gbgpu_mat *m) {
     return GBGPU_OK;
 }
 
-// pivot entries per (window, group): counts, then placement
-__global__ __launch_bounds__(64) void k_gbucket_count(int32_t ncl, int32_t ngroups, int32_t wshift,
-                                                      const int32_t *first, const int64_t *rp,
-                                                      const int32_t *ci, int64_t *gcnt) {
+// pivot entries per (window, group), derived from the per-pivot window
+// segments without atomics: one wave per group, lane t = pivot 64 g + t
+__global__ __launch_bounds__(64) void k_gbucket_count(int32_t ncl, int32_t ngroups, int32_t nwin,
+                                                      const int64_t *seg, int64_t *gcnt) {
     const int lane = threadIdx.x;
-    for (int32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
-        const int32_t r = first[c];
-        if (r == INT32_MAX) continue;
-        const int32_t g = c >> 6;
-        for (int64_t k = rp[r] + 1 + lane; k < rp[r + 1]; k += 64)
-            atomicAdd((unsigned long long *)&gcnt[(int64_t)(ci[k] >> wshift) * ngroups + g], 1ull);
+    const int32_t g = blockIdx.x;
+    const int32_t c = g * 64 + lane;
+    for (int w = 0; w < nwin; ++w) {
+        int len = 0;
+        if (c < ncl) len = (int)(seg[(int64_t)(w + 1) * ncl + c] - seg[(int64_t)w * ncl + c]);
+        const int tot = __builtin_amdgcn_readfirstlane(__shfl(wave_incl_scan(len), 63, 64));
+        gcnt[(int64_t)w * ngroups + g] = tot;  // same value from every lane
     }
 }
 
-__global__ __launch_bounds__(64) void k_gbucket_fill(int32_t ncl, int32_t ngroups, int32_t wshift,
-                                                     const int32_t *first, const int64_t *rp,
-                                                     const int32_t *ci, const int32_t *rcf,
-                                                     const int64_t *cp, const uint32_t *cf,
-                                                     int64_t *gcur, uint2 *gbucket) {
+__global__ __launch_bounds__(64) void k_gbucket_fill(int32_t ncl, int32_t ngroups, int32_t nwin,
+                                                     const int64_t *seg, const uint2 *bucket,
+                                                     const int64_t *gseg, uint2 *gbucket) {
     const int lane = threadIdx.x;
-    for (int32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
-        const int32_t r = first[c];
-        if (r == INT32_MAX) continue;
-        const int32_t g = c >> 6;
-        const uint32_t tag = (uint32_t)(c & 63) << 24;
-        const uint32_t *rc = cf + cp[rcf[r]] - rp[r];
-        for (int64_t k = rp[r] + 1 + lane; k < rp[r + 1]; k += 64) {
-            const int32_t col = ci[k];
-            const int64_t at = (int64_t)atomicAdd(
-                (unsigned long long *)&gcur[(int64_t)(col >> wshift) * ngroups + g], 1ull);
-            gbucket[at] = make_uint2(tag | (uint32_t)col, rc[k]);
+    const int32_t g = blockIdx.x;
+    const int32_t c = g * 64 + lane;
+    const uint32_t tag = (uint32_t)lane << 24;
+    for (int w = 0; w < nwin; ++w) {
+        int64_t sb = 0;
+        int len = 0;
+        if (c < ncl) {
+            sb = seg[(int64_t)w * ncl + c];
+            len = (int)(seg[(int64_t)(w + 1) * ncl + c] - sb);
+        }
+        const int incl = wave_incl_scan(len);
+        const int64_t at = gseg[(int64_t)w * ngroups + g] + (incl - len);
+        for (int k = 0; k < len; ++k) {
+            const uint2 e = bucket[sb + k];
+            gbucket[at + k] = make_uint2(tag | e.x, e.y);
         }
     }
 }
@@ -267,17 +274,13 @@ int prep_group_buckets(gbgpu_mat *m) {
     if ((rc = dalloc(ctx, &gcnt, (size_t)nseg + 1))) return rc;
     if ((rc = dalloc(ctx, &m->gseg, (size_t)nseg + 1))) { dfree(gcnt); return rc; }
     GBGPU_HIP(ctx, hipMemsetAsync(gcnt, 0, ((size_t)nseg + 1) * sizeof(int64_t), s));
-    const int g = m->ncl < 65536 * 4 ? (m->ncl > 0 ? m->ncl : 1) : 65536 * 4;
-    if (m->ncl > 0) k_gbucket_count<<<g, 64, 0, s>>>(m->ncl, ngroups, m->wshift, m->first, m->rp, m->ci, gcnt);
+    if (ngroups > 0) k_gbucket_count<<<ngroups, 64, 0, s>>>(m->ncl, ngroups, m->nwin, m->seg, gcnt);
     int64_t total = 0;
     if ((rc = exclusive_scan_i64(ctx, gcnt, m->gseg, nseg + 1, &total))) { dfree(gcnt); return rc; }
     if ((rc = dalloc(ctx, &m->gbucket, (size_t)total))) { dfree(gcnt); return rc; }
-    // gcnt becomes the running cursor per (window, group)
-    GBGPU_HIP(ctx, hipMemcpyAsync(gcnt, m->gseg, ((size_t)nseg + 1) * sizeof(int64_t),
-                                  hipMemcpyDeviceToDevice, s));
-    if (m->ncl > 0)
-        k_gbucket_fill<<<g, 64, 0, s>>>(m->ncl, ngroups, m->wshift, m->first, m->rp, m->ci, m->rcf,
-                                        m->cp, m->cf, gcnt, m->gbucket);
+    if (ngroups > 0)
+        k_gbucket_fill<<<ngroups, 64, 0, s>>>(m->ncl, ngroups, m->nwin, m->seg, m->bucket, m->gseg,
+                                              m->gbucket);
     GBGPU_HIP(ctx, hipGetLastError());
     GBGPU_HIP(ctx, hipStreamSynchronize(s));
     dfree(gcnt);
@@ -541,10 +544,22 @@ __device__ __forceinline__ void stream_group(uint64_t *acc, int32_t w0, uint32_t
     if (!__any(x != 0) || e <= b) return;
     sh_x[lane] = x;
     __syncthreads();
-    for (int64_t k = b + lane; k < e; k += 64) {
+    int64_t k = b + lane;
+    // four entries in flight per lane
+    for (; k + 192 < e; k += 256) {
+        uint2 en[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) en[u] = gbucket[k + 64 * u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t xt = sh_x[en[u].x >> 24];
+            atomicAdd(&acc[(en[u].x & 0xFFFFFFu) - (uint32_t)w0], addend<BIG>(xt, en[u].y, md));
+        }
+    }
+    for (; k < e; k += 64) {
         const uint2 ent = gbucket[k];
         const uint32_t xt = sh_x[ent.x >> 24];
-        if (xt) atomicAdd(&acc[(ent.x & 0xFFFFFFu) - (uint32_t)w0], addend<BIG>(xt, ent.y, md));
+        atomicAdd(&acc[(ent.x & 0xFFFFFFu) - (uint32_t)w0], addend<BIG>(xt, ent.y, md));
     }
     __syncthreads();
 }
@@ -764,7 +779,7 @@ int exclusive_scan_i64(gbgpu_ctx *ctx, const int64_t *in, int64_t *out, int64_t 
     int64_t last_out = 0;
     GBGPU_HIP(ctx, hipMemcpyAsync(&last_out, out + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     GBGPU_HIP(ctx, hipStreamSynchronize(s));
-    (void)hipFree(d_tmp);
+    dfree(d_tmp);
     *total = last_out + last_in;
     return GBGPU_OK;
 }
@@ -785,7 +800,7 @@ int exclusive_scan_i32(gbgpu_ctx *ctx, const int32_t *in, int32_t *out, int64_t 
     int32_t last_out = 0;
     GBGPU_HIP(ctx, hipMemcpyAsync(&last_out, out + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     GBGPU_HIP(ctx, hipStreamSynchronize(s));
-    (void)hipFree(d_tmp);
+    dfree(d_tmp);
     *total = (int64_t)last_out + last_in;
     return GBGPU_OK;
 }
