@@ -57,15 +57,33 @@ __device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p) 
 }
 
 // ------------------------------------------------------------ select
+//
+// One workgroup finds the panel's pivots.  Non-pivot rows are taken in
+// order, up to 64 at a time, into an LDS tile; every tile row carries
+// [its 64 panel entries | its transform over the selected rows].  The tile
+// is first reduced by the pivot rows already found (E, reduced echelon
+// within the panel), then eliminated column by column: for each panel
+// column without a pivot, any tile row with a nonzero entry there becomes
+// the pivot (normalised), and that column is cleared from every other tile
+// row and from E.  A tile row that is not selected ends up zero on the
+// whole panel (it lies in the span of E) and is dropped.  Updates touch
+// only the columns that can be nonzero: panel columns >= j and transform
+// slots <= k.  The selected rows' transform T satisfies
+// E = T * A[selected rows, panel], so Rp = T * A[selected rows, c0:n] is
+// in reduced form with an identity on the panel's pivot columns.
+
+constexpr int XW = 2 * PB + 1;  // padded tile row: panel part, transform part
 
 __global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t ld, int32_t m,
                                                       int32_t n, int32_t c0, int32_t *rowpiv,
                                                       int32_t *colpiv, Panel *pan, Mod md) {
-    __shared__ uint32_t E[PB][PB];
-    __shared__ uint32_t T[PB][PB];
-    __shared__ int32_t ecol[PB], erow[PB];
-    __shared__ uint32_t F[PB];
-    __shared__ int32_t sh_best, sh_any, sh_k, sh_cstar;
+    __shared__ uint32_t E[PB][XW];
+    __shared__ uint32_t C[PB][XW];
+    __shared__ uint32_t G[PB][PB + 1];
+    __shared__ uint32_t F[2 * PB];
+    __shared__ int32_t ecol[PB], erow[PB], crow[PB], used[PB];
+    __shared__ int32_t sh_ncand, sh_next, sh_istar;
+    __shared__ uint32_t sh_inv;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t p = md.p;
     if (pan->done) {
@@ -73,90 +91,135 @@ __global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t
         return;
     }
     const int bw = min(PB, n - c0);
-    for (int i = tid; i < PB * PB; i += 256) { (&E[0][0])[i] = 0; (&T[0][0])[i] = 0; }
-    // shared flags are written by every thread with the same value: no
-    // thread-specific branches inside the loops below (see sparse.hip)
-    sh_k = 0;
-    sh_any = 0;
-    __syncthreads();
+    int kk = 0;            // pivots found (same value in every thread)
+    uint64_t pivmask = 0;  // panel columns that have a pivot
     int32_t cursor = 0;
+    bool any = false;
     for (;;) {
-        sh_best = INT_MAX;
-        __syncthreads();
-        // values every lane shares are made scalar: loops that contain
-        // cross-lane operations must have wave-uniform trip counts and exits
-        const int kk = __builtin_amdgcn_readfirstlane(sh_k);
-        // parallel scan: first non-pivot row >= cursor not in span(E)
-        for (int32_t r = cursor + wv; r < m; r += 4) {
-            if (r > __builtin_amdgcn_readfirstlane(*(volatile int32_t *)&sh_best)) break;
-            if (__builtin_amdgcn_readfirstlane(rowpiv[r]) >= 0) continue;
-            sh_any = 1;
-            const uint32_t v = lane < bw ? a[(int64_t)r * ld + c0 + lane] : 0u;
-            uint64_t s = 0;
-            for (int t = 0; t < kk; ++t) {
-                const uint32_t ct = __shfl(v, ecol[t] - c0, 64);
-                s += (uint64_t)md.mul(ct, E[t][lane]);
-            }
-            const uint32_t vr = sub_mod(v, md.reduce(s), p);
-            if (__any(vr != 0)) {
-                atomicMin(&sh_best, r);
-                break;
-            }
-        }
-        __syncthreads();
-        const int32_t rs = __builtin_amdgcn_readfirstlane(sh_best);
-        if (rs == INT_MAX) break;
+        // 1. next candidates: non-pivot rows from cursor on (wave 0)
         if (wv == 0) {
-            const uint32_t v = lane < bw ? a[(int64_t)rs * ld + c0 + lane] : 0u;
-            uint64_t s = 0, ts = 0;
-            for (int t = 0; t < kk; ++t) {
-                const uint32_t ct = __shfl(v, ecol[t] - c0, 64);
-                s += (uint64_t)md.mul(ct, E[t][lane]);
-                ts += (uint64_t)md.mul(ct, T[t][lane]);
+            int nc = 0;
+            int32_t r0 = cursor, next = m;
+            while (nc < PB && r0 < m) {
+                const int32_t r = r0 + lane;
+                const bool ok = r < m && rowpiv[r] < 0;
+                const uint64_t bal = __ballot(ok);
+                const int cnt = __popcll(bal);
+                const int before = __popcll(bal & ((1ull << lane) - 1ull));
+                const int room = PB - nc;
+                if (ok && before < room) crow[nc + before] = r;
+                if (cnt > room) {
+                    // first candidate that did not fit starts the next tile
+                    const uint64_t left = bal;
+                    uint64_t msk = left;
+                    for (int q = 0; q < room; ++q) msk &= msk - 1ull;
+                    next = r0 + __ffsll((unsigned long long)msk) - 1;
+                    nc = PB;
+                    break;
+                }
+                nc += cnt;
+                r0 += 64;
+                next = r0 < m ? r0 : m;
             }
-            const uint32_t vr = sub_mod(v, md.reduce(s), p);
-            const uint64_t nzm = __ballot(vr != 0);
-            const int cs = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)nzm) - 1);
-            const uint32_t inv = inv_mod(__shfl(vr, cs, 64), p);
-            E[kk][lane] = md.mul(vr, inv);
-            // T row of the new pivot: (e_new - sum_t ct T[t]) * inv
-            uint32_t tn = sub_mod(0, md.reduce(ts), p);
-            if (lane == kk) tn = 1;
-            T[kk][lane] = md.mul(tn, inv);
-            ecol[kk] = c0 + cs;
-            erow[kk] = rs;
-            sh_cstar = cs;
+            sh_ncand = nc;
+            sh_next = next;
         }
         __syncthreads();
-        // keep E reduced: eliminate the new pivot column from older rows
-        // (factors read into F first: the j == cs thread overwrites E[t][cs])
-        const int cstar = __builtin_amdgcn_readfirstlane(sh_cstar);
-        if (tid < kk) F[tid] = E[tid][cstar];
-        __syncthreads();
-        for (int idx = tid; idx < kk * PB; idx += 256) {
-            const int t = idx >> 6, j = idx & 63;
-            const uint32_t f = F[t];
-            if (f == 0) continue;
-            E[t][j] = sub_mod(E[t][j], md.mul(f, E[kk][j]), p);
-            T[t][j] = sub_mod(T[t][j], md.mul(f, T[kk][j]), p);
+        const int ncand = __builtin_amdgcn_readfirstlane(sh_ncand);
+        cursor = __builtin_amdgcn_readfirstlane(sh_next);
+        if (ncand == 0) break;
+        any = true;
+        // 2. load the tile (transform part zero), reset the used flags
+        for (int idx = tid; idx < PB * 2 * PB; idx += 256) {
+            const int i = idx >> 7, col = idx & 127;
+            uint32_t v = 0;
+            if (i < ncand && col < bw) v = a[(int64_t)crow[i] * ld + c0 + col];
+            C[i][col] = v;
         }
+        if (tid < PB) used[tid] = 0;
         __syncthreads();
-        sh_k = kk + 1;
-        cursor = rs + 1;
-        __syncthreads();
-        if (__builtin_amdgcn_readfirstlane(sh_k) == bw) break;
+        // 3. reduce the tile by E
+        if (kk > 0) {
+            for (int idx = tid; idx < ncand * kk; idx += 256) {
+                const int i = idx / kk, t = idx - i * kk;
+                G[i][t] = C[i][ecol[t] - c0];
+            }
+            __syncthreads();
+            for (int idx = tid; idx < ncand * 2 * PB; idx += 256) {
+                const int i = idx >> 7, col = idx & 127;
+                uint64_t acc = 0;
+                for (int t = 0; t < kk; ++t) acc += (uint64_t)md.mul(G[i][t], E[t][col]);
+                C[i][col] = sub_mod(C[i][col], md.reduce(acc), p);
+            }
+            __syncthreads();
+        }
+        // 4. column by column
+        for (int j = 0; j < bw && kk < bw; ++j) {
+            if ((pivmask >> j) & 1ull) continue;
+            if (wv == 0) {
+                const bool nz = lane < ncand && !used[lane] && C[lane][j] != 0u;
+                const uint64_t bal = __ballot(nz);
+                const int is = bal ? __ffsll((unsigned long long)bal) - 1 : -1;
+                // every lane of wave 0 computes and stores the same values
+                sh_istar = is;
+                sh_inv = is >= 0 ? inv_mod(C[is][j], p) : 0u;
+            }
+            __syncthreads();
+            const int is = __builtin_amdgcn_readfirstlane(sh_istar);
+            if (is < 0) continue;
+            const uint32_t inv = __builtin_amdgcn_readfirstlane(sh_inv);
+            // normalise the pivot row; its own transform slot is kk
+            for (int col = tid; col < 2 * PB; col += 256) {
+                if (col < PB) {
+                    if (col >= j) C[is][col] = md.mul(C[is][col], inv);
+                } else {
+                    const int sl = col - PB;
+                    if (sl < kk) C[is][col] = md.mul(C[is][col], inv);
+                    else if (sl == kk) C[is][col] = inv;
+                }
+            }
+            // elimination factors: E rows, then the other tile rows
+            for (int idx = tid; idx < 2 * PB; idx += 256) {
+                if (idx < PB) F[idx] = idx < kk ? E[idx][j] : 0u;
+                else {
+                    const int r = idx - PB;
+                    F[idx] = (r < ncand && r != is && !used[r]) ? C[r][j] : 0u;
+                }
+            }
+            __syncthreads();
+            const int npan = bw - j, ntr = kk + 1, nwid = npan + ntr;
+            for (int idx = tid; idx < (kk + ncand) * nwid; idx += 256) {
+                const int rr = idx / nwid, cc = idx - rr * nwid;
+                const uint32_t f = rr < kk ? F[rr] : F[PB + rr - kk];
+                if (f == 0u) continue;
+                const int col = cc < npan ? j + cc : PB + (cc - npan);
+                uint32_t *x = rr < kk ? &E[rr][col] : &C[rr - kk][col];
+                *x = sub_mod(*x, md.mul(f, C[is][col]), p);
+            }
+            __syncthreads();
+            for (int col = tid; col < 2 * PB; col += 256) E[kk][col] = C[is][col];
+            ecol[kk] = c0 + j;  // same values from every thread
+            erow[kk] = crow[is];
+            used[is] = 1;
+            __syncthreads();
+            ++kk;
+            pivmask |= 1ull << j;
+        }
+        if (kk == bw || cursor >= m) break;
     }
-    const int kk = __builtin_amdgcn_readfirstlane(sh_k);
     if (tid < kk) {
         pan->erow[tid] = erow[tid];
         pan->ecol[tid] = ecol[tid];
         rowpiv[erow[tid]] = ecol[tid];
         colpiv[ecol[tid]] = erow[tid];
     }
-    for (int i = tid; i < PB * PB; i += 256) pan->T[i] = (&T[0][0])[i];
+    for (int i = tid; i < PB * PB; i += 256) {
+        const int t = i >> 6, sl = i & 63;
+        pan->T[i] = (t < kk && sl < kk) ? E[t][PB + sl] : 0u;
+    }
     if (tid == 0) {
         pan->k = kk;
-        if (kk == 0 && !sh_any) pan->done = 1;
+        if (!any) pan->done = 1;
     }
 }
 
