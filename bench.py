@@ -39,6 +39,11 @@ P = 65521
 NR, NC, NPIV = 20000, 30000, 12000
 SEED = 2024
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_sparse_blocked launch on this workload, from rocprofv3
+# FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), each scaled by what the same
+# counter reads for a plain 256 MiB device copy; the commands and raw counts
+# are in profiles/k_sparse_blocked_f4_20000x30000_p65521.md
+PMC_TRAFFIC_BYTES = 3.11e9 + 0.770e9
 
 
 def _sync():
@@ -136,7 +141,7 @@ def main(argv=None) -> int:
                    "rows": NR, "cols": NC, "known_pivots": NPIV, "to_reduce": nrl,
                    "p": P, "seed": SEED, "nonzero_remainders": nd},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": PMC_TRAFFIC_BYTES,
                      "kernel": "k_sparse_blocked", "kernel_ms": kavg,
                      "bytes_per_launch": bytes_per_launch,
                      "pivots_applied": napp, "pivot_entries": nent},
