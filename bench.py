@@ -171,21 +171,24 @@ def main(argv=None) -> int:
             extra[f"la_option1_{name}_dense_ms"] = ctx.phase_ms(1)
             extra["la_option1_new_pivots"] = npiv
             d2.free()
-        a = synth.dense_block(4096, 8192, P, seed=3)
-        for form, name in ((native.FORM_PLAIN, "plain"), (native.FORM_MFMA, "mfma")):
-            dd = ctx.dense(a, P)
-            dd.rref(form)
-            dd.free()
-            dd = ctx.dense(a, P)
-            _sync()
-            t0 = time.perf_counter()
-            r = dd.rref(form)
-            _sync()
-            td = time.perf_counter() - t0
-            extra[f"dense_4096x8192_{name}_ms"] = 1000.0 * td
-            extra[f"dense_4096x8192_{name}_gops"] = dense_ops(4096, 8192, r) / td / 1e9
-            extra["dense_4096x8192_rank"] = r
-            dd.free()
+        # dense block elimination (4096 x 8192, full rank and rank 3000)
+        for dp, rank in ((251, None), (P, None), (2147483629, None), (P, 3000)):
+            a = synth.dense_block(4096, 8192, dp, seed=3, rank=rank)
+            tag = f"dense_4096x8192_p{dp}" + ("" if rank is None else f"_rank{rank}")
+            for form, name in ((native.FORM_PLAIN, "plain"), (native.FORM_MFMA, "mfma")):
+                dd = ctx.dense(a, dp)
+                dd.rref(form)
+                dd.free()
+                dd = ctx.dense(a, dp)
+                _sync()
+                t0 = time.perf_counter()
+                r = dd.rref(form)
+                _sync()
+                td = time.perf_counter() - t0
+                extra[f"{tag}_{name}_ms"] = 1000.0 * td
+                extra[f"{tag}_{name}_gops"] = dense_ops(4096, 8192, r) / td / 1e9
+                extra[f"{tag}_rank"] = r
+                dd.free()
         out["extra"] = extra
 
     print(json.dumps(out))

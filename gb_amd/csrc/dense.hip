@@ -34,7 +34,10 @@ constexpr int PB = 64;  // panel width
 
 struct Panel {
     int32_t k;           // pivots found in this panel
-    int32_t done;        // set once no non-pivot row is left
+    int32_t done;        // set once no candidate row is left
+    int32_t head;        // cand[0, head) are pivot rows, cand[head, count) candidates
+    int32_t count;
+    int32_t nzero;       // candidates the last select found zero on its panel
     int32_t erow[PB];    // selected rows
     int32_t ecol[PB];    // their pivot columns (absolute)
     uint32_t T[PB * PB]; // Rp[t] = sum_s T[t][s] * A[erow[s]]
@@ -58,94 +61,69 @@ __device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p) 
 
 // ------------------------------------------------------------ select
 //
-// One workgroup finds the panel's pivots.  Non-pivot rows are taken in
-// order, up to 64 at a time, into an LDS tile; every tile row carries
-// [its 64 panel entries | its transform over the selected rows].  The tile
-// is first reduced by the pivot rows already found (E, reduced echelon
-// within the panel), then eliminated column by column: for each panel
-// column without a pivot, any tile row with a nonzero entry there becomes
-// the pivot (normalised), and that column is cleared from every other tile
-// row and from E.  A tile row that is not selected ends up zero on the
-// whole panel (it lies in the span of E) and is dropped.  Updates touch
-// only the columns that can be nonzero: panel columns >= j and transform
-// slots <= k.  The selected rows' transform T satisfies
-// E = T * A[selected rows, panel], so Rp = T * A[selected rows, c0:n] is
-// in reduced form with an identity on the panel's pivot columns.
+// One workgroup finds the panel's pivots.  Candidate rows (rows that are no
+// pivot yet, kept in the list cand[head, count) in any order) are taken up
+// to 64 at a time into an LDS tile; every tile row carries [its 64 panel
+// entries | its transform over the selected rows].  The tile is first
+// reduced by the pivot rows already found (E, reduced echelon within the
+// panel).  Then, repeatedly, the leftmost column in which an unselected
+// tile row is nonzero is the next pivot column: that row is normalised,
+// the column is cleared from every other tile row and from E, and the row
+// joins E.  Tile rows that are never selected end up zero on the panel.
+// Updates touch only what can be nonzero: panel columns >= the pivot
+// column and transform slots <= k.  The selected rows' transform T gives
+// E = T * A[selected rows, panel], so Rp = T * A[selected rows, c0:n] is in
+// reduced form with an identity on the panel's pivot columns.  Selected
+// rows are swapped to the front of the candidate list.
 
 constexpr int XW = 2 * PB + 1;  // padded tile row: panel part, transform part
 
-__global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t ld, int32_t m,
-                                                      int32_t n, int32_t c0, int32_t *rowpiv,
+__global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t ld, int32_t n,
+                                                      int32_t c0, int32_t *cand, int32_t *rowpiv,
                                                       int32_t *colpiv, Panel *pan, Mod md) {
     __shared__ uint32_t E[PB][XW];
     __shared__ uint32_t C[PB][XW];
     __shared__ uint32_t G[PB][PB + 1];
     __shared__ uint32_t F[2 * PB];
-    __shared__ int32_t ecol[PB], erow[PB], crow[PB], used[PB];
-    __shared__ int32_t sh_ncand, sh_next, sh_istar;
+    __shared__ int32_t ecol[PB], erow[PB], epos[PB], crow[PB], used[PB];
+    __shared__ int32_t sh_js, sh_is;
     __shared__ uint32_t sh_inv;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t p = md.p;
-    if (pan->done) {
+    const int32_t done = __builtin_amdgcn_readfirstlane(pan->done);
+    if (done) {
         if (tid == 0) pan->k = 0;
         return;
     }
+    const int32_t head = __builtin_amdgcn_readfirstlane(pan->head);
+    const int32_t count = __builtin_amdgcn_readfirstlane(pan->count);
     const int bw = min(PB, n - c0);
-    int kk = 0;            // pivots found (same value in every thread)
-    uint64_t pivmask = 0;  // panel columns that have a pivot
-    int32_t cursor = 0;
-    bool any = false;
-    for (;;) {
-        // 1. next candidates: non-pivot rows from cursor on (wave 0)
-        if (wv == 0) {
-            int nc = 0;
-            int32_t r0 = cursor, next = m;
-            while (nc < PB && r0 < m) {
-                const int32_t r = r0 + lane;
-                const bool ok = r < m && rowpiv[r] < 0;
-                const uint64_t bal = __ballot(ok);
-                const int cnt = __popcll(bal);
-                const int before = __popcll(bal & ((1ull << lane) - 1ull));
-                const int room = PB - nc;
-                if (ok && before < room) crow[nc + before] = r;
-                if (cnt > room) {
-                    // first candidate that did not fit starts the next tile
-                    const uint64_t left = bal;
-                    uint64_t msk = left;
-                    for (int q = 0; q < room; ++q) msk &= msk - 1ull;
-                    next = r0 + __ffsll((unsigned long long)msk) - 1;
-                    nc = PB;
-                    break;
-                }
-                nc += cnt;
-                r0 += 64;
-                next = r0 < m ? r0 : m;
-            }
-            sh_ncand = nc;
-            sh_next = next;
+    int kk = 0;  // pivots found (same value in every thread)
+    int nzero = 0;
+    int32_t pos = head;
+    while (pos < count && kk < bw) {
+        const int nc = min(PB, count - pos);
+        if (tid < PB) {
+            crow[tid] = tid < nc ? cand[pos + tid] : 0;
+            used[tid] = tid < nc ? 0 : 1;
         }
         __syncthreads();
-        const int ncand = __builtin_amdgcn_readfirstlane(sh_ncand);
-        cursor = __builtin_amdgcn_readfirstlane(sh_next);
-        if (ncand == 0) break;
-        any = true;
-        // 2. load the tile (transform part zero), reset the used flags
+        // 1. load the tile (transform part zero)
         for (int idx = tid; idx < PB * 2 * PB; idx += 256) {
             const int i = idx >> 7, col = idx & 127;
             uint32_t v = 0;
-            if (i < ncand && col < bw) v = a[(int64_t)crow[i] * ld + c0 + col];
+            if (i < nc && col < bw) v = a[(int64_t)crow[i] * ld + c0 + col];
             C[i][col] = v;
         }
-        if (tid < PB) used[tid] = 0;
         __syncthreads();
-        // 3. reduce the tile by E
+        // 2. reduce the tile by E
         if (kk > 0) {
-            for (int idx = tid; idx < ncand * kk; idx += 256) {
+            for (int idx = tid; idx < nc * kk; idx += 256) {
                 const int i = idx / kk, t = idx - i * kk;
                 G[i][t] = C[i][ecol[t] - c0];
             }
             __syncthreads();
-            for (int idx = tid; idx < ncand * 2 * PB; idx += 256) {
+            for (int idx = tid; idx < nc * 2 * PB; idx += 256) {
                 const int i = idx >> 7, col = idx & 127;
                 uint64_t acc = 0;
                 for (int t = 0; t < kk; ++t) acc += (uint64_t)md.mul(G[i][t], E[t][col]);
@@ -153,25 +131,41 @@ __global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t
             }
             __syncthreads();
         }
-        // 4. column by column
-        for (int j = 0; j < bw && kk < bw; ++j) {
-            if ((pivmask >> j) & 1ull) continue;
+        // 3. pivots of the tile, leftmost column first.  Wave 0 keeps, per
+        // lane i, the first nonzero panel column of tile row i (PB: none or
+        // selected); columns left of the last pivot are zero in every
+        // unselected row, so only rows hit by an elimination rescan.
+        int fc = PB;
+        if (wv == 0 && lane < nc) {
+            for (int c = 0; c < bw; ++c)
+                if (C[lane][c] != 0u) { fc = c; break; }
+        }
+        bool exhausted = false;
+        for (;;) {
             if (wv == 0) {
-                const bool nz = lane < ncand && !used[lane] && C[lane][j] != 0u;
-                const uint64_t bal = __ballot(nz);
-                const int is = bal ? __ffsll((unsigned long long)bal) - 1 : -1;
-                // every lane of wave 0 computes and stores the same values
-                sh_istar = is;
-                sh_inv = is >= 0 ? inv_mod(C[is][j], p) : 0u;
+                int mn = fc;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) mn = min(mn, __shfl_xor(mn, d, 64));
+                const uint64_t bal = __ballot(fc == mn);
+                const int is = __ffsll((unsigned long long)bal) - 1;
+                const int js = mn;
+                // every lane of wave 0 stores the same values
+                sh_js = js;
+                sh_is = is;
+                sh_inv = js < bw ? inv_mod(C[is][js], p) : 0u;
             }
             __syncthreads();
-            const int is = __builtin_amdgcn_readfirstlane(sh_istar);
-            if (is < 0) continue;
+            const int js = __builtin_amdgcn_readfirstlane(sh_js);
+            if (js >= bw) {
+                exhausted = true;
+                break;
+            }
+            const int is = __builtin_amdgcn_readfirstlane(sh_is);
             const uint32_t inv = __builtin_amdgcn_readfirstlane(sh_inv);
             // normalise the pivot row; its own transform slot is kk
             for (int col = tid; col < 2 * PB; col += 256) {
                 if (col < PB) {
-                    if (col >= j) C[is][col] = md.mul(C[is][col], inv);
+                    if (col >= js) C[is][col] = md.mul(C[is][col], inv);
                 } else {
                     const int sl = col - PB;
                     if (sl < kk) C[is][col] = md.mul(C[is][col], inv);
@@ -180,74 +174,198 @@ __global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t
             }
             // elimination factors: E rows, then the other tile rows
             for (int idx = tid; idx < 2 * PB; idx += 256) {
-                if (idx < PB) F[idx] = idx < kk ? E[idx][j] : 0u;
+                if (idx < PB) F[idx] = idx < kk ? E[idx][js] : 0u;
                 else {
                     const int r = idx - PB;
-                    F[idx] = (r < ncand && r != is && !used[r]) ? C[r][j] : 0u;
+                    F[idx] = (r < nc && r != is && !used[r]) ? C[r][js] : 0u;
                 }
             }
             __syncthreads();
-            const int npan = bw - j, ntr = kk + 1, nwid = npan + ntr;
-            for (int idx = tid; idx < (kk + ncand) * nwid; idx += 256) {
+            const int npan = bw - js, nwid = npan + kk + 1;
+            for (int idx = tid; idx < (kk + nc) * nwid; idx += 256) {
                 const int rr = idx / nwid, cc = idx - rr * nwid;
                 const uint32_t f = rr < kk ? F[rr] : F[PB + rr - kk];
                 if (f == 0u) continue;
-                const int col = cc < npan ? j + cc : PB + (cc - npan);
+                const int col = cc < npan ? js + cc : PB + (cc - npan);
                 uint32_t *x = rr < kk ? &E[rr][col] : &C[rr - kk][col];
                 *x = sub_mod(*x, md.mul(f, C[is][col]), p);
             }
             __syncthreads();
             for (int col = tid; col < 2 * PB; col += 256) E[kk][col] = C[is][col];
-            ecol[kk] = c0 + j;  // same values from every thread
+            // same values from every thread
+            ecol[kk] = c0 + js;
             erow[kk] = crow[is];
+            epos[kk] = pos + is;
             used[is] = 1;
             __syncthreads();
+            if (wv == 0 && lane < nc) {
+                if (lane == is) fc = PB;
+                else if (fc == js) {
+                    fc = PB;
+                    for (int c = js + 1; c < bw; ++c)
+                        if (C[lane][c] != 0u) { fc = c; break; }
+                }
+            }
             ++kk;
-            pivmask |= 1ull << j;
+            if (kk == bw) break;
         }
-        if (kk == bw || cursor >= m) break;
+        if (exhausted) {
+            // rows of this tile that were not selected are zero on the panel
+            int sel = 0;
+            for (int t = 0; t < kk; ++t) sel += (epos[t] >= pos && epos[t] < pos + nc) ? 1 : 0;
+            nzero += nc - sel;
+        }
+        pos += nc;
     }
+    __syncthreads();
     if (tid < kk) {
-        pan->erow[tid] = erow[tid];
-        pan->ecol[tid] = ecol[tid];
         rowpiv[erow[tid]] = ecol[tid];
         colpiv[ecol[tid]] = erow[tid];
+        pan->erow[tid] = erow[tid];
+        pan->ecol[tid] = ecol[tid];
     }
     for (int i = tid; i < PB * PB; i += 256) {
         const int t = i >> 6, sl = i & 63;
         pan->T[i] = (t < kk && sl < kk) ? E[t][PB + sl] : 0u;
     }
     if (tid == 0) {
+        // move the selected rows to cand[head, head + kk): an unselected row
+        // from the front takes the place of each selected row further back
+        uint64_t front = 0;
+        for (int t = 0; t < kk; ++t) {
+            const int d = epos[t] - head;
+            if (d < kk) front |= 1ull << d;
+        }
+        int u = 0;
+        for (int t = 0; t < kk; ++t) {
+            const int d = epos[t] - head;
+            if (d >= kk) {
+                while ((front >> u) & 1ull) ++u;
+                cand[epos[t]] = cand[head + u];
+                ++u;
+            }
+        }
+        for (int t = 0; t < kk; ++t) cand[head + t] = erow[t];
         pan->k = kk;
-        if (!any) pan->done = 1;
+        pan->head = head + kk;
+        pan->nzero = nzero;
+        if (head + kk >= count) pan->done = 1;
+    }
+}
+
+// ---------------------------------------------------------------- prune
+//
+// After a panel's update, a candidate row whose columns [c1, n) are all
+// zero is zero for good (its columns left of c1 are zero already): when the
+// last select met at least a tile's worth of rows that were zero on its
+// panel, such rows are dropped from the candidate list.
+__global__ void k_prune_flags(const uint32_t *a, int64_t ld, int32_t n, int32_t c1,
+                              const int32_t *cand, const Panel *pan, int32_t *alive) {
+    const int32_t nzero = __builtin_amdgcn_readfirstlane(pan->nzero);
+    const int32_t done = __builtin_amdgcn_readfirstlane(pan->done);
+    if (nzero < PB || done) return;
+    const int32_t head = __builtin_amdgcn_readfirstlane(pan->head);
+    const int32_t count = __builtin_amdgcn_readfirstlane(pan->count);
+    const int lane = threadIdx.x & 63;
+    const int32_t step = gridDim.x * (blockDim.x >> 6);
+    for (int32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); head + w < count;
+         w += step) {
+        const uint32_t *row = a + (int64_t)cand[head + w] * ld;
+        int nz = 0;  // uniform over the wave
+        for (int32_t cb = c1; cb < n; cb += 64) {
+            const int32_t c = cb + lane;
+            if (__any(c < n && row[c] != 0u)) {
+                nz = 1;
+                break;
+            }
+        }
+        alive[w] = nz;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_prune_compact(int32_t *cand, Panel *pan,
+                                                       const int32_t *alive) {
+    __shared__ int32_t wsum[4];
+    const int32_t nzero = __builtin_amdgcn_readfirstlane(pan->nzero);
+    const int32_t done = __builtin_amdgcn_readfirstlane(pan->done);
+    if (nzero < PB || done) return;
+    const int32_t head = __builtin_amdgcn_readfirstlane(pan->head);
+    const int32_t count = __builtin_amdgcn_readfirstlane(pan->count);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int32_t out = head;
+    // stable in-place compaction: writes never pass the reads of later chunks
+    for (int32_t base = head; base < count; base += 256) {
+        const int32_t i = base + tid;
+        const int f = i < count ? alive[i - head] : 0;
+        const int32_t v = i < count ? cand[i] : 0;
+        const int incl = wave_incl_scan(f);
+        wsum[wv] = __shfl(incl, 63, 64);  // every lane stores the wave total
+        __syncthreads();
+        int off = 0, tot = 0;
+        for (int q = 0; q < 4; ++q) {
+            const int ws = wsum[q];
+            if (q < wv) off += ws;
+            tot += ws;
+        }
+        if (f) cand[out + off + incl - 1] = v;
+        out += tot;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        pan->count = out;
+        pan->nzero = 0;
+        if (out <= head) pan->done = 1;
     }
 }
 
 // ---------------------------------------------------------------- rp
 
-// one thread per column j >= c0: Rp[t][j] = sum_s T[t][s] A[erow[s]][j]
+// Rp[t][j] = sum_s T[t][s] A[erow[s]][j] for j >= c0: a block covers 64
+// columns, each wave 16 of the rows t; the 64 x 64 panel of the selected
+// rows is staged in LDS.  Products are summed in 128 bits (lo + carries)
+// and reduced once per output.
 __global__ __launch_bounds__(256) void k_panel_rp(uint32_t *a, int64_t ld, int32_t n, int32_t c0,
                                                   const Panel *pan, uint32_t *rp, int64_t ldr,
-                                                  Mod md) {
-    const int kk = pan->k;
+                                                  Mod md, uint32_t r64) {
+    const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
-    __shared__ uint32_t T[PB][PB];
-    __shared__ int32_t erow[PB];
-    for (int i = threadIdx.x; i < PB * PB; i += blockDim.x) (&T[0][0])[i] = pan->T[i];
-    if (threadIdx.x < PB) erow[threadIdx.x] = pan->erow[threadIdx.x];
+    __shared__ uint32_t T[PB][PB + 1];
+    __shared__ uint32_t V[PB][PB + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int32_t jb = c0 + blockIdx.x * 64;
+    for (int i = tid; i < PB * PB; i += 256) T[i >> 6][i & 63] = pan->T[i];
+    for (int i = tid; i < PB * PB; i += 256) {
+        const int s_ = i >> 6, c = i & 63;
+        V[s_][c] = (s_ < kk && jb + c < n) ? a[(int64_t)pan->erow[s_] * ld + jb + c] : 0u;
+    }
     __syncthreads();
-    const int32_t j = c0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    uint32_t v[PB];
+    const int32_t j = jb + lane;
+    uint32_t res[16];
 #pragma unroll
-    for (int s = 0; s < PB; ++s) v[s] = s < kk ? a[(int64_t)erow[s] * ld + j] : 0u;
-    for (int t = 0; t < kk; ++t) {
-        uint64_t acc = 0;
+    for (int q = 0; q < 16; ++q) {
+        const int t = 16 * wv + q;
+        uint64_t lo = 0;
+        uint32_t hi = 0;
+        for (int s_ = 0; s_ < kk; ++s_) {
+            const uint64_t pr = (uint64_t)T[t][s_] * (uint64_t)V[s_][lane];
+            const uint64_t o = lo;
+            lo += pr;
+            hi += lo < o;
+        }
+        uint32_t r = md.reduce(lo);
+        if (hi) r = (uint32_t)((r + (uint64_t)md.mul(hi, r64)) % md.p);
+        res[q] = r;
+    }
+    __syncthreads();  // all reads of the selected rows are done before any write
+    if (j < n) {
 #pragma unroll
-        for (int s = 0; s < PB; ++s) acc += (uint64_t)md.mul(T[t][s], v[s]);
-        const uint32_t r = md.reduce(acc);
-        rp[(int64_t)t * ldr + (j - c0)] = r;
-        a[(int64_t)erow[t] * ld + j] = r;
+        for (int q = 0; q < 16; ++q) {
+            const int t = 16 * wv + q;
+            if (t < kk) {
+                rp[(int64_t)t * ldr + (j - c0)] = res[q];
+                a[(int64_t)pan->erow[t] * ld + j] = res[q];
+            }
+        }
     }
 }
 
@@ -258,18 +376,18 @@ __global__ __launch_bounds__(256) void k_panel_rp(uint32_t *a, int64_t ld, int32
 __global__ void k_panel_gather(const uint32_t *a, int64_t ld, int32_t m, int32_t c0, int32_t bw,
                                const int32_t *rowpiv, const Panel *pan, uint32_t *L,
                                int32_t *active) {
-    const int kk = pan->k;
+    const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
     const int lane = threadIdx.x & 63;
-    const int32_t r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (r >= m) return;
-    const int32_t rp = rowpiv[r];
-    const bool cur = rp >= c0 && rp < c0 + bw;
-    uint32_t v = 0;
-    if (!cur && lane < kk) v = a[(int64_t)r * ld + pan->ecol[lane]];
-    L[(int64_t)r * PB + lane] = v;
-    const bool any = __any(v != 0);
-    if (lane == 0) active[r] = any ? 1 : 0;
+    const int32_t step = gridDim.x * (blockDim.x >> 6);
+    for (int32_t r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < m; r += step) {
+        const int32_t rp = rowpiv[r];
+        const bool cur = rp >= c0 && rp < c0 + bw;
+        uint32_t v = 0;
+        if (!cur && lane < kk) v = a[(int64_t)r * ld + pan->ecol[lane]];
+        L[(int64_t)r * PB + lane] = v;
+        active[r] = __any(v != 0) ? 1 : 0;  // every lane stores the same flag
+    }
 }
 
 // ------------------------------------------------- update (integer ALU)
@@ -469,10 +587,25 @@ __global__ void k_out_list(int32_t n, const int32_t *colpiv, const int32_t *flag
         }
 }
 
-__global__ void k_fill_dense(int32_t *p, int64_t n, int32_t v) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = v;
+// rowpiv = colpiv = -1, cand = 0, 1, ..., m - 1, panel state reset
+__global__ void k_dense_init(int32_t m, int32_t n, int32_t *rowpiv, int32_t *colpiv,
+                             int32_t *cand, Panel *pan) {
+    const int64_t tot = (int64_t)m > n ? m : n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (i < m) {
+            rowpiv[i] = -1;
+            cand[i] = (int32_t)i;
+        }
+        if (i < n) colpiv[i] = -1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        pan->k = 0;
+        pan->done = m == 0 ? 1 : 0;
+        pan->head = 0;
+        pan->count = m;
+        pan->nzero = 0;
+    }
 }
 
 static int grid_of(int64_t n, int bs) {
@@ -492,33 +625,29 @@ int dense_rref(gbgpu_dense *d, int form) {
     dfree(d->orow);
     dfree(d->olead);
     d->rank = 0;
-    int32_t *rowpiv = nullptr, *colpiv = nullptr, *active = nullptr;
+    int32_t *rowpiv = nullptr, *colpiv = nullptr, *active = nullptr, *cand = nullptr;
     uint32_t *L = nullptr, *rp = nullptr;
     Panel *pan = nullptr;
     const int64_t ldr = ((int64_t)n + 63) & ~63ll;
     auto cleanup = [&]() {
-        dfree(rowpiv); dfree(colpiv); dfree(active); dfree(L); dfree(rp); dfree(pan);
+        dfree(rowpiv); dfree(colpiv); dfree(active); dfree(cand); dfree(L); dfree(rp); dfree(pan);
     };
     if ((rc = dalloc(ctx, &rowpiv, (size_t)m)) || (rc = dalloc(ctx, &colpiv, (size_t)n)) ||
-        (rc = dalloc(ctx, &active, (size_t)m)) || (rc = dalloc(ctx, &L, (size_t)m * PB)) ||
-        (rc = dalloc(ctx, &rp, (size_t)PB * ldr)) || (rc = dalloc(ctx, &pan, 1))) {
+        (rc = dalloc(ctx, &active, (size_t)m)) || (rc = dalloc(ctx, &cand, (size_t)m)) ||
+        (rc = dalloc(ctx, &L, (size_t)m * PB)) || (rc = dalloc(ctx, &rp, (size_t)PB * ldr)) ||
+        (rc = dalloc(ctx, &pan, 1))) {
         cleanup();
         return rc;
     }
-    if (m > 0) k_fill_dense<<<grid_of(m, 256), 256, 0, s>>>(rowpiv, m, -1);
-    if (n > 0) k_fill_dense<<<grid_of(n, 256), 256, 0, s>>>(colpiv, n, -1);
-    if (hipMemsetAsync(pan, 0, sizeof(Panel), s) != hipSuccess) {
-        cleanup();
-        return check_hip(ctx, hipGetLastError(), "hipMemsetAsync");
-    }
+    k_dense_init<<<grid_of(m > n ? m : n, 256), 256, 0, s>>>(m, n, rowpiv, colpiv, cand, pan);
     const uint32_t r64 = (uint32_t)((((unsigned __int128)1) << 64) % d->p);
     if (hipEventRecord(ctx->ev[2], s) != hipSuccess) { cleanup(); return check_hip(ctx, hipGetLastError(), "event"); }
     int32_t pan_done = 0;
     int npan = 0;
     for (int32_t c0 = 0; c0 < n && m > 0; c0 += PB) {
         const int bw = (n - c0) < PB ? (n - c0) : PB;
-        k_panel_select<<<1, 256, 0, s>>>(d->a, d->ld, m, n, c0, rowpiv, colpiv, pan, md);
-        k_panel_rp<<<grid_of(n - c0, 256), 256, 0, s>>>(d->a, d->ld, n, c0, pan, rp, ldr, md);
+        k_panel_select<<<1, 256, 0, s>>>(d->a, d->ld, n, c0, cand, rowpiv, colpiv, pan, md);
+        k_panel_rp<<<grid_of(n - c0, 64), 256, 0, s>>>(d->a, d->ld, n, c0, pan, rp, ldr, md, r64);
         k_panel_gather<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, m, c0, bw, rowpiv, pan, L, active);
         if (form == GBGPU_FORM_MFMA) {
             dim3 g((unsigned)((n - c0 + 63) / 64), (unsigned)((m + MR - 1) / MR));
@@ -528,6 +657,10 @@ int dense_rref(gbgpu_dense *d, int form) {
             dim3 g((unsigned)((n - c0 + 255) / 256), (unsigned)((m + RT - 1) / RT));
             if (big) k_update_plain<true><<<g, 256, 0, s>>>(d->a, d->ld, m, n, c0, pan, rp, ldr, L, active, md, r64);
             else k_update_plain<false><<<g, 256, 0, s>>>(d->a, d->ld, m, n, c0, pan, rp, ldr, L, active, md, r64);
+        }
+        if (c0 + bw < n) {
+            k_prune_flags<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, n, c0 + bw, cand, pan, active);
+            k_prune_compact<<<1, 256, 0, s>>>(cand, pan, active);
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { cleanup(); return check_hip(ctx, e, "dense panel launch"); }

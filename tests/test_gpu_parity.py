@@ -83,6 +83,28 @@ def test_dense_rref(gpu_lib, p, form, shape, rank):
     assert np.array_equal(got, exp)
 
 
+@pytest.mark.parametrize("p", [251, 65521, 2147483647])
+@pytest.mark.parametrize("form", FORMS)
+def test_dense_rref_structured(gpu_lib, p, form):
+    """Rows zero on the first panels, dead (all-zero) rows and dependent
+    rows: exercises candidate tiles that find no pivot and the pruning of
+    rows that are zero for good."""
+    rng = np.random.default_rng(4)
+    m, n = 700, 900
+    a = rng.integers(0, p, size=(m, n), dtype=np.int64)
+    a[90:, :300] = 0
+    a[300:420, :] = 0
+    a[600:680] = (a[0:80] * 3 + a[80:160] * 5) % p
+    a = a.astype(np.uint32)
+    exp, elead = oracle.dense_rref(a, p, nthreads=8)
+    d = gpu_lib.dense(a, p)
+    r = d.rref(form)
+    got, lead = d.result()
+    assert r == len(exp)
+    assert np.array_equal(lead, elead)
+    assert np.array_equal(got, exp)
+
+
 @pytest.mark.parametrize("p", [101, 65521, 2147483647])
 def test_interreduce(gpu_lib, p):
     rng = np.random.default_rng(p % 97)
