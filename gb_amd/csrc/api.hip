@@ -145,8 +145,18 @@ int gbgpu_init(int device, gbgpu_ctx **out) {
         return GBGPU_EDEVICE;
     }
     ctx->own_stream = true;
+    if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) {
+        gbgpu_finalize(ctx);
+        return GBGPU_EDEVICE;
+    }
     for (auto &e : ctx->ev) {
         if (hipEventCreate(&e) != hipSuccess) {
+            gbgpu_finalize(ctx);
+            return GBGPU_EDEVICE;
+        }
+    }
+    for (auto &e : ctx->sync_ev) {
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
             gbgpu_finalize(ctx);
             return GBGPU_EDEVICE;
         }
@@ -160,6 +170,9 @@ void gbgpu_finalize(gbgpu_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : ctx->sync_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

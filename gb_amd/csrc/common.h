@@ -70,7 +70,9 @@ struct gbgpu_ctx {
     int num_cu = 256;
     std::string err;
     double phase_ms[3] = {-1, -1, -1};
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[6] = {};          // timing events of the phases
+    hipStream_t side = nullptr;     // second stream of the dense look-ahead
+    hipEvent_t sync_ev[2] = {};     // ordering events between the two streams
 };
 
 namespace gbgpu {

@@ -34,25 +34,35 @@ constexpr int PB = 64;  // panel width
 
 struct Panel {
     int32_t k;           // pivots found in this panel
-    int32_t done;        // set once no candidate row is left
-    int32_t head;        // cand[0, head) are pivot rows, cand[head, count) candidates
-    int32_t count;
-    int32_t nzero;       // candidates the last select found zero on its panel
     int32_t erow[PB];    // selected rows
     int32_t ecol[PB];    // their pivot columns (absolute)
     uint32_t T[PB * PB]; // Rp[t] = sum_s T[t][s] * A[erow[s]]
 };
 
+// the candidate rows: cand[0, head) are pivot rows, cand[head, count)
+// candidates (in any order)
+struct ListState {
+    int32_t done;   // set once no candidate row is left
+    int32_t head;
+    int32_t count;
+    int32_t nzero;  // candidates found zero on their panel since the last prune
+};
+
 __device__ uint32_t inv_mod(uint32_t a, uint32_t p) {
-    // extended Euclid (tools.c:83-110 computes the same inverse)
-    int64_t r0 = p, r1 = a % p, s0 = 0, s1 = 1;
+    // extended Euclid on 32-bit operands (tools.c:83-110 computes the same
+    // inverse); |s| stays below p < 2^31
+    uint32_t r0 = p, r1 = a % p;
+    int32_t s0 = 0, s1 = 1;
     while (r1 != 0) {
-        const int64_t q = r0 / r1;
-        int64_t t = r0 - q * r1; r0 = r1; r1 = t;
-        t = s0 - q * s1; s0 = s1; s1 = t;
+        const uint32_t q = r0 / r1;
+        const uint32_t t = r0 - q * r1;
+        r0 = r1;
+        r1 = t;
+        const int32_t u = (int32_t)((int64_t)s0 - (int64_t)q * s1);
+        s0 = s1;
+        s1 = u;
     }
-    if (s0 < 0) s0 += p;
-    return (uint32_t)s0;
+    return (uint32_t)(s0 < 0 ? s0 + (int64_t)p : s0);
 }
 
 __device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p) {
@@ -64,70 +74,96 @@ __device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p) 
 // One workgroup finds the panel's pivots.  Candidate rows (rows that are no
 // pivot yet, kept in the list cand[head, count) in any order) are taken up
 // to 64 at a time into an LDS tile; every tile row carries [its 64 panel
-// entries | its transform over the selected rows].  The tile is first
-// reduced by the pivot rows already found (E, reduced echelon within the
-// panel).  Then, repeatedly, the leftmost column in which an unselected
-// tile row is nonzero is the next pivot column: that row is normalised,
-// the column is cleared from every other tile row and from E, and the row
-// joins E.  Tile rows that are never selected end up zero on the panel.
-// Updates touch only what can be nonzero: panel columns >= the pivot
-// column and transform slots <= k.  The selected rows' transform T gives
-// E = T * A[selected rows, panel], so Rp = T * A[selected rows, c0:n] is in
-// reduced form with an identity on the panel's pivot columns.  Selected
-// rows are swapped to the front of the candidate list.
+// entries | its transform over the selected rows] and its own coefficient.
+// The tile is first reduced by the pivot rows already found (E, normalised
+// and reduced within the panel).  Then, repeatedly, the leftmost column in
+// which an unselected tile row is nonzero is the next pivot column: the
+// row with pivot value a there joins E, and the column is cleared from
+// every other row x of the tile and of E without division,
+// x <- a x - x[col] * pivot row; so no inverse sits on the per-pivot path.
+// At the end of a tile the E rows are normalised, their inverses computed
+// in parallel.  Tile rows that are never selected end up zero on the
+// panel.  Updates touch only what can be nonzero: panel columns >= the
+// pivot column and transform slots <= k.  The selected rows' transform T
+// gives E = T * A[selected rows, panel], so Rp = T * A[selected rows, c0:n]
+// is in reduced form with an identity on the panel's pivot columns.
+// Selected rows are swapped to the front of the candidate list.
 
 constexpr int XW = 2 * PB + 1;  // padded tile row: panel part, transform part
 
-__global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t ld, int32_t n,
-                                                      int32_t c0, int32_t *cand, int32_t *rowpiv,
-                                                      int32_t *colpiv, Panel *pan, Mod md) {
+// wave 0: the smallest first-nonzero column over the lanes (tile rows) and
+// the lowest lane holding it; every lane stores the same two values
+__device__ __forceinline__ void pivot_search(int fc, int32_t &sh_js, int32_t &sh_is) {
+    int mn = fc;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) mn = min(mn, __shfl_xor(mn, d, 64));
+    const uint64_t bal = __ballot(fc == mn);
+    sh_js = mn;
+    sh_is = __ffsll((unsigned long long)bal) - 1;
+}
+constexpr int SEL_T = 1024;     // threads of the select workgroup
+constexpr int SEL_RG = SEL_T / (2 * PB);  // row groups: thread = (row group, column)
+
+__global__ __launch_bounds__(SEL_T) void k_panel_select(const uint32_t *a, int64_t ld, int32_t n,
+                                                        int32_t c0, int32_t *cand,
+                                                        int32_t *rowpiv, int32_t *colpiv,
+                                                        Panel *pan, ListState *st, Mod md) {
     __shared__ uint32_t E[PB][XW];
     __shared__ uint32_t C[PB][XW];
     __shared__ uint32_t G[PB][PB + 1];
-    __shared__ uint32_t F[2 * PB];
+    __shared__ uint32_t own[PB], einv[PB];
     __shared__ int32_t ecol[PB], erow[PB], epos[PB], crow[PB], used[PB];
     __shared__ int32_t sh_js, sh_is;
-    __shared__ uint32_t sh_inv;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int col = tid & (2 * PB - 1), rg = tid >> 7;  // this thread's column and row group
     const uint32_t p = md.p;
-    const int32_t done = __builtin_amdgcn_readfirstlane(pan->done);
+    const bool small = p < (1u << 17);  // 64 products of residues fit in 64 bits
+    const int32_t done = __builtin_amdgcn_readfirstlane(st->done);
     if (done) {
         if (tid == 0) pan->k = 0;
         return;
     }
-    const int32_t head = __builtin_amdgcn_readfirstlane(pan->head);
-    const int32_t count = __builtin_amdgcn_readfirstlane(pan->count);
+    const int32_t head = __builtin_amdgcn_readfirstlane(st->head);
+    const int32_t count = __builtin_amdgcn_readfirstlane(st->count);
     const int bw = min(PB, n - c0);
     int kk = 0;  // pivots found (same value in every thread)
     int nzero = 0;
     int32_t pos = head;
     while (pos < count && kk < bw) {
         const int nc = min(PB, count - pos);
+        const int k0 = kk;  // pivots found before this tile
         if (tid < PB) {
             crow[tid] = tid < nc ? cand[pos + tid] : 0;
             used[tid] = tid < nc ? 0 : 1;
+            own[tid] = 1u;
         }
         __syncthreads();
         // 1. load the tile (transform part zero)
-        for (int idx = tid; idx < PB * 2 * PB; idx += 256) {
-            const int i = idx >> 7, col = idx & 127;
+        for (int i = rg; i < PB; i += SEL_RG) {
             uint32_t v = 0;
             if (i < nc && col < bw) v = a[(int64_t)crow[i] * ld + c0 + col];
             C[i][col] = v;
         }
         __syncthreads();
-        // 2. reduce the tile by E
+        // 2. reduce the tile by E (normalised)
         if (kk > 0) {
-            for (int idx = tid; idx < nc * kk; idx += 256) {
-                const int i = idx / kk, t = idx - i * kk;
-                G[i][t] = C[i][ecol[t] - c0];
+            for (int idx = tid; idx < nc * PB; idx += SEL_T) {
+                const int i = idx >> 6, t = idx & 63;
+                G[i][t] = t < kk ? C[i][ecol[t] - c0] : 0u;
             }
             __syncthreads();
-            for (int idx = tid; idx < nc * 2 * PB; idx += 256) {
-                const int i = idx >> 7, col = idx & 127;
-                uint64_t acc = 0;
-                for (int t = 0; t < kk; ++t) acc += (uint64_t)md.mul(G[i][t], E[t][col]);
-                C[i][col] = sub_mod(C[i][col], md.reduce(acc), p);
+            for (int i = rg; i < nc; i += SEL_RG) {
+                uint32_t red;
+                if (small) {
+                    uint64_t acc = 0;
+                    for (int t = 0; t < kk; ++t) acc += (uint64_t)G[i][t] * E[t][col];
+                    red = md.reduce(acc);
+                } else {
+                    uint64_t acc = 0;
+                    for (int t = 0; t < kk; ++t) acc += (uint64_t)md.mul(G[i][t], E[t][col]);
+                    red = md.reduce(acc);
+                }
+                C[i][col] = sub_mod(C[i][col], red, p);
             }
             __syncthreads();
         }
@@ -136,84 +172,93 @@ __global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t
         // selected); columns left of the last pivot are zero in every
         // unselected row, so only rows hit by an elimination rescan.
         int fc = PB;
-        if (wv == 0 && lane < nc) {
-            for (int c = 0; c < bw; ++c)
-                if (C[lane][c] != 0u) { fc = c; break; }
+        if (wv == 0) {
+            if (lane < nc) {
+                for (int c = 0; c < bw; ++c)
+                    if (C[lane][c] != 0u) { fc = c; break; }
+            }
+            pivot_search(fc, sh_js, sh_is);
         }
+        __syncthreads();
         bool exhausted = false;
         for (;;) {
-            if (wv == 0) {
-                int mn = fc;
-#pragma unroll
-                for (int d = 1; d < 64; d <<= 1) mn = min(mn, __shfl_xor(mn, d, 64));
-                const uint64_t bal = __ballot(fc == mn);
-                const int is = __ffsll((unsigned long long)bal) - 1;
-                const int js = mn;
-                // every lane of wave 0 stores the same values
-                sh_js = js;
-                sh_is = is;
-                sh_inv = js < bw ? inv_mod(C[is][js], p) : 0u;
-            }
-            __syncthreads();
             const int js = __builtin_amdgcn_readfirstlane(sh_js);
             if (js >= bw) {
                 exhausted = true;
                 break;
             }
             const int is = __builtin_amdgcn_readfirstlane(sh_is);
-            const uint32_t inv = __builtin_amdgcn_readfirstlane(sh_inv);
-            // normalise the pivot row; its own transform slot is kk
-            for (int col = tid; col < 2 * PB; col += 256) {
-                if (col < PB) {
-                    if (col >= js) C[is][col] = md.mul(C[is][col], inv);
-                } else {
-                    const int sl = col - PB;
-                    if (sl < kk) C[is][col] = md.mul(C[is][col], inv);
-                    else if (sl == kk) C[is][col] = inv;
+            const uint32_t av = C[is][js];
+            const uint32_t ownv = own[is];
+            // x <- a x - f pivot for the rows with f = x[js] != 0 (column js
+            // itself is cleared in the next phase, so every thread reads f
+            // before anything changes it): E rows over all panel columns
+            // (the scaling by a reaches their columns left of js) and tile
+            // rows over columns >= js (zero further left); transform slots
+            // <= kk, where the pivot row's own coefficient sits in slot kk.
+            {
+                const bool tcol = col >= PB;
+                const int sl = col - PB;
+                const bool slot_ok = tcol && sl <= kk;
+                const uint32_t pv = tcol ? (sl == kk ? ownv : (sl < kk ? C[is][col] : 0u))
+                                         : (col < bw ? C[is][col] : 0u);
+                const bool e_ok = (col < bw || slot_ok) && col != js;
+                const bool t_ok = ((col >= js && col < bw) || slot_ok) && col != js;
+                for (int r = rg; r < kk + nc; r += SEL_RG) {
+                    const bool isE = r < kk;
+                    const int rt = r - kk;
+                    uint32_t *x = isE ? &E[r][col] : &C[rt][col];
+                    const uint32_t f = isE ? E[r][js] : ((rt != is && !used[rt]) ? C[rt][js] : 0u);
+                    if (f == 0u || !(isE ? e_ok : t_ok)) continue;
+                    // two products below 2^62 each: the sum fits in 64 bits
+                    *x = md.reduce((uint64_t)av * *x + (uint64_t)(p - f) * pv);
                 }
-            }
-            // elimination factors: E rows, then the other tile rows
-            for (int idx = tid; idx < 2 * PB; idx += 256) {
-                if (idx < PB) F[idx] = idx < kk ? E[idx][js] : 0u;
-                else {
-                    const int r = idx - PB;
-                    F[idx] = (r < nc && r != is && !used[r]) ? C[r][js] : 0u;
-                }
+                if (tid < nc && tid != is && !used[tid] && C[tid][js] != 0u)
+                    own[tid] = md.mul(av, own[tid]);
             }
             __syncthreads();
-            const int npan = bw - js, nwid = npan + kk + 1;
-            for (int idx = tid; idx < (kk + nc) * nwid; idx += 256) {
-                const int rr = idx / nwid, cc = idx - rr * nwid;
-                const uint32_t f = rr < kk ? F[rr] : F[PB + rr - kk];
-                if (f == 0u) continue;
-                const int col = cc < npan ? js + cc : PB + (cc - npan);
-                uint32_t *x = rr < kk ? &E[rr][col] : &C[rr - kk][col];
-                *x = sub_mod(*x, md.mul(f, C[is][col]), p);
+            // the pivot row joins E; column js is cleared in the other rows;
+            // wave 0 finds the next pivot (rows hit by this elimination
+            // rescan from js + 1, which this phase does not write)
+            if (tid < 2 * PB) {
+                const int sl = tid - PB;
+                E[kk][tid] = (tid >= PB && sl == kk) ? ownv : C[is][tid];
             }
-            __syncthreads();
-            for (int col = tid; col < 2 * PB; col += 256) E[kk][col] = C[is][col];
+            if (tid < kk) E[tid][js] = 0u;
+            if (tid >= PB && tid < PB + nc) {
+                const int rt = tid - PB;
+                if (rt != is) C[rt][js] = 0u;
+            }
             // same values from every thread
             ecol[kk] = c0 + js;
             erow[kk] = crow[is];
             epos[kk] = pos + is;
             used[is] = 1;
-            __syncthreads();
-            if (wv == 0 && lane < nc) {
-                if (lane == is) fc = PB;
-                else if (fc == js) {
-                    fc = PB;
-                    for (int c = js + 1; c < bw; ++c)
-                        if (C[lane][c] != 0u) { fc = c; break; }
-                }
-            }
             ++kk;
+            if (wv == 0) {
+                if (lane < nc) {
+                    if (lane == is) fc = PB;
+                    else if (fc == js) {
+                        fc = PB;
+                        for (int c = js + 1; c < bw; ++c)
+                            if (C[lane][c] != 0u) { fc = c; break; }
+                    }
+                }
+                pivot_search(fc, sh_js, sh_is);
+            }
+            __syncthreads();
             if (kk == bw) break;
+        }
+        // 4. normalise E by its pivot values (inverses in parallel)
+        if (kk > k0) {
+            if (tid < kk) einv[tid] = inv_mod(E[tid][ecol[tid] - c0], p);
+            __syncthreads();
+            for (int t = rg; t < kk; t += SEL_RG) E[t][col] = md.mul(E[t][col], einv[t]);
+            __syncthreads();
         }
         if (exhausted) {
             // rows of this tile that were not selected are zero on the panel
-            int sel = 0;
-            for (int t = 0; t < kk; ++t) sel += (epos[t] >= pos && epos[t] < pos + nc) ? 1 : 0;
-            nzero += nc - sel;
+            nzero += nc - (kk - k0);
         }
         pos += nc;
     }
@@ -224,7 +269,7 @@ __global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t
         pan->erow[tid] = erow[tid];
         pan->ecol[tid] = ecol[tid];
     }
-    for (int i = tid; i < PB * PB; i += 256) {
+    for (int i = tid; i < PB * PB; i += SEL_T) {
         const int t = i >> 6, sl = i & 63;
         pan->T[i] = (t < kk && sl < kk) ? E[t][PB + sl] : 0u;
     }
@@ -247,9 +292,9 @@ __global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t
         }
         for (int t = 0; t < kk; ++t) cand[head + t] = erow[t];
         pan->k = kk;
-        pan->head = head + kk;
-        pan->nzero = nzero;
-        if (head + kk >= count) pan->done = 1;
+        st->head = head + kk;
+        st->nzero += nzero;
+        if (head + kk >= count) st->done = 1;
     }
 }
 
@@ -257,15 +302,15 @@ __global__ __launch_bounds__(256) void k_panel_select(const uint32_t *a, int64_t
 //
 // After a panel's update, a candidate row whose columns [c1, n) are all
 // zero is zero for good (its columns left of c1 are zero already): when the
-// last select met at least a tile's worth of rows that were zero on its
-// panel, such rows are dropped from the candidate list.
+// selects since the last prune met at least a tile's worth of rows that
+// were zero on their panel, such rows are dropped from the candidate list.
 __global__ void k_prune_flags(const uint32_t *a, int64_t ld, int32_t n, int32_t c1,
-                              const int32_t *cand, const Panel *pan, int32_t *alive) {
-    const int32_t nzero = __builtin_amdgcn_readfirstlane(pan->nzero);
-    const int32_t done = __builtin_amdgcn_readfirstlane(pan->done);
+                              const int32_t *cand, const ListState *st, int32_t *alive) {
+    const int32_t nzero = __builtin_amdgcn_readfirstlane(st->nzero);
+    const int32_t done = __builtin_amdgcn_readfirstlane(st->done);
     if (nzero < PB || done) return;
-    const int32_t head = __builtin_amdgcn_readfirstlane(pan->head);
-    const int32_t count = __builtin_amdgcn_readfirstlane(pan->count);
+    const int32_t head = __builtin_amdgcn_readfirstlane(st->head);
+    const int32_t count = __builtin_amdgcn_readfirstlane(st->count);
     const int lane = threadIdx.x & 63;
     const int32_t step = gridDim.x * (blockDim.x >> 6);
     for (int32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); head + w < count;
@@ -283,14 +328,14 @@ __global__ void k_prune_flags(const uint32_t *a, int64_t ld, int32_t n, int32_t 
     }
 }
 
-__global__ __launch_bounds__(256) void k_prune_compact(int32_t *cand, Panel *pan,
+__global__ __launch_bounds__(256) void k_prune_compact(int32_t *cand, ListState *st,
                                                        const int32_t *alive) {
     __shared__ int32_t wsum[4];
-    const int32_t nzero = __builtin_amdgcn_readfirstlane(pan->nzero);
-    const int32_t done = __builtin_amdgcn_readfirstlane(pan->done);
+    const int32_t nzero = __builtin_amdgcn_readfirstlane(st->nzero);
+    const int32_t done = __builtin_amdgcn_readfirstlane(st->done);
     if (nzero < PB || done) return;
-    const int32_t head = __builtin_amdgcn_readfirstlane(pan->head);
-    const int32_t count = __builtin_amdgcn_readfirstlane(pan->count);
+    const int32_t head = __builtin_amdgcn_readfirstlane(st->head);
+    const int32_t count = __builtin_amdgcn_readfirstlane(st->count);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     int32_t out = head;
     // stable in-place compaction: writes never pass the reads of later chunks
@@ -312,9 +357,9 @@ __global__ __launch_bounds__(256) void k_prune_compact(int32_t *cand, Panel *pan
         __syncthreads();
     }
     if (tid == 0) {
-        pan->count = out;
-        pan->nzero = 0;
-        if (out <= head) pan->done = 1;
+        st->count = out;
+        st->nzero = 0;
+        if (out <= head) st->done = 1;
     }
 }
 
@@ -392,12 +437,14 @@ __global__ void k_panel_gather(const uint32_t *a, int64_t ld, int32_t m, int32_t
 
 // ------------------------------------------------- update (integer ALU)
 
-// block: 256 columns x RT rows; A[r][j] -= sum_t L[r][t] Rp[t][j]
+// block: 256 columns x RT rows; A[r][j] -= sum_t L[r][t] Rp[t][j] for the
+// columns j in [cb, ce)
 constexpr int RT = 16;
 
 template <bool BIG>
 __global__ __launch_bounds__(256) void k_update_plain(uint32_t *a, int64_t ld, int32_t m,
-                                                      int32_t n, int32_t c0, const Panel *pan,
+                                                      int32_t cb, int32_t ce, int32_t c0,
+                                                      const Panel *pan,
                                                       const uint32_t *rp, int64_t ldr,
                                                       const uint32_t *L, const int32_t *active,
                                                       Mod md, uint32_t r64) {
@@ -416,8 +463,8 @@ __global__ __launch_bounds__(256) void k_update_plain(uint32_t *a, int64_t ld, i
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr) anyact |= act[rr];
     if (!anyact) return;
-    const int32_t j = c0 + blockIdx.x * 256 + threadIdx.x;
-    if (j >= n) return;
+    const int32_t j = cb + blockIdx.x * 256 + threadIdx.x;
+    if (j >= ce) return;
     uint32_t rv[PB];
 #pragma unroll
     for (int t = 0; t < PB; ++t) rv[t] = t < kk ? rp[(int64_t)t * ldr + (j - c0)] : 0u;
@@ -490,7 +537,8 @@ __device__ __forceinline__ void tile_product(const uint32_t (*Ls)[PB + 1], const
 
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_update_mfma(uint32_t *a, int64_t ld, int32_t m,
-                                                     int32_t n, int32_t c0, const Panel *pan,
+                                                     int32_t cb, int32_t ce, int32_t c0,
+                                                     const Panel *pan,
                                                      const uint32_t *rp, int64_t ldr,
                                                      const uint32_t *L, const int32_t *active,
                                                      Mod md) {
@@ -500,7 +548,7 @@ __global__ __launch_bounds__(256) void k_update_mfma(uint32_t *a, int64_t ld, in
     __shared__ uint32_t Ls[MR][PB + 1];
     __shared__ int32_t act[MR];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int32_t jc = c0 + blockIdx.x * 64;
+    const int32_t jc = cb + blockIdx.x * 64;
     const int32_t rb = blockIdx.y * MR;
     if (tid < MR) act[tid] = (rb + tid < m) ? active[rb + tid] : 0;
     __syncthreads();
@@ -510,7 +558,7 @@ __global__ __launch_bounds__(256) void k_update_mfma(uint32_t *a, int64_t ld, in
     const int kp = (kk + 3) & ~3;
     for (int i = tid; i < PB * 64; i += 256) {
         const int t = i >> 6, c = i & 63;
-        Rs[t][c] = (t < kk && jc + c < n) ? rp[(int64_t)t * ldr + (jc + c - c0)] : 0u;
+        Rs[t][c] = (t < kk && jc + c < ce) ? rp[(int64_t)t * ldr + (jc + c - c0)] : 0u;
     }
     for (int i = tid; i < MR * PB; i += 256) {
         const int rr = i >> 6, t = i & 63;
@@ -564,7 +612,7 @@ __global__ __launch_bounds__(256) void k_update_mfma(uint32_t *a, int64_t ld, in
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int32_t c = jc + 16 * j + (lane & 15);
-                if (c >= n) continue;
+                if (c >= ce) continue;
                 uint32_t *x = a + (int64_t)r * ld + c;
                 *x = sub_mod(*x, res[i][j][e], md.p);
             }
@@ -589,7 +637,7 @@ __global__ void k_out_list(int32_t n, const int32_t *colpiv, const int32_t *flag
 
 // rowpiv = colpiv = -1, cand = 0, 1, ..., m - 1, panel state reset
 __global__ void k_dense_init(int32_t m, int32_t n, int32_t *rowpiv, int32_t *colpiv,
-                             int32_t *cand, Panel *pan) {
+                             int32_t *cand, Panel *pan, ListState *st) {
     const int64_t tot = (int64_t)m > n ? m : n;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -600,11 +648,12 @@ __global__ void k_dense_init(int32_t m, int32_t n, int32_t *rowpiv, int32_t *col
         if (i < n) colpiv[i] = -1;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        pan->k = 0;
-        pan->done = m == 0 ? 1 : 0;
-        pan->head = 0;
-        pan->count = m;
-        pan->nzero = 0;
+        pan[0].k = 0;
+        pan[1].k = 0;
+        st->done = m == 0 ? 1 : 0;
+        st->head = 0;
+        st->count = m;
+        st->nzero = 0;
     }
 }
 
@@ -628,49 +677,86 @@ int dense_rref(gbgpu_dense *d, int form) {
     int32_t *rowpiv = nullptr, *colpiv = nullptr, *active = nullptr, *cand = nullptr;
     uint32_t *L = nullptr, *rp = nullptr;
     Panel *pan = nullptr;
+    ListState *st = nullptr;
     const int64_t ldr = ((int64_t)n + 63) & ~63ll;
     auto cleanup = [&]() {
-        dfree(rowpiv); dfree(colpiv); dfree(active); dfree(cand); dfree(L); dfree(rp); dfree(pan);
+        dfree(rowpiv); dfree(colpiv); dfree(active); dfree(cand); dfree(L); dfree(rp);
+        dfree(pan); dfree(st);
     };
     if ((rc = dalloc(ctx, &rowpiv, (size_t)m)) || (rc = dalloc(ctx, &colpiv, (size_t)n)) ||
         (rc = dalloc(ctx, &active, (size_t)m)) || (rc = dalloc(ctx, &cand, (size_t)m)) ||
         (rc = dalloc(ctx, &L, (size_t)m * PB)) || (rc = dalloc(ctx, &rp, (size_t)PB * ldr)) ||
-        (rc = dalloc(ctx, &pan, 1))) {
+        (rc = dalloc(ctx, &pan, 2)) || (rc = dalloc(ctx, &st, 1))) {
         cleanup();
         return rc;
     }
-    k_dense_init<<<grid_of(m > n ? m : n, 256), 256, 0, s>>>(m, n, rowpiv, colpiv, cand, pan);
+    k_dense_init<<<grid_of(m > n ? m : n, 256), 256, 0, s>>>(m, n, rowpiv, colpiv, cand, pan, st);
     const uint32_t r64 = (uint32_t)((((unsigned __int128)1) << 64) % d->p);
-    if (hipEventRecord(ctx->ev[2], s) != hipSuccess) { cleanup(); return check_hip(ctx, hipGetLastError(), "event"); }
-    int32_t pan_done = 0;
-    int npan = 0;
-    for (int32_t c0 = 0; c0 < n && m > 0; c0 += PB) {
-        const int bw = (n - c0) < PB ? (n - c0) : PB;
-        k_panel_select<<<1, 256, 0, s>>>(d->a, d->ld, n, c0, cand, rowpiv, colpiv, pan, md);
-        k_panel_rp<<<grid_of(n - c0, 64), 256, 0, s>>>(d->a, d->ld, n, c0, pan, rp, ldr, md, r64);
-        k_panel_gather<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, m, c0, bw, rowpiv, pan, L, active);
+    // A[r][cb, ce) -= L[r] * Rp for the rows of the current panel's update
+    auto update = [&](const Panel *pp, int32_t c0, int32_t cb, int32_t ce) {
+        if (cb >= ce) return;
         if (form == GBGPU_FORM_MFMA) {
-            dim3 g((unsigned)((n - c0 + 63) / 64), (unsigned)((m + MR - 1) / MR));
-            if (big) k_update_mfma<true><<<g, 256, 0, s>>>(d->a, d->ld, m, n, c0, pan, rp, ldr, L, active, md);
-            else k_update_mfma<false><<<g, 256, 0, s>>>(d->a, d->ld, m, n, c0, pan, rp, ldr, L, active, md);
+            dim3 g((unsigned)((ce - cb + 63) / 64), (unsigned)((m + MR - 1) / MR));
+            if (big) k_update_mfma<true><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, rp, ldr, L, active, md);
+            else k_update_mfma<false><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, rp, ldr, L, active, md);
         } else {
-            dim3 g((unsigned)((n - c0 + 255) / 256), (unsigned)((m + RT - 1) / RT));
-            if (big) k_update_plain<true><<<g, 256, 0, s>>>(d->a, d->ld, m, n, c0, pan, rp, ldr, L, active, md, r64);
-            else k_update_plain<false><<<g, 256, 0, s>>>(d->a, d->ld, m, n, c0, pan, rp, ldr, L, active, md, r64);
+            dim3 g((unsigned)((ce - cb + 255) / 256), (unsigned)((m + RT - 1) / RT));
+            if (big) k_update_plain<true><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, rp, ldr, L, active, md, r64);
+            else k_update_plain<false><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, rp, ldr, L, active, md, r64);
         }
-        if (c0 + bw < n) {
-            k_prune_flags<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, n, c0 + bw, cand, pan, active);
-            k_prune_compact<<<1, 256, 0, s>>>(cand, pan, active);
+    };
+    if (hipEventRecord(ctx->ev[2], s) != hipSuccess) { cleanup(); return check_hip(ctx, hipGetLastError(), "event"); }
+    // Look-ahead: once the next panel's columns are updated, its select (one
+    // workgroup) runs on the side stream while the rest of this panel's
+    // update fills the other CUs.  Panel records alternate between two
+    // slots; the candidate list is touched by the select and the prune only,
+    // which the stream order keeps apart.
+    hipStream_t x = ctx->side;
+    if (m > 0 && n > 0)
+        k_panel_select<<<1, SEL_T, 0, s>>>(d->a, d->ld, n, 0, cand, rowpiv, colpiv, pan, st, md);
+    struct { int32_t done, k; } hs = {0, 0};
+    int it = 0;
+    for (int32_t c0 = 0; c0 < n && m > 0; c0 += PB, ++it) {
+        Panel *cur = pan + (it & 1), *nxt = pan + ((it + 1) & 1);
+        const int bw = (n - c0) < PB ? (n - c0) : PB;
+        const int32_t c1 = c0 + bw;                       // next panel
+        const int32_t c2 = (n - c1) < PB ? n : c1 + PB;
+        k_panel_rp<<<grid_of(n - c0, 64), 256, 0, s>>>(d->a, d->ld, n, c0, cur, rp, ldr, md, r64);
+        k_panel_gather<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, m, c0, bw, rowpiv, cur, L, active);
+        if (c1 < n) {
+            update(cur, c0, c1, c2);
+            if (hipEventRecord(ctx->sync_ev[0], s) != hipSuccess ||
+                hipStreamWaitEvent(x, ctx->sync_ev[0], 0) != hipSuccess) {
+                cleanup();
+                return check_hip(ctx, hipGetLastError(), "dense look-ahead");
+            }
+            k_panel_select<<<1, SEL_T, 0, x>>>(d->a, d->ld, n, c1, cand, rowpiv, colpiv, nxt, st, md);
+            if (hipEventRecord(ctx->sync_ev[1], x) != hipSuccess) {
+                cleanup();
+                return check_hip(ctx, hipGetLastError(), "dense look-ahead");
+            }
+            update(cur, c0, c0, c1);
+            update(cur, c0, c2, n);
+            if (hipStreamWaitEvent(s, ctx->sync_ev[1], 0) != hipSuccess) {
+                cleanup();
+                return check_hip(ctx, hipGetLastError(), "dense look-ahead");
+            }
+            k_prune_flags<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, n, c1, cand, st, active);
+            k_prune_compact<<<1, 256, 0, s>>>(cand, st, active);
+        } else {
+            update(cur, c0, c0, n);
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { cleanup(); return check_hip(ctx, e, "dense panel launch"); }
-        if (++npan % 8 == 0) {
-            if (hipMemcpyAsync(&pan_done, &pan->done, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        if ((it + 1) % 8 == 0) {
+            // stop once no candidate is left and the selected next panel is empty
+            if (hipMemcpyAsync(&hs.done, &st->done, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipMemcpyAsync(&hs.k, &nxt->k, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess) {
                 cleanup();
                 return check_hip(ctx, hipGetLastError(), "dense panel sync");
             }
-            if (pan_done) break;
+            if (hs.done && hs.k == 0) break;
         }
     }
     // output list: pivot rows by decreasing lead column
