@@ -189,6 +189,42 @@ def main(argv=None) -> int:
                 extra[f"{tag}_{name}_gops"] = dense_ops(4096, 8192, r) / td / 1e9
                 extra[f"{tag}_rank"] = r
                 dd.free()
+        # batched multimodular launch: 64 F4-shaped matrices of uneven sizes
+        # (rows lognormal in [500, 20000], columns 1.5 x rows, density
+        # 0.5-2 %), each modulo its own prime below 2^31; la_option 1 on all
+        # of them as one graph launch, against the same matrices one by one
+        mats = synth.batch_31bit(64, seed=7)
+        nrl_b = sum(mm.nr - len(np.unique(mm.leads())) for mm in mats)
+        bt = ctx.batch(mats)
+        for form, name in ((native.FORM_MFMA, "mfma"), (native.FORM_PLAIN, "plain")):
+            bt.la(form)  # device layout, tuning, graph capture
+            ts, gs = [], []
+            for _ in range(3):
+                _sync()
+                t0 = time.perf_counter()
+                bt.la(form)
+                _sync()
+                ts.append(time.perf_counter() - t0)
+                gs.append(ctx.phase_ms(2))
+            tb = sorted(ts)[1]
+            extra[f"batch64_{name}_ms"] = 1000.0 * tb
+            extra[f"batch64_{name}_graph_ms"] = sorted(gs)[1]
+            extra[f"batch64_{name}_rows_per_s"] = nrl_b / tb
+            extra[f"batch64_{name}_tuned"] = bt.info()
+        bt.free()
+        singles = [ctx.upload(mm) for mm in mats]
+        for dm in singles:
+            dm.la_sparse_dense(native.FORM_MFMA)
+        _sync()
+        t0 = time.perf_counter()
+        for dm in singles:
+            dm.la_sparse_dense(native.FORM_MFMA)
+        _sync()
+        t1 = time.perf_counter() - t0
+        extra["batch64_one_by_one_mfma_ms"] = 1000.0 * t1
+        extra["batch64_to_reduce_rows"] = nrl_b
+        for dm in singles:
+            dm.free()
         out["extra"] = extra
 
     print(json.dumps(out))

@@ -104,10 +104,9 @@ __device__ __forceinline__ void pivot_search(int fc, int32_t &sh_js, int32_t &sh
 constexpr int SEL_T = 1024;     // threads of the select workgroup
 constexpr int SEL_RG = SEL_T / (2 * PB);  // row groups: thread = (row group, column)
 
-__global__ __launch_bounds__(SEL_T) void k_panel_select(const uint32_t *a, int64_t ld, int32_t n,
-                                                        int32_t c0, int32_t *cand,
-                                                        int32_t *rowpiv, int32_t *colpiv,
-                                                        Panel *pan, ListState *st, Mod md) {
+__device__ __forceinline__ void select_body(const uint32_t *a, int64_t ld, int32_t n, int32_t c0,
+                                            int32_t *cand, int32_t *rowpiv, int32_t *colpiv,
+                                            Panel *pan, ListState *st, Mod md) {
     __shared__ uint32_t E[PB][XW];
     __shared__ uint32_t C[PB][XW];
     __shared__ uint32_t G[PB][PB + 1];
@@ -298,23 +297,30 @@ __global__ __launch_bounds__(SEL_T) void k_panel_select(const uint32_t *a, int64
     }
 }
 
+__global__ __launch_bounds__(SEL_T) void k_panel_select(const uint32_t *a, int64_t ld, int32_t n,
+                                                        int32_t c0, int32_t *cand,
+                                                        int32_t *rowpiv, int32_t *colpiv,
+                                                        Panel *pan, ListState *st, Mod md) {
+    select_body(a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md);
+}
+
 // ---------------------------------------------------------------- prune
 //
 // After a panel's update, a candidate row whose columns [c1, n) are all
 // zero is zero for good (its columns left of c1 are zero already): when the
 // selects since the last prune met at least a tile's worth of rows that
 // were zero on their panel, such rows are dropped from the candidate list.
-__global__ void k_prune_flags(const uint32_t *a, int64_t ld, int32_t n, int32_t c1,
-                              const int32_t *cand, const ListState *st, int32_t *alive) {
+__device__ __forceinline__ void prune_flags_body(const uint32_t *a, int64_t ld, int32_t n,
+                                                 int32_t c1, const int32_t *cand,
+                                                 const ListState *st, int32_t *alive,
+                                                 int32_t first, int32_t step) {
     const int32_t nzero = __builtin_amdgcn_readfirstlane(st->nzero);
     const int32_t done = __builtin_amdgcn_readfirstlane(st->done);
     if (nzero < PB || done) return;
     const int32_t head = __builtin_amdgcn_readfirstlane(st->head);
     const int32_t count = __builtin_amdgcn_readfirstlane(st->count);
     const int lane = threadIdx.x & 63;
-    const int32_t step = gridDim.x * (blockDim.x >> 6);
-    for (int32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); head + w < count;
-         w += step) {
+    for (int32_t w = first; head + w < count; w += step) {
         const uint32_t *row = a + (int64_t)cand[head + w] * ld;
         int nz = 0;  // uniform over the wave
         for (int32_t cb = c1; cb < n; cb += 64) {
@@ -328,8 +334,8 @@ __global__ void k_prune_flags(const uint32_t *a, int64_t ld, int32_t n, int32_t 
     }
 }
 
-__global__ __launch_bounds__(256) void k_prune_compact(int32_t *cand, ListState *st,
-                                                       const int32_t *alive) {
+__device__ __forceinline__ void prune_compact_body(int32_t *cand, ListState *st,
+                                                   const int32_t *alive) {
     __shared__ int32_t wsum[4];
     const int32_t nzero = __builtin_amdgcn_readfirstlane(st->nzero);
     const int32_t done = __builtin_amdgcn_readfirstlane(st->done);
@@ -363,21 +369,33 @@ __global__ __launch_bounds__(256) void k_prune_compact(int32_t *cand, ListState 
     }
 }
 
+__global__ void k_prune_flags(const uint32_t *a, int64_t ld, int32_t n, int32_t c1,
+                              const int32_t *cand, const ListState *st, int32_t *alive) {
+    prune_flags_body(a, ld, n, c1, cand, st, alive,
+                     blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6),
+                     gridDim.x * (blockDim.x >> 6));
+}
+
+__global__ __launch_bounds__(256) void k_prune_compact(int32_t *cand, ListState *st,
+                                                       const int32_t *alive) {
+    prune_compact_body(cand, st, alive);
+}
+
 // ---------------------------------------------------------------- rp
 
 // Rp[t][j] = sum_s T[t][s] A[erow[s]][j] for j >= c0: a block covers 64
 // columns, each wave 16 of the rows t; the 64 x 64 panel of the selected
 // rows is staged in LDS.  Products are summed in 128 bits (lo + carries)
 // and reduced once per output.
-__global__ __launch_bounds__(256) void k_panel_rp(uint32_t *a, int64_t ld, int32_t n, int32_t c0,
-                                                  const Panel *pan, uint32_t *rp, int64_t ldr,
-                                                  Mod md, uint32_t r64) {
+__device__ __forceinline__ void rp_body(uint32_t *a, int64_t ld, int32_t n, int32_t c0,
+                                        const Panel *pan, uint32_t *rp, int64_t ldr, Mod md,
+                                        uint32_t r64, int32_t tile) {
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
     __shared__ uint32_t T[PB][PB + 1];
     __shared__ uint32_t V[PB][PB + 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int32_t jb = c0 + blockIdx.x * 64;
+    const int32_t jb = c0 + tile * 64;
     for (int i = tid; i < PB * PB; i += 256) T[i >> 6][i & 63] = pan->T[i];
     for (int i = tid; i < PB * PB; i += 256) {
         const int s_ = i >> 6, c = i & 63;
@@ -414,18 +432,24 @@ __global__ __launch_bounds__(256) void k_panel_rp(uint32_t *a, int64_t ld, int32
     }
 }
 
+__global__ __launch_bounds__(256) void k_panel_rp(uint32_t *a, int64_t ld, int32_t n, int32_t c0,
+                                                  const Panel *pan, uint32_t *rp, int64_t ldr,
+                                                  Mod md, uint32_t r64) {
+    rp_body(a, ld, n, c0, pan, rp, ldr, md, r64, blockIdx.x);
+}
+
 // ------------------------------------------------------------ gather
 
 // L[r][t] = A[r][ecol[t]] for rows that are not this panel's pivots;
 // active[r] = 1 if any of them is nonzero
-__global__ void k_panel_gather(const uint32_t *a, int64_t ld, int32_t m, int32_t c0, int32_t bw,
-                               const int32_t *rowpiv, const Panel *pan, uint32_t *L,
-                               int32_t *active) {
+__device__ __forceinline__ void gather_body(const uint32_t *a, int64_t ld, int32_t m, int32_t c0,
+                                            int32_t bw, const int32_t *rowpiv, const Panel *pan,
+                                            uint32_t *L, int32_t *active, int32_t first,
+                                            int32_t step) {
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
     const int lane = threadIdx.x & 63;
-    const int32_t step = gridDim.x * (blockDim.x >> 6);
-    for (int32_t r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < m; r += step) {
+    for (int32_t r = first; r < m; r += step) {
         const int32_t rp = rowpiv[r];
         const bool cur = rp >= c0 && rp < c0 + bw;
         uint32_t v = 0;
@@ -435,6 +459,13 @@ __global__ void k_panel_gather(const uint32_t *a, int64_t ld, int32_t m, int32_t
     }
 }
 
+__global__ void k_panel_gather(const uint32_t *a, int64_t ld, int32_t m, int32_t c0, int32_t bw,
+                               const int32_t *rowpiv, const Panel *pan, uint32_t *L,
+                               int32_t *active) {
+    gather_body(a, ld, m, c0, bw, rowpiv, pan, L, active,
+                blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), gridDim.x * (blockDim.x >> 6));
+}
+
 // ------------------------------------------------- update (integer ALU)
 
 // block: 256 columns x RT rows; A[r][j] -= sum_t L[r][t] Rp[t][j] for the
@@ -442,17 +473,16 @@ __global__ void k_panel_gather(const uint32_t *a, int64_t ld, int32_t m, int32_t
 constexpr int RT = 16;
 
 template <bool BIG>
-__global__ __launch_bounds__(256) void k_update_plain(uint32_t *a, int64_t ld, int32_t m,
-                                                      int32_t cb, int32_t ce, int32_t c0,
-                                                      const Panel *pan,
-                                                      const uint32_t *rp, int64_t ldr,
-                                                      const uint32_t *L, const int32_t *active,
-                                                      Mod md, uint32_t r64) {
-    const int kk = pan->k;
+__device__ __forceinline__ void update_plain_body(uint32_t *a, int64_t ld, int32_t m, int32_t cb,
+                                                  int32_t ce, int32_t c0, const Panel *pan,
+                                                  const uint32_t *rp, int64_t ldr,
+                                                  const uint32_t *L, const int32_t *active,
+                                                  Mod md, uint32_t r64, int32_t bx, int32_t by) {
+    const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
     __shared__ uint32_t Ls[RT][PB];
     __shared__ int32_t act[RT];
-    const int32_t r0 = blockIdx.y * RT;
+    const int32_t r0 = by * RT;
     for (int i = threadIdx.x; i < RT * PB; i += 256) {
         const int rr = i / PB, t = i % PB;
         Ls[rr][t] = (r0 + rr < m) ? L[(int64_t)(r0 + rr) * PB + t] : 0u;
@@ -463,7 +493,7 @@ __global__ __launch_bounds__(256) void k_update_plain(uint32_t *a, int64_t ld, i
 #pragma unroll
     for (int rr = 0; rr < RT; ++rr) anyact |= act[rr];
     if (!anyact) return;
-    const int32_t j = cb + blockIdx.x * 256 + threadIdx.x;
+    const int32_t j = cb + bx * 256 + threadIdx.x;
     if (j >= ce) return;
     uint32_t rv[PB];
 #pragma unroll
@@ -488,6 +518,17 @@ __global__ __launch_bounds__(256) void k_update_plain(uint32_t *a, int64_t ld, i
         uint32_t *x = a + (int64_t)(r0 + rr) * ld + j;
         *x = sub_mod(*x, s, md.p);
     }
+}
+
+template <bool BIG>
+__global__ __launch_bounds__(256) void k_update_plain(uint32_t *a, int64_t ld, int32_t m,
+                                                      int32_t cb, int32_t ce, int32_t c0,
+                                                      const Panel *pan,
+                                                      const uint32_t *rp, int64_t ldr,
+                                                      const uint32_t *L, const int32_t *active,
+                                                      Mod md, uint32_t r64) {
+    update_plain_body<BIG>(a, ld, m, cb, ce, c0, pan, rp, ldr, L, active, md, r64, blockIdx.x,
+                           blockIdx.y);
 }
 
 // --------------------------------------------------- update (MFMA f64)
@@ -536,20 +577,19 @@ __device__ __forceinline__ void tile_product(const uint32_t (*Ls)[PB + 1], const
 }
 
 template <bool SPLIT>
-__global__ __launch_bounds__(256) void k_update_mfma(uint32_t *a, int64_t ld, int32_t m,
-                                                     int32_t cb, int32_t ce, int32_t c0,
-                                                     const Panel *pan,
-                                                     const uint32_t *rp, int64_t ldr,
-                                                     const uint32_t *L, const int32_t *active,
-                                                     Mod md) {
-    const int kk = pan->k;
+__device__ __forceinline__ void update_mfma_body(uint32_t *a, int64_t ld, int32_t m, int32_t cb,
+                                                 int32_t ce, int32_t c0, const Panel *pan,
+                                                 const uint32_t *rp, int64_t ldr,
+                                                 const uint32_t *L, const int32_t *active,
+                                                 Mod md, int32_t bx, int32_t by) {
+    const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
     __shared__ uint32_t Rs[PB][PB + 1];
     __shared__ uint32_t Ls[MR][PB + 1];
     __shared__ int32_t act[MR];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int32_t jc = cb + blockIdx.x * 64;
-    const int32_t rb = blockIdx.y * MR;
+    const int32_t jc = cb + bx * 64;
+    const int32_t rb = by * MR;
     if (tid < MR) act[tid] = (rb + tid < m) ? active[rb + tid] : 0;
     __syncthreads();
     int anyact = 0;
@@ -617,6 +657,17 @@ __global__ __launch_bounds__(256) void k_update_mfma(uint32_t *a, int64_t ld, in
                 *x = sub_mod(*x, res[i][j][e], md.p);
             }
         }
+}
+
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_update_mfma(uint32_t *a, int64_t ld, int32_t m,
+                                                     int32_t cb, int32_t ce, int32_t c0,
+                                                     const Panel *pan,
+                                                     const uint32_t *rp, int64_t ldr,
+                                                     const uint32_t *L, const int32_t *active,
+                                                     Mod md) {
+    update_mfma_body<SPLIT>(a, ld, m, cb, ce, c0, pan, rp, ldr, L, active, md, blockIdx.x,
+                            blockIdx.y);
 }
 
 // ------------------------------------------------------------ output
@@ -759,27 +810,215 @@ int dense_rref(gbgpu_dense *d, int form) {
             if (hs.done && hs.k == 0) break;
         }
     }
-    // output list: pivot rows by decreasing lead column
-    int32_t *flag = nullptr, *rank = nullptr;
-    if ((rc = dalloc(ctx, &flag, (size_t)n)) || (rc = dalloc(ctx, &rank, (size_t)n))) {
-        dfree(flag); cleanup(); return rc;
-    }
-    if (n > 0) k_out_flags<<<grid_of(n, 256), 256, 0, s>>>(n, colpiv, flag);
-    int64_t total = 0;
-    if ((rc = exclusive_scan_i32(ctx, flag, rank, n, &total))) { dfree(flag); dfree(rank); cleanup(); return rc; }
-    d->rank = (int32_t)total;
-    if ((rc = dalloc(ctx, &d->orow, (size_t)total)) || (rc = dalloc(ctx, &d->olead, (size_t)total))) {
-        dfree(flag); dfree(rank); cleanup(); return rc;
-    }
-    if (n > 0) k_out_list<<<grid_of(n, 256), 256, 0, s>>>(n, colpiv, flag, rank, d->orow, d->olead);
     (void)hipEventRecord(ctx->ev[3], s);
-    hipError_t e = hipStreamSynchronize(s);
-    dfree(flag); dfree(rank);
+    rc = dense_output(d, colpiv);
     cleanup();
-    if (e != hipSuccess) return check_hip(ctx, e, "dense rref");
+    if (rc) return rc;
     float ms = 0;
     if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]) == hipSuccess) ctx->phase_ms[1] = ms;
     return GBGPU_OK;
+}
+
+int dense_output(gbgpu_dense *d, const int32_t *colpiv) {
+    gbgpu_ctx *ctx = d->ctx;
+    hipStream_t s = ctx->stream;
+    const int32_t n = d->n;
+    int rc;
+    dfree(d->orow);
+    dfree(d->olead);
+    d->rank = 0;
+    int32_t *flag = nullptr, *rank = nullptr;
+    if ((rc = dalloc(ctx, &flag, (size_t)n)) || (rc = dalloc(ctx, &rank, (size_t)n))) {
+        dfree(flag);
+        return rc;
+    }
+    if (n > 0) k_out_flags<<<grid_of(n, 256), 256, 0, s>>>(n, colpiv, flag);
+    int64_t total = 0;
+    if ((rc = exclusive_scan_i32(ctx, flag, rank, n, &total))) { dfree(flag); dfree(rank); return rc; }
+    d->rank = (int32_t)total;
+    if ((rc = dalloc(ctx, &d->orow, (size_t)total)) || (rc = dalloc(ctx, &d->olead, (size_t)total))) {
+        dfree(flag); dfree(rank); return rc;
+    }
+    if (n > 0) k_out_list<<<grid_of(n, 256), 256, 0, s>>>(n, colpiv, flag, rank, d->orow, d->olead);
+    hipError_t e = hipStreamSynchronize(s);
+    dfree(flag);
+    dfree(rank);
+    if (e != hipSuccess) return check_hip(ctx, e, "dense output");
+    return GBGPU_OK;
+}
+
+// ------------------------------------------------------------ batches
+//
+// The panel loop over many dense blocks at once: step k works on columns
+// [64 k, 64 k + 64) of every block that is that wide.  Block counts per
+// (step, kernel, matrix) depend on the shapes only; the host precomputes
+// their prefix sums (tab[step][kind][0..nmat]) and every batch kernel finds
+// its matrix by a binary search over them.  No host synchronisation is
+// needed inside the loop, so the whole sequence can be captured in a graph.
+
+__device__ __forceinline__ int tab_find(const int32_t *pre, int32_t nmat, int32_t b) {
+    int lo = 0, hi = nmat - 1;  // last i with pre[i] <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+// one block per matrix: rowpiv = colpiv = -1, the candidate list = the rows
+// flagged nonzero (all rows without flags), panel state reset
+__global__ __launch_bounds__(256) void k_batch_dense_init(const DenseDesc *D) {
+    __shared__ int32_t wsum[4];
+    const DenseDesc &d = D[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int32_t m = d.m, n = d.n;
+    for (int32_t i = tid; i < m; i += 256) d.rowpiv[i] = -1;
+    for (int32_t i = tid; i < n; i += 256) d.colpiv[i] = -1;
+    int32_t out = 0;
+    for (int32_t base = 0; base < m; base += 256) {
+        const int32_t i = base + tid;
+        const int f = i < m ? (d.dnz ? d.dnz[i] : 1) : 0;
+        const int incl = wave_incl_scan(f);
+        wsum[wv] = __shfl(incl, 63, 64);  // every lane stores the wave total
+        __syncthreads();
+        int off = 0, tot = 0;
+        for (int q = 0; q < 4; ++q) {
+            const int ws = wsum[q];
+            if (q < wv) off += ws;
+            tot += ws;
+        }
+        if (f) d.cand[out + off + incl - 1] = i;
+        out += tot;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        d.pan->k = 0;
+        d.st->done = out == 0 ? 1 : 0;
+        d.st->head = 0;
+        d.st->count = out;
+        d.st->nzero = 0;
+    }
+}
+
+__global__ __launch_bounds__(SEL_T) void k_batch_select(const DenseDesc *D, int32_t c0) {
+    const DenseDesc &d = D[blockIdx.x];
+    if (c0 >= d.n) return;
+    select_body(d.a, d.ld, d.n, c0, d.cand, d.rowpiv, d.colpiv, d.pan, d.st, d.md);
+}
+
+__global__ __launch_bounds__(256) void k_batch_rp(const DenseDesc *D, const int32_t *pre,
+                                                  int32_t nmat, int32_t c0) {
+    const int i = tab_find(pre, nmat, blockIdx.x);
+    const DenseDesc &d = D[i];
+    rp_body(d.a, d.ld, d.n, c0, d.pan, d.rp, d.ldr, d.md, d.r64, blockIdx.x - pre[i]);
+}
+
+__global__ void k_batch_gather(const DenseDesc *D, const int32_t *pre, int32_t nmat, int32_t c0) {
+    const int i = tab_find(pre, nmat, blockIdx.x);
+    const DenseDesc &d = D[i];
+    const int32_t nb = pre[i + 1] - pre[i];
+    const int32_t bw = min(PB, d.n - c0);
+    gather_body(d.a, d.ld, d.m, c0, bw, d.rowpiv, d.pan, d.L, d.active,
+                (blockIdx.x - pre[i]) * 4 + (threadIdx.x >> 6), nb * 4);
+}
+
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_batch_update_mfma(const DenseDesc *D, const int32_t *pre,
+                                                           int32_t nmat, int32_t c0) {
+    const int i = tab_find(pre, nmat, blockIdx.x);
+    const DenseDesc &d = D[i];
+    const int32_t l = blockIdx.x - pre[i];
+    const int32_t ct = (d.n - c0 + 63) / 64;
+    update_mfma_body<SPLIT>(d.a, d.ld, d.m, c0, d.n, c0, d.pan, d.rp, d.ldr, d.L, d.active, d.md,
+                            l % ct, l / ct);
+}
+
+template <bool BIG>
+__global__ __launch_bounds__(256) void k_batch_update_plain(const DenseDesc *D, const int32_t *pre,
+                                                            int32_t nmat, int32_t c0) {
+    const int i = tab_find(pre, nmat, blockIdx.x);
+    const DenseDesc &d = D[i];
+    const int32_t l = blockIdx.x - pre[i];
+    const int32_t ct = (d.n - c0 + 255) / 256;
+    update_plain_body<BIG>(d.a, d.ld, d.m, c0, d.n, c0, d.pan, d.rp, d.ldr, d.L, d.active, d.md,
+                           d.r64, l % ct, l / ct);
+}
+
+__global__ void k_batch_prune_flags(const DenseDesc *D, const int32_t *pre, int32_t nmat,
+                                    int32_t c0) {
+    const int i = tab_find(pre, nmat, blockIdx.x);
+    const DenseDesc &d = D[i];
+    const int32_t nb = pre[i + 1] - pre[i];
+    prune_flags_body(d.a, d.ld, d.n, c0 + PB, d.cand, d.st, d.active,
+                     (blockIdx.x - pre[i]) * 4 + (threadIdx.x >> 6), nb * 4);
+}
+
+__global__ __launch_bounds__(256) void k_batch_prune_compact(const DenseDesc *D, int32_t c0) {
+    const DenseDesc &d = D[blockIdx.x];
+    if (c0 + PB >= d.n) return;
+    prune_compact_body(d.cand, d.st, d.active);
+}
+
+size_t dense_panel_bytes() { return sizeof(Panel); }
+size_t dense_liststate_bytes() { return sizeof(ListState); }
+
+int batch_dense_init(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat) {
+    if (nmat > 0) k_batch_dense_init<<<nmat, 256, 0, ctx->stream>>>(D);
+    GBGPU_HIP(ctx, hipGetLastError());
+    return GBGPU_OK;
+}
+
+int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int32_t *tab,
+                     const int32_t *htab, int32_t step, int form, bool big) {
+    hipStream_t s = ctx->stream;
+    const int32_t c0 = step * PB;
+    const int32_t *pre = tab + (int64_t)step * 4 * (nmat + 1);
+    const int32_t *hpre = htab + (int64_t)step * 4 * (nmat + 1);
+    const int32_t nrp = hpre[nmat], nga = hpre[(nmat + 1) + nmat];
+    const int32_t nup = hpre[2 * (nmat + 1) + nmat], npr = hpre[3 * (nmat + 1) + nmat];
+    k_batch_select<<<nmat, SEL_T, 0, s>>>(D, c0);
+    if (nrp > 0) k_batch_rp<<<nrp, 256, 0, s>>>(D, pre, nmat, c0);
+    if (nga > 0) k_batch_gather<<<nga, 256, 0, s>>>(D, pre + (nmat + 1), nmat, c0);
+    if (nup > 0) {
+        const int32_t *pu = pre + 2 * (nmat + 1);
+        if (form == GBGPU_FORM_MFMA) {
+            if (big) k_batch_update_mfma<true><<<nup, 256, 0, s>>>(D, pu, nmat, c0);
+            else k_batch_update_mfma<false><<<nup, 256, 0, s>>>(D, pu, nmat, c0);
+        } else {
+            if (big) k_batch_update_plain<true><<<nup, 256, 0, s>>>(D, pu, nmat, c0);
+            else k_batch_update_plain<false><<<nup, 256, 0, s>>>(D, pu, nmat, c0);
+        }
+    }
+    if (npr > 0) {
+        k_batch_prune_flags<<<npr, 256, 0, s>>>(D, pre + 3 * (nmat + 1), nmat, c0);
+        k_batch_prune_compact<<<nmat, 256, 0, s>>>(D, c0);
+    }
+    GBGPU_HIP(ctx, hipGetLastError());
+    return GBGPU_OK;
+}
+
+// block counts of one batch step (prefix sums over the matrices)
+void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int form, int32_t *out) {
+    const int32_t c0 = step * PB;
+    int32_t *prp = out, *pga = out + (nmat + 1), *pup = out + 2 * (nmat + 1),
+            *ppr = out + 3 * (nmat + 1);
+    prp[0] = pga[0] = pup[0] = ppr[0] = 0;
+    for (int32_t i = 0; i < nmat; ++i) {
+        const int32_t m = hd[i].m, n = hd[i].n;
+        const bool on = c0 < n && m > 0;
+        const int32_t rows4 = on ? (m + 3) / 4 : 0;
+        const int32_t cap = rows4 < 2048 ? rows4 : 2048;
+        int32_t nup = 0;
+        if (on) {
+            if (form == GBGPU_FORM_MFMA) nup = ((n - c0 + 63) / 64) * ((m + MR - 1) / MR);
+            else nup = ((n - c0 + 255) / 256) * ((m + RT - 1) / RT);
+        }
+        prp[i + 1] = prp[i] + (on ? (n - c0 + 63) / 64 : 0);
+        pga[i + 1] = pga[i] + cap;
+        pup[i + 1] = pup[i] + nup;
+        ppr[i + 1] = ppr[i] + ((on && c0 + PB < n) ? cap : 0);
+    }
 }
 
 }  // namespace gbgpu

@@ -63,8 +63,48 @@ struct gbgpu_dense {
 namespace gbgpu {
 
 // sparse.hip
+struct SpArgs {
+    int32_t nc, ncl, ncr, nrl, nwin;
+    const int64_t *rp;
+    const int32_t *ci;
+    const int32_t *rcf;
+    const int64_t *cp;
+    const uint32_t *cf;
+    const int32_t *upiv;
+    const uint32_t *pbits;
+    const int64_t *seg;
+    const uint2 *bucket;
+    uint32_t *xs;       // gridDim.x x ncl multipliers
+    uint32_t *dprime;   // nrl x ncr
+    int32_t *dnz;       // nrl
+    int32_t *queue;     // work counter
+    unsigned long long *work;  // COUNT only: [pivots applied, pivot entries]
+    Mod md;
+};
+
+// what the blocked pass needs of one matrix
+struct BlockedArgs {
+    SpArgs a;
+    const uint32_t *tinv;
+    const int64_t *gseg;
+    const uint2 *gbucket;
+    int32_t ngroups;
+};
+
 int prep_split(gbgpu_mat *m, int32_t lim);
-int prep_buckets(gbgpu_mat *m);
+// force_wshift: window of 2^force_wshift columns (0: the smallest that fits)
+int prep_buckets(gbgpu_mat *m, int force_wshift = 0);
+int prep_group_buckets(gbgpu_mat *m);
+// group buckets and the inverses of the pivot groups' diagonal blocks
+int prep_blocked(gbgpu_mat *m, uint32_t **tinv);
+// the kernel arguments of m (xs, queue, work left null)
+void sparse_args(const gbgpu_mat *m, SpArgs *a);
+// LDS bytes of one wave of the sparse pass (occupancy)
+int sparse_wave_lds(int wshift, int form);
+// one persistent launch over the to-reduce rows of nmat matrices
+int batch_sparse_launch(gbgpu_ctx *ctx, const BlockedArgs *mats, const int32_t *row_off,
+                        int32_t nmat, int32_t *queue, uint32_t *xs, int64_t xs_stride, int wshift,
+                        bool big, int form, int grid);
 // work (optional, 2 values): pivots applied and pivot entries streamed,
 // counted by an instrumented launch
 // form: GBGPU_FORM_PLAIN = ballot chain, GBGPU_FORM_MFMA = blocked (group
@@ -73,6 +113,35 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work = nullptr, int form = GBGPU_FOR
 
 // dense.hip: reduced echelon form of d (in place); form = GBGPU_FORM_*
 int dense_rref(gbgpu_dense *d, int form);
+// d->rank, d->orow, d->olead from the column -> pivot row map of an
+// eliminated block (decreasing lead order)
+int dense_output(gbgpu_dense *d, const int32_t *colpiv);
+
+// dense.hip, batches: one dense block of a batch and its elimination state
+struct Panel;
+struct ListState;
+struct DenseDesc {
+    uint32_t *a;
+    int64_t ld;
+    int32_t m, n;
+    Mod md;
+    uint32_t r64;              // 2^64 mod p
+    int32_t *rowpiv, *colpiv;  // m, n
+    int32_t *active, *cand;    // m, m
+    uint32_t *L, *rp;          // m x 64, 64 x ldr
+    int64_t ldr;
+    Panel *pan;                // one panel record
+    ListState *st;
+    const int32_t *dnz;        // m: rows that take part (nullptr: all)
+};
+int batch_dense_init(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat);
+// block counts of step `step` for the four grids (rp, gather, update,
+// prune), prefix sums over the matrices: out[4 * (nmat + 1)]
+void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int form, int32_t *out);
+int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int32_t *tab,
+                     const int32_t *htab, int32_t step, int form, bool big);
+size_t dense_panel_bytes();      // sizeof(Panel)
+size_t dense_liststate_bytes();  // sizeof(ListState)
 
 // rows.hip: rows of a dense RREF to CSR (columns shifted by col_shift);
 // with reverse the output order is increasing lead

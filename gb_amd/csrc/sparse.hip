@@ -178,7 +178,7 @@ __global__ __launch_bounds__(64) void k_bucket(int32_t ncl, int32_t nwin, int32_
     }
 }
 
-int prep_buckets(gbgpu_mat *m) {
+int prep_buckets(gbgpu_mat *m, int force_wshift) {
     gbgpu_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
     dfree(m->seg);
@@ -187,6 +187,7 @@ int prep_buckets(gbgpu_mat *m) {
     // smallest window (2048 .. 8192 columns) that keeps nwin <= 63
     int wshift = 11;
     while (((int64_t)m->nc + (1 << wshift) - 1) >> wshift > 63 && wshift < 13) ++wshift;
+    if (force_wshift) wshift = force_wshift;
     const int nwin = (int)(((int64_t)m->nc + (1 << wshift) - 1) >> wshift);
     if (nwin > 63) return set_error(ctx, GBGPU_ERANGE, "too many columns for the sparse pass");
     m->wshift = wshift;
@@ -289,52 +290,36 @@ int prep_group_buckets(gbgpu_mat *m) {
 
 // ------------------------------------------------------ sparse pass
 
-struct SpArgs {
-    int32_t nc, ncl, ncr, nrl, nwin;
-    const int64_t *rp;
-    const int32_t *ci;
-    const int32_t *rcf;
-    const int64_t *cp;
-    const uint32_t *cf;
-    const int32_t *upiv;
-    const uint32_t *pbits;
-    const int64_t *seg;
-    const uint2 *bucket;
-    uint32_t *xs;       // gridDim.x x ncl multipliers
-    uint32_t *dprime;   // nrl x ncr
-    int32_t *dnz;       // nrl
-    int32_t *queue;     // work counter
-    unsigned long long *work;  // COUNT only: [pivots applied, pivot entries]
-    Mod md;
-};
 
 // pivot entries staged per 64-column group (6 KB of LDS)
 constexpr int STAGE = 768;
 
-template <int WSHIFT, bool BIG, bool COUNT>
-__global__ __launch_bounds__(64) void k_sparse_plain(SpArgs a) {
-    constexpr int W = 1 << WSHIFT;
-    __shared__ uint64_t acc[W];
-    __shared__ uint32_t pb[W / 32];
-    __shared__ int64_t sh_base[64];
-    __shared__ int32_t sh_start[64];
-    __shared__ uint32_t sh_x[64];
-    __shared__ uint2 stage[STAGE];
-    const int lane = threadIdx.x;
-    const uint32_t p = a.md.p;
-    uint32_t *xs = a.xs + (size_t)blockIdx.x * (size_t)(a.ncl > 0 ? a.ncl : 1);
+// LDS of one wave of the plain pass
+template <int WSHIFT>
+struct PlainLds {
+    uint64_t acc[1 << WSHIFT];
+    uint32_t pb[(1 << WSHIFT) / 32];
+    int64_t sh_base[64];
+    int32_t sh_start[64];
+    uint32_t sh_x[64];
+    uint2 stage[STAGE];
+};
 
-    for (;;) {
-        // No lane-specific code may sit in this loop: with a lane-0-only
-        // atomic here the compiler gives lanes 1..63 a back edge that skips
-        // it (in a one-wave workgroup __syncthreads() is only a wave
-        // barrier), and they re-read a stale row index forever.  Every lane
-        // issues the atomic (lane 0 adds 1, the others 0) and lane 0's
-        // result is broadcast as a scalar.
-        const int q0 = atomicAdd(a.queue, lane == 0 ? 1 : 0);
-        int q = __builtin_amdgcn_readfirstlane(q0);
-        if (q >= a.nrl) break;
-        q = a.nrl - 1 - q;  // upiv ends with the smallest leads: longest rows first
+// reduce to-reduce row number q of a (a wave-uniform value) into
+// dprime[q] and dnz[q]; xs is this wave's multiplier scratch (ncl entries)
+template <int WSHIFT, bool BIG, bool COUNT>
+__device__ __forceinline__ void plain_row(const SpArgs &a, int q, uint32_t *xs,
+                                          PlainLds<WSHIFT> &L) {
+    constexpr int W = 1 << WSHIFT;
+    uint64_t *acc = L.acc;
+    uint32_t *pb = L.pb;
+    int64_t *sh_base = L.sh_base;
+    int32_t *sh_start = L.sh_start;
+    uint32_t *sh_x = L.sh_x;
+    uint2 *stage = L.stage;
+    const int lane = threadIdx.x & 63;
+    const uint32_t p = a.md.p;
+    {
         // per-row values are the same in every lane: keep them scalar so that
         // the loops below (which hold ballots and shuffles) stay uniform
         const int32_t row = __builtin_amdgcn_readfirstlane(a.upiv[q]);
@@ -482,6 +467,26 @@ __global__ __launch_bounds__(64) void k_sparse_plain(SpArgs a) {
     }
 }
 
+template <int WSHIFT, bool BIG, bool COUNT>
+__global__ __launch_bounds__(64) void k_sparse_plain(SpArgs a) {
+    __shared__ PlainLds<WSHIFT> L;
+    const int lane = threadIdx.x;
+    uint32_t *xs = a.xs + (size_t)blockIdx.x * (size_t)(a.ncl > 0 ? a.ncl : 1);
+    for (;;) {
+        // No lane-specific code may sit in this loop: with a lane-0-only
+        // atomic here the compiler gives lanes 1..63 a back edge that skips
+        // it (in a one-wave workgroup __syncthreads() is only a wave
+        // barrier), and they re-read a stale row index forever.  Every lane
+        // issues the atomic (lane 0 adds 1, the others 0) and lane 0's
+        // result is broadcast as a scalar.
+        const int q0 = atomicAdd(a.queue, lane == 0 ? 1 : 0);
+        const int q = __builtin_amdgcn_readfirstlane(q0);
+        if (q >= a.nrl) break;
+        // upiv ends with the smallest leads: longest rows first
+        plain_row<WSHIFT, BIG, COUNT>(a, a.nrl - 1 - q, xs, L);
+    }
+}
+
 // ------------------------------------------------ blocked sparse pass
 //
 // The chain of the plain pass (one pivot column after the other) is what
@@ -564,24 +569,27 @@ __device__ __forceinline__ void stream_group(uint64_t *acc, int32_t w0, uint32_t
     __syncthreads();
 }
 
-template <int WSHIFT, bool BIG>
-__global__ __launch_bounds__(64) void k_sparse_blocked(SpArgs a, const uint32_t *tinv,
-                                                       const int64_t *gseg, const uint2 *gbucket,
-                                                       int32_t ngroups) {
-    constexpr int W = 1 << WSHIFT;
-    __shared__ uint64_t acc[W];
-    __shared__ uint32_t pb[W / 32];
-    __shared__ uint32_t sh_x[64];
-    const int lane = threadIdx.x;
-    const uint32_t p = a.md.p;
-    uint32_t *xs = a.xs + (size_t)blockIdx.x * (size_t)(a.ncl > 0 ? a.ncl : 1);
+// LDS of one wave of the blocked pass
+template <int WSHIFT>
+struct BlockedLds {
+    uint64_t acc[1 << WSHIFT];
+    uint32_t pb[(1 << WSHIFT) / 32];
+    uint32_t sh_x[64];
+};
 
-    for (;;) {
-        // row index broadcast as a scalar (see k_sparse_plain)
-        const int q0 = atomicAdd(a.queue, lane == 0 ? 1 : 0);
-        int q = __builtin_amdgcn_readfirstlane(q0);
-        if (q >= a.nrl) break;
-        q = a.nrl - 1 - q;
+
+template <int WSHIFT, bool BIG>
+__device__ __forceinline__ void blocked_row(const SpArgs &a, int q, uint32_t *xs,
+                                            BlockedLds<WSHIFT> &L, const uint32_t *tinv,
+                                            const int64_t *gseg, const uint2 *gbucket,
+                                            int32_t ngroups) {
+    constexpr int W = 1 << WSHIFT;
+    uint64_t *acc = L.acc;
+    uint32_t *pb = L.pb;
+    uint32_t *sh_x = L.sh_x;
+    const int lane = threadIdx.x & 63;
+    const uint32_t p = a.md.p;
+    {
         const int32_t row = __builtin_amdgcn_readfirstlane(a.upiv[q]);
         const int64_t rb = a.rp[row], re = a.rp[row + 1];
         const uint32_t *rcf = a.cf + a.cp[a.rcf[row]] - rb;
@@ -657,6 +665,84 @@ __global__ __launch_bounds__(64) void k_sparse_blocked(SpArgs a, const uint32_t 
     }
 }
 
+template <int WSHIFT, bool BIG>
+__global__ __launch_bounds__(64) void k_sparse_blocked(SpArgs a, const uint32_t *tinv,
+                                                       const int64_t *gseg, const uint2 *gbucket,
+                                                       int32_t ngroups) {
+    __shared__ BlockedLds<WSHIFT> L;
+    const int lane = threadIdx.x;
+    uint32_t *xs = a.xs + (size_t)blockIdx.x * (size_t)(a.ncl > 0 ? a.ncl : 1);
+    for (;;) {
+        // row index broadcast as a scalar (see k_sparse_plain)
+        const int q0 = atomicAdd(a.queue, lane == 0 ? 1 : 0);
+        const int q = __builtin_amdgcn_readfirstlane(q0);
+        if (q >= a.nrl) break;
+        blocked_row<WSHIFT, BIG>(a, a.nrl - 1 - q, xs, L, tinv, gseg, gbucket, ngroups);
+    }
+}
+
+// ------------------------------------------------------ batched passes
+//
+// One persistent launch over the to-reduce rows of many matrices: the
+// queue runs over the concatenation of the matrices' rows (row_off holds
+// the prefix sums, matrices in the caller's order), each wave takes the
+// next row, finds its matrix by a scalar binary search and reduces it with
+// that matrix's arguments (its own prime among them).  xs holds xs_stride
+// multipliers per wave (the largest ncl of the batch).
+
+__device__ __forceinline__ int batch_find(const int32_t *row_off, int32_t nmat, int32_t q) {
+    int lo = 0, hi = nmat - 1;  // last i with row_off[i] <= q
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (row_off[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+template <int WSHIFT, bool BIG>
+__global__ __launch_bounds__(64) void k_sparse_blocked_batch(const BlockedArgs *mats,
+                                                             const int32_t *row_off,
+                                                             int32_t nmat, int32_t *queue,
+                                                             uint32_t *xs_all,
+                                                             int64_t xs_stride) {
+    __shared__ BlockedLds<WSHIFT> L;
+    const int lane = threadIdx.x;
+    uint32_t *xs = xs_all + (int64_t)blockIdx.x * xs_stride;
+    const int32_t total = __builtin_amdgcn_readfirstlane(row_off[nmat]);
+    for (;;) {
+        const int q0 = atomicAdd(queue, lane == 0 ? 1 : 0);
+        const int q = __builtin_amdgcn_readfirstlane(q0);
+        if (q >= total) break;
+        const int i = batch_find(row_off, nmat, q);
+        const BlockedArgs &b = mats[i];
+        const int32_t nrl = __builtin_amdgcn_readfirstlane(b.a.nrl);
+        const int32_t r = nrl - 1 - (q - __builtin_amdgcn_readfirstlane(row_off[i]));
+        blocked_row<WSHIFT, BIG>(b.a, r, xs, L, b.tinv, b.gseg, b.gbucket, b.ngroups);
+    }
+}
+
+template <int WSHIFT, bool BIG>
+__global__ __launch_bounds__(64) void k_sparse_plain_batch(const BlockedArgs *mats,
+                                                           const int32_t *row_off, int32_t nmat,
+                                                           int32_t *queue, uint32_t *xs_all,
+                                                           int64_t xs_stride) {
+    __shared__ PlainLds<WSHIFT> L;
+    const int lane = threadIdx.x;
+    uint32_t *xs = xs_all + (int64_t)blockIdx.x * xs_stride;
+    const int32_t total = __builtin_amdgcn_readfirstlane(row_off[nmat]);
+    for (;;) {
+        const int q0 = atomicAdd(queue, lane == 0 ? 1 : 0);
+        const int q = __builtin_amdgcn_readfirstlane(q0);
+        if (q >= total) break;
+        const int i = batch_find(row_off, nmat, q);
+        const SpArgs &a = mats[i].a;
+        const int32_t nrl = __builtin_amdgcn_readfirstlane(a.nrl);
+        const int32_t r = nrl - 1 - (q - __builtin_amdgcn_readfirstlane(row_off[i]));
+        plain_row<WSHIFT, BIG, false>(a, r, xs, L);
+    }
+}
+
 __global__ void k_compact(int32_t n, const int32_t *flag, const int32_t *rank, int32_t *list) {
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         if (flag[i]) list[rank[i]] = i;
@@ -678,6 +764,59 @@ static void launch_blocked(const SpArgs &a, const uint32_t *tinv, const int64_t 
                            const uint2 *gbucket, int32_t ngroups, bool big, int grid, hipStream_t s) {
     if (big) k_sparse_blocked<WS, true><<<grid, 64, 0, s>>>(a, tinv, gseg, gbucket, ngroups);
     else k_sparse_blocked<WS, false><<<grid, 64, 0, s>>>(a, tinv, gseg, gbucket, ngroups);
+}
+
+void sparse_args(const gbgpu_mat *m, SpArgs *a) {
+    a->nc = m->nc; a->ncl = m->ncl; a->ncr = m->nc - m->ncl; a->nrl = m->nrl; a->nwin = m->nwin;
+    a->rp = m->rp; a->ci = m->ci; a->rcf = m->rcf; a->cp = m->cp; a->cf = m->cf;
+    a->upiv = m->upiv; a->pbits = m->pbits; a->seg = m->seg; a->bucket = m->bucket;
+    a->xs = nullptr; a->dprime = m->dprime; a->dnz = m->dnz;
+    a->queue = nullptr; a->work = nullptr;
+    a->md = Mod::make(m->p);
+}
+
+int prep_blocked(gbgpu_mat *m, uint32_t **tinv) {
+    gbgpu_ctx *ctx = m->ctx;
+    int rc;
+    *tinv = nullptr;
+    if ((rc = prep_group_buckets(m))) return rc;
+    const int32_t ngroups = (m->ncl + 63) / 64;
+    if (ngroups > 0) {
+        if ((rc = dalloc(ctx, tinv, (size_t)ngroups * 4096))) return rc;
+        k_group_inverse<<<ngroups, 64, 0, ctx->stream>>>(m->ncl, m->wshift, m->pbits, m->seg,
+                                                          m->bucket, Mod::make(m->p), *tinv);
+        GBGPU_HIP(ctx, hipGetLastError());
+    }
+    return GBGPU_OK;
+}
+
+int batch_sparse_launch(gbgpu_ctx *ctx, const BlockedArgs *mats, const int32_t *row_off,
+                        int32_t nmat, int32_t *queue, uint32_t *xs, int64_t xs_stride, int wshift,
+                        bool big, int form, int grid) {
+    hipStream_t s = ctx->stream;
+#define GBGPU_BATCH_SP(WS)                                                                       \
+    do {                                                                                         \
+        if (form == GBGPU_FORM_MFMA) {                                                           \
+            if (big) k_sparse_blocked_batch<WS, true><<<grid, 64, 0, s>>>(mats, row_off, nmat, queue, xs, xs_stride); \
+            else k_sparse_blocked_batch<WS, false><<<grid, 64, 0, s>>>(mats, row_off, nmat, queue, xs, xs_stride); \
+        } else {                                                                                 \
+            if (big) k_sparse_plain_batch<WS, true><<<grid, 64, 0, s>>>(mats, row_off, nmat, queue, xs, xs_stride); \
+            else k_sparse_plain_batch<WS, false><<<grid, 64, 0, s>>>(mats, row_off, nmat, queue, xs, xs_stride); \
+        }                                                                                        \
+    } while (0)
+    switch (wshift) {
+        case 11: GBGPU_BATCH_SP(11); break;
+        case 12: GBGPU_BATCH_SP(12); break;
+        default: GBGPU_BATCH_SP(13); break;
+    }
+#undef GBGPU_BATCH_SP
+    GBGPU_HIP(ctx, hipGetLastError());
+    return GBGPU_OK;
+}
+
+int sparse_wave_lds(int wshift, int form) {
+    const int W = 1 << wshift;
+    return W * 8 + (form == GBGPU_FORM_MFMA ? 0 : STAGE * 8) + 2048;
 }
 
 int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {

@@ -80,6 +80,11 @@ def lib():
         "gbgpu_dense_free": ([vp], None),
         "gbgpu_dense_rref": ([vp, vp, C.c_int, pi32], C.c_int),
         "gbgpu_dense_download": ([vp, vp, vp, vp], C.c_int),
+        "gbgpu_batch_upload": ([vp, C.POINTER(_Csr), i32, C.POINTER(vp)], C.c_int),
+        "gbgpu_batch_la": ([vp, vp, C.c_int, pi32], C.c_int),
+        "gbgpu_batch_rows_download": ([vp, vp, i32, C.POINTER(_Rows)], C.c_int),
+        "gbgpu_batch_info": ([vp, pi32, i32], C.c_int),
+        "gbgpu_batch_free": ([vp], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -162,6 +167,20 @@ class Context:
         self._check(lib().gbgpu_la_ff(self.h, C.byref(s), la_option, form, C.byref(r)), "la_ff")
         return _take_rows(r)
 
+    def batch(self, mats) -> "DeviceBatch":
+        """Upload many F4 matrices (each with its own prime) as one batch."""
+        mats = list(mats)
+        structs, keep = [], []
+        for m in mats:
+            st, k = _csr(m)
+            structs.append(st)
+            keep.append(k)
+        arr = (_Csr * max(len(structs), 1))(*structs)
+        h = C.c_void_p()
+        self._check(lib().gbgpu_batch_upload(self.h, arr, len(structs), C.byref(h)),
+                    "batch upload")
+        return DeviceBatch(self, h, len(mats))
+
     def dense(self, a: np.ndarray, p: int) -> "DeviceDense":
         a = np.ascontiguousarray(a, dtype=np.uint32)
         h = C.c_void_p()
@@ -226,6 +245,44 @@ class DeviceMatrix:
         r = _Rows()
         self.ctx._check(lib().gbgpu_rows_download(self.ctx.h, self.h, C.byref(r)), "rows download")
         return _take_rows(r)
+
+
+class DeviceBatch:
+    """Many F4 matrices resident on the device (gbgpu_batch)."""
+
+    def __init__(self, ctx: Context, h, nmat: int):
+        self.ctx, self.h, self.nmat = ctx, h, nmat
+
+    def free(self):
+        if self.h:
+            lib().gbgpu_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def la(self, form: int = FORM_MFMA) -> np.ndarray:
+        """la_option 1 on every matrix (one graph launch); new pivots per matrix."""
+        npv = np.zeros(max(self.nmat, 1), dtype=np.int32)
+        self.ctx._check(lib().gbgpu_batch_la(self.ctx.h, self.h, form,
+                                             npv.ctypes.data_as(C.POINTER(C.c_int32))),
+                        "batch la")
+        return npv[: self.nmat]
+
+    def rows(self, i: int) -> SparseRows:
+        r = _Rows()
+        self.ctx._check(lib().gbgpu_batch_rows_download(self.ctx.h, self.h, i, C.byref(r)),
+                        "batch rows download")
+        return _take_rows(r)
+
+    def info(self) -> dict:
+        v = np.zeros(5, dtype=np.int32)
+        lib().gbgpu_batch_info(self.h, v.ctypes.data_as(C.POINTER(C.c_int32)), 5)
+        return {"matrices": int(v[0]), "window_shift": int(v[1]), "waves_per_cu": int(v[2]),
+                "sparse_grid": int(v[3]), "replays": int(v[4])}
 
 
 class DeviceDense:

@@ -174,6 +174,31 @@ int gbgpu_dense_rref(gbgpu_ctx *ctx, gbgpu_dense *d, int form, int32_t *rank);
 int gbgpu_dense_download(gbgpu_ctx *ctx, gbgpu_dense *d, uint32_t *rows,
                          int32_t *lead);
 
+/* ---- batches: la_option 1 over many matrices, each modulo its own prime.
+ * No counterpart in the reference (its rational path is fraction-free over
+ * GMP, la_qq.c:271-362); the per-matrix result is exactly what
+ * gbgpu_la_sparse_dense gives for that matrix (rows in decreasing lead
+ * order, columns shifted by ncl). */
+typedef struct gbgpu_batch gbgpu_batch;
+
+/* Upload nmat matrices (each checked as by gbgpu_mat_upload). */
+int gbgpu_batch_upload(gbgpu_ctx *ctx, const gbgpu_csr_t *mats, int32_t nmat,
+                       gbgpu_batch **out);
+/* la_option 1 on every matrix as one graph launch: one persistent sparse
+ * pass over all matrices' rows, then the dense panel loop of all blocks.
+ * The first call in a form lays the batch out on the device, tunes the
+ * sparse pass's window width and waves per CU on it (kept per process for
+ * batches of the same shape) and captures the graph; later calls replay
+ * it.  np[i] receives the new pivots of matrix i (may be NULL).
+ * gbgpu_phase_ms(ctx, 2) is the graph's time on the device. */
+int gbgpu_batch_la(gbgpu_ctx *ctx, gbgpu_batch *b, int form, int32_t *np);
+/* Rows of matrix i's result, as gbgpu_rows_download. */
+int gbgpu_batch_rows_download(gbgpu_ctx *ctx, gbgpu_batch *b, int32_t i,
+                              gbgpu_rows_t *out);
+/* out[0..nout): matrices, window shift, waves per CU, sparse grid, replays. */
+int gbgpu_batch_info(gbgpu_batch *b, int32_t *out, int32_t nout);
+void gbgpu_batch_free(gbgpu_batch *b);
+
 #ifdef __cplusplus
 }
 #endif
