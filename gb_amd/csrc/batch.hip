@@ -36,6 +36,7 @@ struct gbgpu_batch {
     int form = -1;                  // form the device state is laid out for
     int wshift = 0, waves_per_cu = 0;
     bool big = false;               // some prime needs reduced products
+    int nd = 0;                     // digits of the MFMA form's dense update
     // sparse pass
     gbgpu::BlockedArgs *d_args = nullptr;
     int32_t *d_rowoff = nullptr;
@@ -207,6 +208,11 @@ int layout_dense(gbgpu_batch *b, int form) {
     const int32_t nmat = (int32_t)b->mats.size();
     b->hd.assign(nmat, DenseDesc{});
     int32_t max_n = 0;
+    // one digit count for the whole batch (enough for its largest prime)
+    int nd = 0;
+    if (form == GBGPU_FORM_MFMA)
+        for (auto *m : b->mats) nd = std::max(nd, digits_for(m->p));
+    b->nd = nd;
     for (int32_t i = 0; i < nmat; ++i) {
         gbgpu_mat *m = b->mats[b->order[i]];
         DenseDesc &d = b->hd[i];
@@ -224,6 +230,14 @@ int layout_dense(gbgpu_batch *b, int form) {
             (rc = balloc(b, &d.active, (size_t)mr)) || (rc = balloc(b, &d.cand, (size_t)mr)) ||
             (rc = balloc(b, &d.L, (size_t)mr * 64)) || (rc = balloc(b, &d.rp, (size_t)64 * d.ldr)))
             return rc;
+        if (nd) {
+            digits_setup(d.dg, m->p, nd);
+            d.dg.lplane = (int64_t)mr * 64;
+            d.dg.rplane = d.ldr * 64;
+            if ((rc = balloc(b, &d.dg.L, (size_t)nd * d.dg.lplane)) ||
+                (rc = balloc(b, &d.dg.R, (size_t)nd * d.dg.rplane)))
+                return rc;
+        }
         void *pan = pool_alloc(dense_panel_bytes());
         void *st = pool_alloc(dense_liststate_bytes());
         if (pan) b->dense_bufs.push_back(pan);
@@ -257,7 +271,8 @@ int enqueue_run(gbgpu_batch *b, int form) {
     if ((rc = launch_sparse(b, form))) return rc;
     if ((rc = batch_dense_init(ctx, b->d_dense, nmat))) return rc;
     for (int32_t k = 0; k < b->nsteps; ++k)
-        if ((rc = batch_dense_step(ctx, b->d_dense, nmat, b->d_tab, b->htab.data(), k, form, b->big)))
+        if ((rc = batch_dense_step(ctx, b->d_dense, nmat, b->d_tab, b->htab.data(), k, form, b->big,
+                                   b->nd)))
             return rc;
     return GBGPU_OK;
 }

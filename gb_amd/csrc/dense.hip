@@ -69,6 +69,17 @@ __device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p) 
     return a >= b ? a - b : a + p - b;
 }
 
+__device__ __forceinline__ uint32_t digit_word(uint32_t x, uint32_t p, uint32_t hi, int i) {
+    // digit i of x's representative (as an unsigned byte)
+    int64_t v = x <= hi ? (int64_t)x : (int64_t)x - (int64_t)p;
+    int8_t d = 0;
+    for (int k = 0; k <= i; ++k) {
+        d = (int8_t)(uint8_t)(v & 0xFF);
+        v = (v - d) >> 8;
+    }
+    return (uint8_t)d;
+}
+
 // ------------------------------------------------------------ select
 //
 // One workgroup finds the panel's pivots.  Candidate rows (rows that are no
@@ -389,7 +400,7 @@ __global__ __launch_bounds__(256) void k_prune_compact(int32_t *cand, ListState 
 // and reduced once per output.
 __device__ __forceinline__ void rp_body(uint32_t *a, int64_t ld, int32_t n, int32_t c0,
                                         const Panel *pan, uint32_t *rp, int64_t ldr, Mod md,
-                                        uint32_t r64, int32_t tile) {
+                                        uint32_t r64, int32_t tile, const Digits &dg) {
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
     __shared__ uint32_t T[PB][PB + 1];
@@ -430,12 +441,32 @@ __device__ __forceinline__ void rp_body(uint32_t *a, int64_t ld, int32_t n, int3
             }
         }
     }
+    // digits of Rp^T: column j - c0, rows t = 16 wv .. 16 wv + 15 (zero
+    // beyond k), one 16-byte store per digit
+    if (dg.nd > 0 && j < n) {
+        for (int i = 0; i < dg.nd; ++i) {
+            uint32_t wd[4];
+#pragma unroll
+            for (int q4 = 0; q4 < 4; ++q4) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int q = 4 * q4 + b;
+                    const uint32_t r = (16 * wv + q) < kk ? res[q] : 0u;
+                    v |= digit_word(r, md.p, dg.hi, i) << (8 * b);
+                }
+                wd[q4] = v;
+            }
+            uint4 *dst = (uint4 *)(dg.R + i * dg.rplane + (int64_t)(j - c0) * 64 + 16 * wv);
+            *dst = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_panel_rp(uint32_t *a, int64_t ld, int32_t n, int32_t c0,
                                                   const Panel *pan, uint32_t *rp, int64_t ldr,
-                                                  Mod md, uint32_t r64) {
-    rp_body(a, ld, n, c0, pan, rp, ldr, md, r64, blockIdx.x);
+                                                  Mod md, uint32_t r64, Digits dg) {
+    rp_body(a, ld, n, c0, pan, rp, ldr, md, r64, blockIdx.x, dg);
 }
 
 // ------------------------------------------------------------ gather
@@ -445,7 +476,7 @@ __global__ __launch_bounds__(256) void k_panel_rp(uint32_t *a, int64_t ld, int32
 __device__ __forceinline__ void gather_body(const uint32_t *a, int64_t ld, int32_t m, int32_t c0,
                                             int32_t bw, const int32_t *rowpiv, const Panel *pan,
                                             uint32_t *L, int32_t *active, int32_t first,
-                                            int32_t step) {
+                                            int32_t step, uint32_t p, const Digits &dg) {
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
     const int lane = threadIdx.x & 63;
@@ -455,15 +486,18 @@ __device__ __forceinline__ void gather_body(const uint32_t *a, int64_t ld, int32
         uint32_t v = 0;
         if (!cur && lane < kk) v = a[(int64_t)r * ld + pan->ecol[lane]];
         L[(int64_t)r * PB + lane] = v;
+        for (int i = 0; i < dg.nd; ++i)
+            dg.L[i * dg.lplane + (int64_t)r * PB + lane] = (int8_t)digit_word(v, p, dg.hi, i);
         active[r] = __any(v != 0) ? 1 : 0;  // every lane stores the same flag
     }
 }
 
 __global__ void k_panel_gather(const uint32_t *a, int64_t ld, int32_t m, int32_t c0, int32_t bw,
                                const int32_t *rowpiv, const Panel *pan, uint32_t *L,
-                               int32_t *active) {
+                               int32_t *active, uint32_t p, Digits dg) {
     gather_body(a, ld, m, c0, bw, rowpiv, pan, L, active,
-                blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), gridDim.x * (blockDim.x >> 6));
+                blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), gridDim.x * (blockDim.x >> 6),
+                p, dg);
 }
 
 // ------------------------------------------------- update (integer ALU)
@@ -531,143 +565,91 @@ __global__ __launch_bounds__(256) void k_update_plain(uint32_t *a, int64_t ld, i
                            blockIdx.y);
 }
 
-// --------------------------------------------------- update (MFMA f64)
+// ------------------------------------------------- update (MFMA int8)
 
-// Block: 4 waves, 128 rows x 64 columns; each wave a 32 x 64 tile held as
-// 2 x 4 accumulators of v_mfma_f64_16x16x4f64 (A 16x4, B 4x16, C 16x16).
-// Operand layout per lane l: A[l % 16][l / 16], B[l / 16][l % 16]; result
-// register e of lane l is C[4 e + l / 16][l % 16] (measured on gfx950 with
-// an in-bounds probe of the builtin).
-// Exactness: operands are integers below 2^17 (p < 2^17) or 16-bit halves
-// of residues (p >= 2^17), so every 64-term dot product is an integer below
-// 2^40 (2^36 for halves) and FP64 represents it and every partial sum
-// exactly.
-typedef double dx4 __attribute__((ext_vector_type(4)));
+// Block: 4 independent waves, 64 rows x 64 columns; each wave a 32 x 32
+// tile of 2 x 2 products v_mfma_i32_16x16x64_i8 (A 16 x 64, B 64 x 16,
+// C 16 x 16; K = the panel's 64 pivot slots).  With L = sum_i 256^i Li and
+// Rp = sum_j 256^j Rj in signed digits, L Rp = sum_s 256^s Cs with
+// Cs = sum_{i+j=s} Li Rj: nd^2 products into 2 nd - 1 int32 accumulators,
+// each sum below nd 64 128^2 = nd 2^20 and so exact.  The combination
+// sum_s Cs (256^s mod p) is formed in 64-bit integers and reduced once.
+// Operand layout per lane l: A row l % 16, bytes 16 (l / 16) .. + 15 of the
+// 64 slots; B column l % 16, the same slots; result register e of lane l
+// is C[4 (l / 16) + e][l % 16] (measured on gfx950 with an in-bounds probe
+// of the builtin).  No LDS and no barrier: the operands are 16-byte loads
+// of the digit planes, which stay in L2 across the grid.
+typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int MR = 128;  // rows per block
-
-__device__ __forceinline__ dx4 mfma16(double a, double b, dx4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-// part: 0 = whole value, 1 = high 16 bits, 2 = low 16 bits
-__device__ __forceinline__ double piece(uint32_t v, int part) {
-    return part == 0 ? (double)v : part == 1 ? (double)(v >> 16) : (double)(v & 0xFFFFu);
-}
-
-__device__ __forceinline__ void tile_product(const uint32_t (*Ls)[PB + 1], const uint32_t (*Rs)[PB + 1],
-                                             int kp, int lpart, int rpart, dx4 acc[2][4]) {
-    const int lane = threadIdx.x & 63;
-    const int lr = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
-    for (int t0 = 0; t0 < kp; t0 += 4) {
-        double av[2], bv[4];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) av[i] = piece(Ls[16 * i + lr][t0 + lk], lpart);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) bv[j] = piece(Rs[t0 + lk][16 * j + lr], rpart);
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = mfma16(av[i], bv[j], acc[i][j]);
-    }
-}
-
-template <bool SPLIT>
-__device__ __forceinline__ void update_mfma_body(uint32_t *a, int64_t ld, int32_t m, int32_t cb,
-                                                 int32_t ce, int32_t c0, const Panel *pan,
-                                                 const uint32_t *rp, int64_t ldr,
-                                                 const uint32_t *L, const int32_t *active,
-                                                 Mod md, int32_t bx, int32_t by) {
+template <int ND>
+__device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t m, int32_t cb,
+                                               int32_t ce, int32_t c0, const Panel *pan,
+                                               const int32_t *active, Mod md, const Digits &dg,
+                                               int32_t bx, int32_t by) {
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
-    __shared__ uint32_t Rs[PB][PB + 1];
-    __shared__ uint32_t Ls[MR][PB + 1];
-    __shared__ int32_t act[MR];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int32_t jc = cb + bx * 64;
-    const int32_t rb = by * MR;
-    if (tid < MR) act[tid] = (rb + tid < m) ? active[rb + tid] : 0;
-    __syncthreads();
-    int anyact = 0;
-    for (int i = 0; i < MR; ++i) anyact |= act[i];
-    if (!anyact) return;  // uniform over the block
-    const int kp = (kk + 3) & ~3;
-    for (int i = tid; i < PB * 64; i += 256) {
-        const int t = i >> 6, c = i & 63;
-        Rs[t][c] = (t < kk && jc + c < ce) ? rp[(int64_t)t * ldr + (jc + c - c0)] : 0u;
-    }
-    for (int i = tid; i < MR * PB; i += 256) {
-        const int rr = i >> 6, t = i & 63;
-        Ls[rr][t] = (t < kk && act[rr]) ? L[(int64_t)(rb + rr) * PB + t] : 0u;
-    }
-    __syncthreads();
-    const uint32_t (*Lw)[PB + 1] = Ls + 32 * wv;
-    uint32_t res[2][4][4];
-    dx4 acc[2][4];
-    if (!SPLIT) {
-        tile_product(Lw, Rs, kp, 0, 0, acc);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int32_t r0 = by * 64 + (wv >> 1) * 32;
+    const int32_t q0 = cb + bx * 64 + (wv & 1) * 32;
+    // rows of this wave with a nonzero multiplier (wave-uniform exit)
+    const bool act = lane < 32 && r0 + lane < m && active[r0 + lane] != 0;
+    if (!__any(act) || q0 >= ce) return;
+    const int lr = lane & 15, lk = lane >> 4;
+    v4i A[ND][2], B[ND][2];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < ND; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) res[i][j][e] = md.reduce((uint64_t)acc[i][j][e]);
-    } else {
-        // L R = (Lh Rh) 2^32 + (Lh Rl + Ll Rh) 2^16 + Ll Rl
-        const uint32_t s16 = (uint32_t)((1ull << 16) % md.p);
-        const uint32_t s32 = (uint32_t)((1ull << 32) % md.p);
-        const int parts[4][2] = {{1, 1}, {1, 2}, {2, 1}, {2, 2}};
-        const uint32_t scale[4] = {s32, s16, s16, 1u};
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) res[i][j][e] = 0;
-        for (int q = 0; q < 4; ++q) {
-            tile_product(Lw, Rs, kp, parts[q][0], parts[q][1], acc);
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const uint32_t v = md.mul(md.reduce((uint64_t)acc[i][j][e]), scale[q]);
-                        const uint32_t s = res[i][j][e] + v;
-                        res[i][j][e] = s >= md.p ? s - md.p : s;
-                    }
+        for (int h = 0; h < 2; ++h) {
+            const int32_t row = r0 + 16 * h + lr;
+            const int32_t col = q0 + 16 * h + lr;
+            A[i][h] = row < m ? *(const v4i *)(dg.L + i * dg.lplane + (int64_t)row * 64 + 16 * lk)
+                              : v4i{0, 0, 0, 0};
+            B[i][h] = col < ce ? *(const v4i *)(dg.R + i * dg.rplane + (int64_t)(col - c0) * 64 + 16 * lk)
+                               : v4i{0, 0, 0, 0};
         }
-    }
+    v4i acc[2][2][2 * ND - 1];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int rs = 0; rs < 2; ++rs)
+#pragma unroll
+        for (int cs = 0; cs < 2; ++cs)
+#pragma unroll
+            for (int q = 0; q < 2 * ND - 1; ++q) acc[rs][cs][q] = v4i{0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+#pragma unroll
+        for (int j = 0; j < ND; ++j)
+#pragma unroll
+            for (int rs = 0; rs < 2; ++rs)
+#pragma unroll
+                for (int cs = 0; cs < 2; ++cs)
+                    acc[rs][cs][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                        A[i][rs], B[j][cs], acc[rs][cs][i + j], 0, 0, 0);
+#pragma unroll
+    for (int rs = 0; rs < 2; ++rs)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int rloc = 32 * wv + 16 * i + 4 * e + (lane >> 4);
-            const int32_t r = rb + rloc;
-            if (r >= m || !act[rloc]) continue;
+            const int32_t row = r0 + 16 * rs + 4 * lk + e;
+            if (row >= m || !active[row]) continue;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int32_t c = jc + 16 * j + (lane & 15);
-                if (c >= ce) continue;
-                uint32_t *x = a + (int64_t)r * ld + c;
-                *x = sub_mod(*x, res[i][j][e], md.p);
+            for (int cs = 0; cs < 2; ++cs) {
+                const int32_t col = q0 + 16 * cs + lr;
+                if (col >= ce) continue;
+                int64_t v = 0;
+#pragma unroll
+                for (int q = 0; q < 2 * ND - 1; ++q) v += (int64_t)acc[rs][cs][q][e] * dg.w[q];
+                const uint32_t u = md.reduce((uint64_t)(v + (int64_t)dg.off));
+                uint32_t *x = a + (int64_t)row * ld + col;
+                *x = sub_mod(*x, u, md.p);
             }
         }
 }
 
-template <bool SPLIT>
+template <int ND>
 __global__ __launch_bounds__(256) void k_update_mfma(uint32_t *a, int64_t ld, int32_t m,
                                                      int32_t cb, int32_t ce, int32_t c0,
-                                                     const Panel *pan,
-                                                     const uint32_t *rp, int64_t ldr,
-                                                     const uint32_t *L, const int32_t *active,
-                                                     Mod md) {
-    update_mfma_body<SPLIT>(a, ld, m, cb, ce, c0, pan, rp, ldr, L, active, md, blockIdx.x,
-                            blockIdx.y);
+                                                     const Panel *pan, const int32_t *active,
+                                                     Mod md, Digits dg) {
+    update_i8_body<ND>(a, ld, m, cb, ce, c0, pan, active, md, dg, blockIdx.x, blockIdx.y);
 }
 
 // ------------------------------------------------------------ output
@@ -727,29 +709,42 @@ int dense_rref(gbgpu_dense *d, int form) {
     d->rank = 0;
     int32_t *rowpiv = nullptr, *colpiv = nullptr, *active = nullptr, *cand = nullptr;
     uint32_t *L = nullptr, *rp = nullptr;
+    int8_t *dl = nullptr, *dr = nullptr;
     Panel *pan = nullptr;
     ListState *st = nullptr;
     const int64_t ldr = ((int64_t)n + 63) & ~63ll;
     auto cleanup = [&]() {
         dfree(rowpiv); dfree(colpiv); dfree(active); dfree(cand); dfree(L); dfree(rp);
-        dfree(pan); dfree(st);
+        dfree(dl); dfree(dr); dfree(pan); dfree(st);
     };
+    Digits dg;
+    if (form == GBGPU_FORM_MFMA) digits_setup(dg, d->p, digits_for(d->p));
     if ((rc = dalloc(ctx, &rowpiv, (size_t)m)) || (rc = dalloc(ctx, &colpiv, (size_t)n)) ||
         (rc = dalloc(ctx, &active, (size_t)m)) || (rc = dalloc(ctx, &cand, (size_t)m)) ||
         (rc = dalloc(ctx, &L, (size_t)m * PB)) || (rc = dalloc(ctx, &rp, (size_t)PB * ldr)) ||
+        (dg.nd && ((rc = dalloc(ctx, &dl, (size_t)dg.nd * m * PB)) ||
+                   (rc = dalloc(ctx, &dr, (size_t)dg.nd * ldr * PB)))) ||
         (rc = dalloc(ctx, &pan, 2)) || (rc = dalloc(ctx, &st, 1))) {
         cleanup();
         return rc;
     }
+    dg.L = dl;
+    dg.R = dr;
+    dg.lplane = (int64_t)m * PB;
+    dg.rplane = ldr * PB;
     k_dense_init<<<grid_of(m > n ? m : n, 256), 256, 0, s>>>(m, n, rowpiv, colpiv, cand, pan, st);
     const uint32_t r64 = (uint32_t)((((unsigned __int128)1) << 64) % d->p);
     // A[r][cb, ce) -= L[r] * Rp for the rows of the current panel's update
     auto update = [&](const Panel *pp, int32_t c0, int32_t cb, int32_t ce) {
         if (cb >= ce) return;
         if (form == GBGPU_FORM_MFMA) {
-            dim3 g((unsigned)((ce - cb + 63) / 64), (unsigned)((m + MR - 1) / MR));
-            if (big) k_update_mfma<true><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, rp, ldr, L, active, md);
-            else k_update_mfma<false><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, rp, ldr, L, active, md);
+            dim3 g((unsigned)((ce - cb + 63) / 64), (unsigned)((m + 63) / 64));
+            switch (dg.nd) {
+                case 1: k_update_mfma<1><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg); break;
+                case 2: k_update_mfma<2><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg); break;
+                case 3: k_update_mfma<3><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg); break;
+                default: k_update_mfma<4><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg); break;
+            }
         } else {
             dim3 g((unsigned)((ce - cb + 255) / 256), (unsigned)((m + RT - 1) / RT));
             if (big) k_update_plain<true><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, rp, ldr, L, active, md, r64);
@@ -772,8 +767,9 @@ int dense_rref(gbgpu_dense *d, int form) {
         const int bw = (n - c0) < PB ? (n - c0) : PB;
         const int32_t c1 = c0 + bw;                       // next panel
         const int32_t c2 = (n - c1) < PB ? n : c1 + PB;
-        k_panel_rp<<<grid_of(n - c0, 64), 256, 0, s>>>(d->a, d->ld, n, c0, cur, rp, ldr, md, r64);
-        k_panel_gather<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, m, c0, bw, rowpiv, cur, L, active);
+        k_panel_rp<<<grid_of(n - c0, 64), 256, 0, s>>>(d->a, d->ld, n, c0, cur, rp, ldr, md, r64, dg);
+        k_panel_gather<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, m, c0, bw, rowpiv, cur, L, active,
+                                                      d->p, dg);
         if (c1 < n) {
             update(cur, c0, c1, c2);
             if (hipEventRecord(ctx->sync_ev[0], s) != hipSuccess ||
@@ -911,7 +907,7 @@ __global__ __launch_bounds__(256) void k_batch_rp(const DenseDesc *D, const int3
                                                   int32_t nmat, int32_t c0) {
     const int i = tab_find(pre, nmat, blockIdx.x);
     const DenseDesc &d = D[i];
-    rp_body(d.a, d.ld, d.n, c0, d.pan, d.rp, d.ldr, d.md, d.r64, blockIdx.x - pre[i]);
+    rp_body(d.a, d.ld, d.n, c0, d.pan, d.rp, d.ldr, d.md, d.r64, blockIdx.x - pre[i], d.dg);
 }
 
 __global__ void k_batch_gather(const DenseDesc *D, const int32_t *pre, int32_t nmat, int32_t c0) {
@@ -920,18 +916,17 @@ __global__ void k_batch_gather(const DenseDesc *D, const int32_t *pre, int32_t n
     const int32_t nb = pre[i + 1] - pre[i];
     const int32_t bw = min(PB, d.n - c0);
     gather_body(d.a, d.ld, d.m, c0, bw, d.rowpiv, d.pan, d.L, d.active,
-                (blockIdx.x - pre[i]) * 4 + (threadIdx.x >> 6), nb * 4);
+                (blockIdx.x - pre[i]) * 4 + (threadIdx.x >> 6), nb * 4, d.md.p, d.dg);
 }
 
-template <bool SPLIT>
+template <int ND>
 __global__ __launch_bounds__(256) void k_batch_update_mfma(const DenseDesc *D, const int32_t *pre,
                                                            int32_t nmat, int32_t c0) {
     const int i = tab_find(pre, nmat, blockIdx.x);
     const DenseDesc &d = D[i];
     const int32_t l = blockIdx.x - pre[i];
     const int32_t ct = (d.n - c0 + 63) / 64;
-    update_mfma_body<SPLIT>(d.a, d.ld, d.m, c0, d.n, c0, d.pan, d.rp, d.ldr, d.L, d.active, d.md,
-                            l % ct, l / ct);
+    update_i8_body<ND>(d.a, d.ld, d.m, c0, d.n, c0, d.pan, d.active, d.md, d.dg, l % ct, l / ct);
 }
 
 template <bool BIG>
@@ -970,7 +965,7 @@ int batch_dense_init(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat) {
 }
 
 int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int32_t *tab,
-                     const int32_t *htab, int32_t step, int form, bool big) {
+                     const int32_t *htab, int32_t step, int form, bool big, int nd) {
     hipStream_t s = ctx->stream;
     const int32_t c0 = step * PB;
     const int32_t *pre = tab + (int64_t)step * 4 * (nmat + 1);
@@ -983,8 +978,12 @@ int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int
     if (nup > 0) {
         const int32_t *pu = pre + 2 * (nmat + 1);
         if (form == GBGPU_FORM_MFMA) {
-            if (big) k_batch_update_mfma<true><<<nup, 256, 0, s>>>(D, pu, nmat, c0);
-            else k_batch_update_mfma<false><<<nup, 256, 0, s>>>(D, pu, nmat, c0);
+            switch (nd) {
+                case 1: k_batch_update_mfma<1><<<nup, 256, 0, s>>>(D, pu, nmat, c0); break;
+                case 2: k_batch_update_mfma<2><<<nup, 256, 0, s>>>(D, pu, nmat, c0); break;
+                case 3: k_batch_update_mfma<3><<<nup, 256, 0, s>>>(D, pu, nmat, c0); break;
+                default: k_batch_update_mfma<4><<<nup, 256, 0, s>>>(D, pu, nmat, c0); break;
+            }
         } else {
             if (big) k_batch_update_plain<true><<<nup, 256, 0, s>>>(D, pu, nmat, c0);
             else k_batch_update_plain<false><<<nup, 256, 0, s>>>(D, pu, nmat, c0);
@@ -1011,7 +1010,7 @@ void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int for
         const int32_t cap = rows4 < 2048 ? rows4 : 2048;
         int32_t nup = 0;
         if (on) {
-            if (form == GBGPU_FORM_MFMA) nup = ((n - c0 + 63) / 64) * ((m + MR - 1) / MR);
+            if (form == GBGPU_FORM_MFMA) nup = ((n - c0 + 63) / 64) * ((m + 63) / 64);
             else nup = ((n - c0 + 255) / 256) * ((m + RT - 1) / RT);
         }
         prp[i + 1] = prp[i] + (on ? (n - c0 + 63) / 64 : 0);

@@ -117,6 +117,40 @@ int dense_rref(gbgpu_dense *d, int form);
 // eliminated block (decreasing lead order)
 int dense_output(gbgpu_dense *d, const int32_t *colpiv);
 
+// Signed base-256 digits for the int8 matrix cores.  With nd digits in
+// [-128, 127] the integers 127 S .. -128 S are reachable (S = 1 + 256 + ...
+// + 256^(nd-1)), a window of 256^nd consecutive integers; so for
+// p <= 256^nd every residue x has a representative in it: x itself when
+// x <= 127 S, else x - p.  nd = 1 for p <= 255, 2 for p <= 65536, 3 for
+// p <= 2^24, 4 below 2^31.  The digit planes hold, per digit, L as
+// [row][64] and Rp transposed as [column][64], so that one lane's 16
+// consecutive bytes are an MFMA operand.
+struct Digits {
+    int8_t *L = nullptr;   // nd planes of m x 64
+    int8_t *R = nullptr;   // nd planes of ldr x 64
+    int64_t lplane = 0;    // bytes per L plane
+    int64_t rplane = 0;    // bytes per R plane
+    int nd = 0;            // 0: no digit planes (plain form)
+    uint32_t hi = 0;       // 127 S
+    uint64_t off = 0;      // multiple of p above any combined product
+    uint32_t w[7] = {};    // 256^s mod p
+};
+
+__host__ inline int digits_for(uint32_t p) {
+    return p <= 255u ? 1 : p <= 65536u ? 2 : p <= (1u << 24) ? 3 : 4;
+}
+
+__host__ inline void digits_setup(Digits &g, uint32_t p, int nd) {
+    g.nd = nd;
+    uint64_t S = 0, pw = 1;
+    for (int i = 0; i < nd; ++i, pw *= 256) S += pw;
+    g.hi = (uint32_t)(127 * S);
+    uint64_t w = 1 % p;
+    for (int s_ = 0; s_ < 7; ++s_, w = (w * 256) % p) g.w[s_] = (uint32_t)w;
+    // |sum_s C_s w_s| < (2 nd - 1) nd 64 128^2 p < 2^56
+    g.off = ((1ull << 56) / p + 1) * p;
+}
+
 // dense.hip, batches: one dense block of a batch and its elimination state
 struct Panel;
 struct ListState;
@@ -133,13 +167,15 @@ struct DenseDesc {
     Panel *pan;                // one panel record
     ListState *st;
     const int32_t *dnz;        // m: rows that take part (nullptr: all)
+    Digits dg;                 // digit planes (MFMA form)
 };
 int batch_dense_init(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat);
 // block counts of step `step` for the four grids (rp, gather, update,
 // prune), prefix sums over the matrices: out[4 * (nmat + 1)]
 void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int form, int32_t *out);
+// nd: digits of the MFMA form's update (the largest of the batch)
 int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int32_t *tab,
-                     const int32_t *htab, int32_t step, int form, bool big);
+                     const int32_t *htab, int32_t step, int form, bool big, int nd);
 size_t dense_panel_bytes();      // sizeof(Panel)
 size_t dense_liststate_bytes();  // sizeof(ListState)
 

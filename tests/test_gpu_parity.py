@@ -11,7 +11,7 @@ from oracle import oracle
 
 pytestmark = pytest.mark.gpu
 
-PRIMES = [127, 251, 32003, 65521, 2147483629, 2147483647]
+PRIMES = [127, 251, 32003, 65521, 131071, 2147483629, 2147483647]
 FORMS = [FORM_PLAIN, FORM_MFMA]
 
 
@@ -68,7 +68,7 @@ def test_la_ff_one_call_all_options(gpu_lib, la_option):
     assert got.equals(exp)
 
 
-@pytest.mark.parametrize("p", [251, 65521, 2147483629])
+@pytest.mark.parametrize("p", [251, 65521, 131071, 16777213, 2147483629])
 @pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("shape,rank", [((64, 100), None), ((200, 330), None), ((330, 200), None),
                                         ((256, 520), 100), ((130, 70), 69)])
