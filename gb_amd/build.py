@@ -20,7 +20,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          "-I", os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-function"]
 
 SOURCES = ["api.hip", "sparse.hip", "dense.hip", "rows.hip", "batch.hip"]
-C_SOURCES = ["gb_adapter.c"]
+C_SOURCES = ["gb_adapter.c", "f4.c"]
 
 
 def _compile(src: str) -> str:

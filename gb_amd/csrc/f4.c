@@ -1,0 +1,980 @@
+/* f4.c -- the F4 host around libgbgpu's linear algebra (C11).
+ *
+ * The loop of gb's f4_julia (f4.c:190-372): select the pairs of minimal
+ * degree, build the matrix by symbolic preprocessing, reduce it, turn the
+ * new rows into basis elements, update the pairs (Gebauer-Moeller); at the
+ * end drop redundant elements and, if asked, interreduce.  Own design
+ * throughout; where the matrix has to look like gb's (row format, known
+ * pivot = first row per lead, columns = pivot columns first), it does.
+ *
+ *  * Monomials live in one table: exponent vectors (one byte per variable),
+ *    degree, a divisibility mask (bit i set when x_i occurs) and an
+ *    additive hash h(e) = sum_i r_i e_i, so h(a b) = h(a) + h(b) and a
+ *    product is looked up without hashing its exponents again.
+ *  * A basis element keeps its terms as monomial ids in decreasing order
+ *    and its coefficients (leading one 1).  A matrix row is (element,
+ *    multiplier): its coefficient array is the element's, shared by all
+ *    its multiples -- as in gb (hash.c:903-926).
+ *  * Pairs: the Gebauer-Moeller installation of new elements (Becker and
+ *    Weispfenning, Groebner Bases, 1993, algorithm UPDATE).
+ *  * Symbolic preprocessing: rows of the selected pairs (both generators
+ *    of every pair, one row per distinct (element, multiplier)), then a
+ *    reducer for every column monomial divisible by a basis lead term --
+ *    the first element in basis order whose lead divides it, the rule of
+ *    gb's find_multiplied_reducer (symbol.c:151-200).
+ *  * Columns: pivot columns (leads of rows) first, then the others, each
+ *    block in decreasing monomial order; rows sorted by (lead, length)
+ *    (order.c:98-131). */
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "../../include/gbgpu_f4.h"
+
+static double now_ms(void) {
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return 1e3 * (double)t.tv_sec + 1e-6 * (double)t.tv_nsec;
+}
+
+/* ------------------------------------------------------------ monomials */
+
+typedef struct {
+    int nv, nvp, order;
+    uint8_t *ev;     /* nvp bytes per monomial */
+    uint32_t *deg;
+    uint64_t *hv;
+    uint64_t *dm;
+    int32_t n, cap;
+    int32_t *slot;   /* id + 1, 0 = empty */
+    uint64_t smask;
+    uint64_t rnd[64];
+    uint8_t *tmp;    /* nvp scratch bytes */
+} mt_t;
+
+static int mt_init(mt_t *t, int nv, int order) {
+    memset(t, 0, sizeof(*t));
+    t->nv = nv;
+    t->nvp = (nv + 7) & ~7;
+    t->order = order;
+    t->cap = 1 << 12;
+    t->ev = (uint8_t *)calloc((size_t)t->cap, (size_t)t->nvp);
+    t->deg = (uint32_t *)malloc((size_t)t->cap * sizeof(uint32_t));
+    t->hv = (uint64_t *)malloc((size_t)t->cap * sizeof(uint64_t));
+    t->dm = (uint64_t *)malloc((size_t)t->cap * sizeof(uint64_t));
+    t->smask = (1u << 13) - 1;
+    t->slot = (int32_t *)calloc((size_t)t->smask + 1, sizeof(int32_t));
+    t->tmp = (uint8_t *)calloc(1, (size_t)t->nvp);
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    for (int i = 0; i < 64; ++i) {
+        /* splitmix64 */
+        x += 0x9E3779B97F4A7C15ull;
+        uint64_t z = x;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        t->rnd[i] = z ^ (z >> 31);
+    }
+    return (t->ev && t->deg && t->hv && t->dm && t->slot && t->tmp) ? 0 : -1;
+}
+
+static void mt_free(mt_t *t) {
+    free(t->ev); free(t->deg); free(t->hv); free(t->dm); free(t->slot); free(t->tmp);
+    memset(t, 0, sizeof(*t));
+}
+
+static inline uint64_t mt_pos(uint64_t h) { return h ^ (h >> 29) ^ (h >> 43); }
+
+static int mt_grow(mt_t *t) {
+    if (t->n + 1 >= t->cap) {
+        int32_t nc = t->cap * 2;
+        uint8_t *ev = (uint8_t *)realloc(t->ev, (size_t)nc * (size_t)t->nvp);
+        uint32_t *dg = (uint32_t *)realloc(t->deg, (size_t)nc * sizeof(uint32_t));
+        uint64_t *hv = (uint64_t *)realloc(t->hv, (size_t)nc * sizeof(uint64_t));
+        uint64_t *dm = (uint64_t *)realloc(t->dm, (size_t)nc * sizeof(uint64_t));
+        if (ev) t->ev = ev;
+        if (dg) t->deg = dg;
+        if (hv) t->hv = hv;
+        if (dm) t->dm = dm;
+        if (!ev || !dg || !hv || !dm) return -1;
+        t->cap = nc;
+    }
+    if ((uint64_t)t->n * 2 >= t->smask) {
+        const uint64_t nm = t->smask * 2 + 1;
+        int32_t *sl = (int32_t *)calloc((size_t)nm + 1, sizeof(int32_t));
+        if (!sl) return -1;
+        for (int32_t i = 0; i < t->n; ++i) {
+            uint64_t p = mt_pos(t->hv[i]) & nm;
+            while (sl[p]) p = (p + 1) & nm;
+            sl[p] = i + 1;
+        }
+        free(t->slot);
+        t->slot = sl;
+        t->smask = nm;
+    }
+    return 0;
+}
+
+/* id of the monomial with exponents e (nvp bytes, zero padded) and hash h */
+static int32_t mt_get(mt_t *t, const uint8_t *e, uint64_t h) {
+    uint64_t p = mt_pos(h) & t->smask;
+    for (;;) {
+        const int32_t s = t->slot[p];
+        if (!s) break;
+        const int32_t id = s - 1;
+        if (t->hv[id] == h && memcmp(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp) == 0)
+            return id;
+        p = (p + 1) & t->smask;
+    }
+    if (mt_grow(t)) return -1;
+    if ((uint64_t)t->n * 2 >= t->smask) return -1;
+    p = mt_pos(h) & t->smask;
+    while (t->slot[p]) p = (p + 1) & t->smask;
+    const int32_t id = t->n++;
+    memcpy(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp);
+    uint32_t d = 0;
+    uint64_t m = 0;
+    for (int i = 0; i < t->nv; ++i) {
+        d += e[i];
+        if (e[i]) m |= 1ull << i;
+    }
+    t->deg[id] = d;
+    t->hv[id] = h;
+    t->dm[id] = m;
+    t->slot[p] = id + 1;
+    return id;
+}
+
+static uint64_t mt_hash(const mt_t *t, const uint8_t *e) {
+    uint64_t h = 0;
+    for (int i = 0; i < t->nv; ++i) h += t->rnd[i] * e[i];
+    return h;
+}
+
+static inline const uint8_t *EV(const mt_t *t, int32_t a) { return t->ev + (size_t)a * t->nvp; }
+
+static int32_t mt_mul(mt_t *t, int32_t a, int32_t b) {
+    const uint8_t *ea = EV(t, a), *eb = EV(t, b);
+    for (int i = 0; i < t->nvp; ++i) t->tmp[i] = (uint8_t)(ea[i] + eb[i]);
+    return mt_get(t, t->tmp, t->hv[a] + t->hv[b]);
+}
+
+/* b / a (a divides b) */
+static int32_t mt_div(mt_t *t, int32_t b, int32_t a) {
+    const uint8_t *ea = EV(t, a), *eb = EV(t, b);
+    for (int i = 0; i < t->nvp; ++i) t->tmp[i] = (uint8_t)(eb[i] - ea[i]);
+    return mt_get(t, t->tmp, t->hv[b] - t->hv[a]);
+}
+
+static int32_t mt_lcm(mt_t *t, int32_t a, int32_t b) {
+    const uint8_t *ea = EV(t, a), *eb = EV(t, b);
+    for (int i = 0; i < t->nvp; ++i) t->tmp[i] = ea[i] > eb[i] ? ea[i] : eb[i];
+    return mt_get(t, t->tmp, mt_hash(t, t->tmp));
+}
+
+static inline int mt_divides(const mt_t *t, int32_t a, int32_t b) {
+    if (t->dm[a] & ~t->dm[b]) return 0;
+    if (t->deg[a] > t->deg[b]) return 0;
+    const uint8_t *ea = EV(t, a), *eb = EV(t, b);
+    for (int i = 0; i < t->nv; ++i)
+        if (ea[i] > eb[i]) return 0;
+    return 1;
+}
+
+/* > 0 when a > b in the monomial order */
+static inline int mt_cmp(const mt_t *t, int32_t a, int32_t b) {
+    if (a == b) return 0;
+    const uint8_t *ea = EV(t, a), *eb = EV(t, b);
+    if (t->order == 0) {
+        if (t->deg[a] != t->deg[b]) return t->deg[a] > t->deg[b] ? 1 : -1;
+        for (int i = t->nv - 1; i >= 0; --i)
+            if (ea[i] != eb[i]) return ea[i] < eb[i] ? 1 : -1;
+        return 0;
+    }
+    for (int i = 0; i < t->nv; ++i)
+        if (ea[i] != eb[i]) return ea[i] > eb[i] ? 1 : -1;
+    return 0;
+}
+
+/* ---------------------------------------------------------------- basis */
+
+typedef struct {
+    int32_t len;
+    int32_t *m;      /* monomials, decreasing */
+    uint32_t *c;     /* coefficients, c[0] = 1 */
+} poly_t;
+
+typedef struct {
+    poly_t *e;
+    int32_t n, cap;
+    uint8_t *red;    /* redundant */
+    int32_t *lm;     /* non-redundant elements in insertion order */
+    int32_t nlm;
+} basis_t;
+
+typedef struct {
+    int32_t i, j;    /* i < j, basis indices */
+    int32_t lcm;
+    uint32_t deg;
+} pair_t;
+
+typedef struct {
+    pair_t *p;
+    int64_t n, cap;
+} pairs_t;
+
+static int pairs_push(pairs_t *P, pair_t x) {
+    if (P->n == P->cap) {
+        int64_t nc = P->cap ? 2 * P->cap : 1024;
+        pair_t *np = (pair_t *)realloc(P->p, (size_t)nc * sizeof(pair_t));
+        if (!np) return -1;
+        P->p = np;
+        P->cap = nc;
+    }
+    P->p[P->n++] = x;
+    return 0;
+}
+
+static int basis_push(basis_t *B, poly_t f) {
+    if (B->n == B->cap) {
+        int32_t nc = B->cap ? 2 * B->cap : 256;
+        poly_t *ne = (poly_t *)realloc(B->e, (size_t)nc * sizeof(poly_t));
+        uint8_t *nr = (uint8_t *)realloc(B->red, (size_t)nc);
+        int32_t *nl = (int32_t *)realloc(B->lm, (size_t)nc * sizeof(int32_t));
+        if (ne) B->e = ne;
+        if (nr) B->red = nr;
+        if (nl) B->lm = nl;
+        if (!ne || !nr || !nl) return -1;
+        B->cap = nc;
+    }
+    B->e[B->n] = f;
+    B->red[B->n] = 0;
+    B->n++;
+    return 0;
+}
+
+static uint32_t inv_mod(uint32_t a, uint32_t p) {
+    int64_t r0 = p, r1 = a % p, s0 = 0, s1 = 1;
+    while (r1) {
+        const int64_t q = r0 / r1, t = r0 - q * r1;
+        r0 = r1; r1 = t;
+        const int64_t u = s0 - q * s1;
+        s0 = s1; s1 = u;
+    }
+    return (uint32_t)(s0 < 0 ? s0 + p : s0);
+}
+
+/* Gebauer-Moeller installation of element k (its lead not yet in B->lm).
+ * Elements lo .. k-1 came from the same batch: when the lead of one of
+ * them divides k's lead, k is redundant from the start and only the pair
+ * with that element is kept (its S-polynomial reduces k further) -- gb's
+ * rule in insert_and_update_spairs (update.c:96-112). */
+static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) {
+    const int32_t lk = B->e[k].m[0];
+    const int32_t nl = B->nlm;
+    int rc = 0;
+    for (int32_t i = lo; i < k; ++i) {
+        if (B->red[i] || !mt_divides(T, B->e[i].m[0], lk)) continue;
+        pair_t x = {i, k, lk, T->deg[lk]};
+        B->red[k] = 1;
+        return pairs_push(P, x);
+    }
+    /* candidate pairs (i, k) */
+    int32_t *lc = (int32_t *)malloc((size_t)(nl ? nl : 1) * sizeof(int32_t));
+    uint8_t *st = (uint8_t *)malloc((size_t)(nl ? nl : 1));  /* 0 dropped, 1 kept, 2 coprime */
+    if (!lc || !st) { free(lc); free(st); return -1; }
+    for (int32_t q = 0; q < nl; ++q) {
+        const int32_t i = B->lm[q];
+        const int32_t li = B->e[i].m[0];
+        lc[q] = mt_lcm(T, li, lk);
+        if (lc[q] < 0) { rc = -1; goto out; }
+        /* pairs with redundant elements (kept as reducers in lex) are not formed */
+        st[q] = B->red[i] ? 0 : (T->deg[lc[q]] == T->deg[li] + T->deg[lk]) ? 2 : 1;
+    }
+    /* old pairs: chain criterion */
+    {
+        int64_t w = 0;
+        for (int64_t q = 0; q < P->n; ++q) {
+            const pair_t x = P->p[q];
+            int drop = 0;
+            if (mt_divides(T, lk, x.lcm)) {
+                const int32_t a = mt_lcm(T, B->e[x.i].m[0], lk);
+                const int32_t b = mt_lcm(T, B->e[x.j].m[0], lk);
+                if (a < 0 || b < 0) { rc = -1; goto out; }
+                if (a != x.lcm && b != x.lcm) drop = 1;
+            }
+            if (!drop) P->p[w++] = x;
+        }
+        P->n = w;
+    }
+    /* new pairs: a pair is dropped when another remaining pair's lcm
+     * divides its lcm (for equal lcms the later one survives); coprime
+     * pairs take part in this and are dropped afterwards */
+    for (int32_t q = 0; q < nl; ++q) {
+        if (st[q] == 2) continue;
+        for (int32_t r = 0; r < nl; ++r) {
+            if (r == q || st[r] == 0) continue;
+            if (lc[r] == lc[q] ? r > q : mt_divides(T, lc[r], lc[q])) {
+                st[q] = 0;
+                break;
+            }
+        }
+    }
+    for (int32_t q = 0; q < nl; ++q) {
+        if (st[q] != 1) continue;
+        /* a coprime pair with the same lcm removes the whole group */
+        int cop = 0;
+        for (int32_t r = 0; r < nl; ++r)
+            if (st[r] == 2 && lc[r] == lc[q]) { cop = 1; break; }
+        if (cop) continue;
+        const int32_t i = B->lm[q];
+        pair_t x = {i < k ? i : k, i < k ? k : i, lc[q], T->deg[lc[q]]};
+        if (pairs_push(P, x)) { rc = -1; goto out; }
+    }
+    /* redundant elements: dropped from the reducer list for DRL; for lex
+     * they stay reducers, as in gb (update.c:247-257 compacts the list for
+     * mon_order 0 only) */
+    {
+        int32_t w = 0;
+        for (int32_t q = 0; q < nl; ++q) {
+            const int32_t i = B->lm[q];
+            if (!B->red[i] && mt_divides(T, lk, B->e[i].m[0])) B->red[i] = 1;
+            if (!B->red[i] || T->order != 0) B->lm[w++] = i;
+        }
+        B->nlm = w;
+        B->lm[B->nlm++] = k;
+    }
+out:
+    free(lc);
+    free(st);
+    return rc;
+}
+
+/* pairs whose elements went redundant stay: their S-polynomials are still
+ * needed (the chain criterion only removes pairs covered by others) */
+
+/* ------------------------------------------------------ one F4 round */
+
+typedef struct {
+    int32_t e;    /* basis element */
+    int32_t mul;  /* multiplier monomial */
+    int32_t lead; /* lead monomial of the row */
+    int32_t len;
+} row_t;
+
+typedef struct {
+    row_t *r;
+    int64_t n, cap;
+} rows_t;
+
+static int rows_push(rows_t *R, row_t x) {
+    if (R->n == R->cap) {
+        int64_t nc = R->cap ? 2 * R->cap : 1024;
+        row_t *nr = (row_t *)realloc(R->r, (size_t)nc * sizeof(row_t));
+        if (!nr) return -1;
+        R->r = nr;
+        R->cap = nc;
+    }
+    R->r[R->n++] = x;
+    return 0;
+}
+
+/* round-local marks on monomials */
+typedef struct {
+    int32_t *stamp;   /* round of the last mark */
+    int32_t *col;     /* column index in this round; -1 unprocessed */
+    uint8_t *piv;     /* has a row with this lead */
+    int32_t cap;
+} marks_t;
+
+static int marks_fit(marks_t *M, int32_t n) {
+    if (n <= M->cap) return 0;
+    int32_t nc = M->cap ? M->cap : 1024;
+    while (nc < n) nc *= 2;
+    int32_t *s = (int32_t *)realloc(M->stamp, (size_t)nc * sizeof(int32_t));
+    int32_t *c = (int32_t *)realloc(M->col, (size_t)nc * sizeof(int32_t));
+    uint8_t *p = (uint8_t *)realloc(M->piv, (size_t)nc);
+    if (s) M->stamp = s;
+    if (c) M->col = c;
+    if (p) M->piv = p;
+    if (!s || !c || !p) return -1;
+    for (int32_t i = M->cap; i < nc; ++i) M->stamp[i] = -1;
+    M->cap = nc;
+    return 0;
+}
+
+static const mt_t *g_sort_t;
+static int cmp_mon_desc(const void *a, const void *b) {
+    return -mt_cmp(g_sort_t, *(const int32_t *)a, *(const int32_t *)b);
+}
+
+typedef struct {
+    int32_t key;  /* lead column */
+    int32_t len;
+    int64_t idx;
+} rkey_t;
+static int cmp_rkey(const void *a, const void *b) {
+    const rkey_t *x = (const rkey_t *)a, *y = (const rkey_t *)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    if (x->len != y->len) return x->len < y->len ? -1 : 1;
+    return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+typedef struct {
+    mt_t T;
+    basis_t B;
+    pairs_t P;
+    marks_t M;
+    uint32_t p;
+    int32_t round;
+    gbgpu_f4_la_t la;
+    void *user;
+    gbgpu_f4_out_t *out;
+    int info;
+} f4_t;
+
+/* turn rows (element, multiplier) into gb's CSR over the given columns */
+static int build_csr(f4_t *F, rows_t *R, int32_t *cols, int32_t nc, int32_t ncl,
+                     gbgpu_csr_t *m, int64_t **rp_, int32_t **ci_, int32_t **rcf_,
+                     int64_t **cp_, uint32_t **cf_) {
+    mt_t *T = &F->T;
+    int64_t nnz = 0;
+    for (int64_t r = 0; r < R->n; ++r) nnz += R->r[r].len;
+    /* rows by (lead column, length) */
+    rkey_t *key = (rkey_t *)malloc((size_t)(R->n ? R->n : 1) * sizeof(rkey_t));
+    /* coefficient arrays: one per distinct element */
+    int32_t *eid = (int32_t *)malloc((size_t)(F->B.n ? F->B.n : 1) * sizeof(int32_t));
+    int64_t *rp = (int64_t *)malloc((size_t)(R->n + 1) * sizeof(int64_t));
+    int32_t *ci = (int32_t *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int32_t));
+    int32_t *rcf = (int32_t *)malloc((size_t)(R->n ? R->n : 1) * sizeof(int32_t));
+    if (!key || !eid || !rp || !ci || !rcf) goto fail;
+    for (int32_t i = 0; i < F->B.n; ++i) eid[i] = -1;
+    int32_t ncf = 0;
+    int64_t npool = 0;
+    for (int64_t r = 0; r < R->n; ++r) {
+        key[r].key = F->M.col[R->r[r].lead];
+        key[r].len = R->r[r].len;
+        key[r].idx = r;
+        const int32_t e = R->r[r].e;
+        if (eid[e] < 0) {
+            eid[e] = ncf++;
+            npool += F->B.e[e].len;
+        }
+    }
+    qsort(key, (size_t)R->n, sizeof(rkey_t), cmp_rkey);
+    int64_t *cp = (int64_t *)malloc((size_t)(ncf + 1) * sizeof(int64_t));
+    uint32_t *cf = (uint32_t *)malloc((size_t)(npool ? npool : 1) * sizeof(uint32_t));
+    if (!cp || !cf) { free(cp); free(cf); goto fail; }
+    cp[0] = 0;
+    for (int32_t i = 0; i < F->B.n; ++i) {
+        if (eid[i] < 0) continue;
+        const poly_t *f = &F->B.e[i];
+        cp[eid[i] + 1] = f->len;
+    }
+    for (int32_t i = 0; i < ncf; ++i) cp[i + 1] += cp[i];
+    for (int32_t i = 0; i < F->B.n; ++i) {
+        if (eid[i] < 0) continue;
+        const poly_t *f = &F->B.e[i];
+        memcpy(cf + cp[eid[i]], f->c, (size_t)f->len * sizeof(uint32_t));
+    }
+    rp[0] = 0;
+    for (int64_t q = 0; q < R->n; ++q) {
+        const row_t *x = &R->r[key[q].idx];
+        const poly_t *f = &F->B.e[x->e];
+        int32_t *o = ci + rp[q];
+        for (int32_t t = 0; t < f->len; ++t) {
+            const int32_t mon = mt_mul(T, x->mul, f->m[t]);
+            if (mon < 0) { free(cp); free(cf); goto fail; }
+            o[t] = F->M.col[mon];
+        }
+        rp[q + 1] = rp[q] + f->len;
+        rcf[q] = eid[x->e];
+    }
+    free(key);
+    free(eid);
+    m->nr = (int32_t)R->n;
+    m->nc = nc;
+    m->ncl = ncl;
+    m->ncf = ncf;
+    m->p = F->p;
+    m->rp = rp; m->ci = ci; m->rcf = rcf; m->cp = cp; m->cf = cf;
+    *rp_ = rp; *ci_ = ci; *rcf_ = rcf; *cp_ = cp; *cf_ = cf;
+    (void)cols;
+    return 0;
+fail:
+    free(key); free(eid); free(rp); free(ci); free(rcf);
+    return -1;
+}
+
+static int find_reducer(f4_t *F, int32_t mon) {
+    const mt_t *T = &F->T;
+    for (int32_t q = 0; q < F->B.nlm; ++q) {
+        const int32_t i = F->B.lm[q];
+        if (mt_divides(T, F->B.e[i].m[0], mon)) return i;
+    }
+    return -1;
+}
+
+/* add row (e, mul) if new; marks its monomials as columns, queueing the
+ * ones not seen before */
+static int add_row(f4_t *F, rows_t *R, int32_t e, int32_t mul, int32_t **queue, int64_t *nq,
+                   int64_t *capq) {
+    mt_t *T = &F->T;
+    const poly_t *f = &F->B.e[e];
+    for (int32_t t = 0; t < f->len; ++t) {
+        const int32_t mon = mt_mul(T, mul, f->m[t]);
+        if (mon < 0) return -1;
+        if (marks_fit(&F->M, T->n)) return -1;
+        if (F->M.stamp[mon] != F->round) {
+            F->M.stamp[mon] = F->round;
+            F->M.col[mon] = -1;
+            F->M.piv[mon] = 0;
+            if (*nq == *capq) {
+                int64_t nc = *capq ? 2 * *capq : 4096;
+                int32_t *nqp = (int32_t *)realloc(*queue, (size_t)nc * sizeof(int32_t));
+                if (!nqp) return -1;
+                *queue = nqp;
+                *capq = nc;
+            }
+            (*queue)[(*nq)++] = mon;
+        }
+        if (t == 0) {
+            F->M.piv[mon] = 1;
+            row_t x = {e, mul, mon, f->len};
+            if (rows_push(R, x)) return -1;
+        }
+    }
+    return 0;
+}
+
+typedef struct {
+    int32_t e, mul;
+} em_t;
+static int cmp_em(const void *a, const void *b) {
+    const em_t *x = (const em_t *)a, *y = (const em_t *)b;
+    if (x->e != y->e) return x->e < y->e ? -1 : 1;
+    return x->mul < y->mul ? -1 : (x->mul > y->mul);
+}
+
+static int cmp_pair_deg(const void *a, const void *b) {
+    const pair_t *x = (const pair_t *)a, *y = (const pair_t *)b;
+    if (x->deg != y->deg) return x->deg < y->deg ? -1 : 1;
+    return 0;
+}
+
+static int run_la(f4_t *F, const gbgpu_csr_t *m, int interreduce, gbgpu_rows_t *out) {
+    const double t0 = now_ms();
+    const int rc = F->la(F->user, m, interreduce, out);
+    F->out->la_ms += now_ms() - t0;
+    return rc;
+}
+
+/* new basis element from row k of the LA output (columns -> monomials) */
+static int take_row(f4_t *F, const gbgpu_rows_t *o, int32_t k, const int32_t *cols, poly_t *f) {
+    const int64_t b = o->rp[k], e = o->rp[k + 1];
+    f->len = (int32_t)(e - b);
+    f->m = (int32_t *)malloc((size_t)f->len * sizeof(int32_t));
+    f->c = (uint32_t *)malloc((size_t)f->len * sizeof(uint32_t));
+    if (!f->m || !f->c) return -1;
+    for (int64_t q = b; q < e; ++q) {
+        f->m[q - b] = cols[o->ci[q]];
+        f->c[q - b] = o->cf[q];
+    }
+    (void)F;
+    return 0;
+}
+
+static int f4_round(f4_t *F) {
+    mt_t *T = &F->T;
+    basis_t *B = &F->B;
+    pairs_t *P = &F->P;
+    int rc = GBGPU_ENOMEM;
+    ++F->round;
+    /* selection: all pairs of minimal degree */
+    qsort(P->p, (size_t)P->n, sizeof(pair_t), cmp_pair_deg);
+    const uint32_t dmin = P->p[0].deg;
+    int64_t nsel = 0;
+    while (nsel < P->n && P->p[nsel].deg == dmin) ++nsel;
+    /* generators of the selected pairs, one row per distinct (e, mul) */
+    em_t *gen = (em_t *)malloc((size_t)(2 * nsel) * sizeof(em_t));
+    rows_t R = {0};
+    int32_t *queue = NULL;
+    int64_t nq = 0, capq = 0;
+    int32_t *cols = NULL;
+    gbgpu_rows_t out = {0};
+    int64_t *rp = NULL, *cp = NULL;
+    int32_t *ci = NULL, *rcf = NULL;
+    uint32_t *cf = NULL;
+    if (!gen) goto done;
+    for (int64_t q = 0; q < nsel; ++q) {
+        const pair_t x = P->p[q];
+        gen[2 * q].e = x.i;
+        gen[2 * q].mul = mt_div(T, x.lcm, B->e[x.i].m[0]);
+        gen[2 * q + 1].e = x.j;
+        gen[2 * q + 1].mul = mt_div(T, x.lcm, B->e[x.j].m[0]);
+        if (gen[2 * q].mul < 0 || gen[2 * q + 1].mul < 0) goto done;
+    }
+    memmove(P->p, P->p + nsel, (size_t)(P->n - nsel) * sizeof(pair_t));
+    P->n -= nsel;
+    qsort(gen, (size_t)(2 * nsel), sizeof(em_t), cmp_em);
+    for (int64_t q = 0; q < 2 * nsel; ++q) {
+        if (q > 0 && gen[q].e == gen[q - 1].e && gen[q].mul == gen[q - 1].mul) continue;
+        if (add_row(F, &R, gen[q].e, gen[q].mul, &queue, &nq, &capq)) goto done;
+    }
+    const int64_t nspair_rows = R.n;
+    /* symbolic preprocessing: a reducer for every column monomial that is
+     * no lead yet and is divisible by a basis lead */
+    for (int64_t q = 0; q < nq; ++q) {
+        const int32_t mon = queue[q];
+        if (F->M.piv[mon]) continue;
+        const int32_t e = find_reducer(F, mon);
+        if (e < 0) continue;
+        const int32_t mul = mt_div(T, mon, B->e[e].m[0]);
+        if (mul < 0) goto done;
+        if (add_row(F, &R, e, mul, &queue, &nq, &capq)) goto done;
+    }
+    /* columns: pivot columns, then the rest, each decreasing */
+    cols = (int32_t *)malloc((size_t)(nq ? nq : 1) * sizeof(int32_t));
+    if (!cols) goto done;
+    int32_t ncl = 0, nc = (int32_t)nq;
+    for (int64_t q = 0; q < nq; ++q)
+        if (F->M.piv[queue[q]]) cols[ncl++] = queue[q];
+    {
+        int32_t w = ncl;
+        for (int64_t q = 0; q < nq; ++q)
+            if (!F->M.piv[queue[q]]) cols[w++] = queue[q];
+    }
+    g_sort_t = T;
+    qsort(cols, (size_t)ncl, sizeof(int32_t), cmp_mon_desc);
+    qsort(cols + ncl, (size_t)(nc - ncl), sizeof(int32_t), cmp_mon_desc);
+    for (int32_t c = 0; c < nc; ++c) F->M.col[cols[c]] = c;
+    gbgpu_csr_t m;
+    if (build_csr(F, &R, cols, nc, ncl, &m, &rp, &ci, &rcf, &cp, &cf)) goto done;
+    int64_t nnz = rp[m.nr];
+    if (m.nr > F->out->max_rows) F->out->max_rows = m.nr;
+    if (nc > F->out->max_cols) F->out->max_cols = nc;
+    if (nnz > F->out->max_nnz) F->out->max_nnz = nnz;
+    F->out->rows_total += m.nr - ncl;
+    const double t0 = now_ms();
+    if ((rc = run_la(F, &m, 0, &out)) != 0) goto done;
+    const double tla = now_ms() - t0;
+    /* new elements in the order the LA returns them (decreasing lead) */
+    rc = GBGPU_ENOMEM;
+    const int32_t lo = B->n;
+    for (int32_t k = 0; k < out.np; ++k) {
+        poly_t f;
+        if (take_row(F, &out, k, cols, &f)) goto done;
+        if (basis_push(B, f)) goto done;
+        if (update_pairs(T, B, P, B->n - 1, lo)) goto done;
+    }
+    if (F->info > 1)
+        printf("%3u  %6lld sel  %8lld pairs  %7d x %-7d (%lld spair rows)  %6.2f%%  %5d new  %8.1f ms LA\n",
+               dmin, (long long)nsel, (long long)P->n, m.nr, nc, (long long)nspair_rows,
+               nc && m.nr ? 100.0 * (double)nnz / ((double)m.nr * nc) : 0.0, out.np, tla);
+    if (F->info > 1) fflush(stdout);
+    rc = GBGPU_OK;
+done:
+    gbgpu_rows_free(&out);
+    free(gen); free(R.r); free(queue); free(cols);
+    free(rp); free(ci); free(rcf); free(cp); free(cf);
+    return rc;
+}
+
+/* interreduction of the non-redundant elements (reduce_basis, f4.c:90-179) */
+static int f4_reduce(f4_t *F) {
+    mt_t *T = &F->T;
+    basis_t *B = &F->B;
+    int rc = GBGPU_ENOMEM;
+    ++F->round;
+    rows_t R = {0};
+    int32_t *queue = NULL, *cols = NULL;
+    int64_t nq = 0, capq = 0;
+    int64_t *rp = NULL, *cp = NULL;
+    int32_t *ci = NULL, *rcf = NULL;
+    uint32_t *cf = NULL;
+    gbgpu_rows_t out = {0};
+    int32_t one;
+    {
+        uint8_t *z = (uint8_t *)calloc(1, (size_t)T->nvp);
+        if (!z) goto done;
+        one = mt_get(T, z, 0);
+        free(z);
+        if (one < 0) goto done;
+    }
+    for (int32_t q = 0; q < B->nlm; ++q)
+        if (add_row(F, &R, B->lm[q], one, &queue, &nq, &capq)) goto done;
+    /* reducers for the tail monomials divisible by a lead, so that the
+     * reduced rows have no such monomial left */
+    for (int64_t q = 0; q < nq; ++q) {
+        const int32_t mon = queue[q];
+        if (F->M.piv[mon]) continue;
+        const int32_t e = find_reducer(F, mon);
+        if (e < 0) continue;
+        const int32_t mul = mt_div(T, mon, B->e[e].m[0]);
+        if (mul < 0) goto done;
+        if (add_row(F, &R, e, mul, &queue, &nq, &capq)) goto done;
+    }
+    cols = (int32_t *)malloc((size_t)(nq ? nq : 1) * sizeof(int32_t));
+    if (!cols) goto done;
+    memcpy(cols, queue, (size_t)nq * sizeof(int32_t));
+    g_sort_t = T;
+    qsort(cols, (size_t)nq, sizeof(int32_t), cmp_mon_desc);
+    for (int64_t c = 0; c < nq; ++c) F->M.col[cols[c]] = (int32_t)c;
+    gbgpu_csr_t m;
+    if (build_csr(F, &R, cols, (int32_t)nq, 0, &m, &rp, &ci, &rcf, &cp, &cf)) goto done;
+    if ((rc = run_la(F, &m, 1, &out)) != 0) goto done;
+    rc = GBGPU_ENOMEM;
+    /* the reduced basis replaces the old one: the rows whose lead no
+     * other row's lead divides (the others are the reducers' rows), by
+     * decreasing lead as gb lists them (f4.c:151-162) */
+    B->nlm = 0;
+    for (int32_t k = out.np - 1; k >= 0; --k) {
+        const int32_t lead = cols[out.ci[out.rp[k]]];
+        int mult = 0;
+        for (int32_t j = 0; j < out.np && !mult; ++j)
+            if (j != k && mt_divides(T, cols[out.ci[out.rp[j]]], lead)) mult = 1;
+        if (mult) continue;
+        poly_t f;
+        if (take_row(F, &out, k, cols, &f)) goto done;
+        if (basis_push(B, f)) goto done;
+        B->lm[B->nlm++] = B->n - 1;
+    }
+    rc = GBGPU_OK;
+done:
+    gbgpu_rows_free(&out);
+    free(R.r); free(queue); free(cols);
+    free(rp); free(ci); free(rcf); free(cp); free(cf);
+    return rc;
+}
+
+static int cmp_i32(const void *a, const void *b) {
+    const int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+    return x < y ? -1 : (x > y);
+}
+
+int gbgpu_f4(const gbgpu_f4_in_t *in, gbgpu_f4_la_t la, void *user, gbgpu_f4_out_t *out) {
+    if (!in || !la || !out) return GBGPU_EINVAL;
+    memset(out, 0, sizeof(*out));
+    if (in->nr_vars < 1 || in->nr_vars > 64 || in->nr_gens < 0 || in->field_char < 2 ||
+        (in->mon_order != 0 && in->mon_order != 1) || !in->lens || !in->exps || !in->cfs)
+        return GBGPU_EINVAL;
+    const double t_start = now_ms();
+    f4_t F;
+    memset(&F, 0, sizeof(F));
+    F.p = (uint32_t)in->field_char;
+    F.la = la;
+    F.user = user;
+    F.out = out;
+    F.info = in->info_level;
+    int rc = GBGPU_ENOMEM;
+    if (mt_init(&F.T, in->nr_vars, in->mon_order)) goto done;
+    /* input: terms sorted decreasing, equal monomials merged, zero terms
+     * dropped, made monic; zero generators dropped */
+    {
+        int64_t off = 0;
+        const int nv = in->nr_vars;
+        for (int32_t g = 0; g < in->nr_gens; ++g) {
+            const int32_t len = in->lens[g];
+            if (len < 0) { rc = GBGPU_EINVAL; goto done; }
+            int32_t *mm = (int32_t *)malloc((size_t)(len ? len : 1) * sizeof(int32_t));
+            uint32_t *cc = (uint32_t *)malloc((size_t)(len ? len : 1) * sizeof(uint32_t));
+            int32_t *ord = (int32_t *)malloc((size_t)(len ? len : 1) * sizeof(int32_t));
+            if (!mm || !cc || !ord) { free(mm); free(cc); free(ord); goto done; }
+            for (int32_t t = 0; t < len; ++t) {
+                uint8_t *e = F.T.tmp;
+                memset(e, 0, (size_t)F.T.nvp);
+                for (int i = 0; i < nv; ++i) {
+                    const int32_t x = in->exps[(off + t) * nv + i];
+                    if (x < 0 || x > 255) { free(mm); free(cc); free(ord); rc = GBGPU_ERANGE; goto done; }
+                    e[i] = (uint8_t)x;
+                }
+                uint8_t ebuf[64];
+                memcpy(ebuf, e, (size_t)F.T.nvp);
+                mm[t] = mt_get(&F.T, ebuf, mt_hash(&F.T, ebuf));
+                if (mm[t] < 0) { free(mm); free(cc); free(ord); goto done; }
+                int64_t c = (int64_t)in->cfs[off + t] % (int64_t)F.p;
+                if (c < 0) c += F.p;
+                cc[t] = (uint32_t)c;
+                ord[t] = t;
+            }
+            off += len;
+            /* insertion sort by monomial, decreasing */
+            for (int32_t a = 1; a < len; ++a) {
+                const int32_t x = ord[a];
+                int32_t b = a - 1;
+                while (b >= 0 && mt_cmp(&F.T, mm[ord[b]], mm[x]) < 0) {
+                    ord[b + 1] = ord[b];
+                    --b;
+                }
+                ord[b + 1] = x;
+            }
+            poly_t f = {0, NULL, NULL};
+            f.m = (int32_t *)malloc((size_t)(len ? len : 1) * sizeof(int32_t));
+            f.c = (uint32_t *)malloc((size_t)(len ? len : 1) * sizeof(uint32_t));
+            if (!f.m || !f.c) { free(f.m); free(f.c); free(mm); free(cc); free(ord); goto done; }
+            for (int32_t a = 0; a < len; ++a) {
+                const int32_t t = ord[a];
+                if (f.len > 0 && f.m[f.len - 1] == mm[t]) {
+                    f.c[f.len - 1] = (uint32_t)(((uint64_t)f.c[f.len - 1] + cc[t]) % F.p);
+                } else {
+                    f.m[f.len] = mm[t];
+                    f.c[f.len] = cc[t];
+                    f.len++;
+                }
+            }
+            free(mm); free(cc); free(ord);
+            int32_t w = 0;
+            for (int32_t a = 0; a < f.len; ++a)
+                if (f.c[a]) { f.m[w] = f.m[a]; f.c[w] = f.c[a]; ++w; }
+            f.len = w;
+            if (f.len == 0) { free(f.m); free(f.c); continue; }
+            const uint32_t inv = inv_mod(f.c[0], F.p);
+            for (int32_t a = 0; a < f.len; ++a) f.c[a] = (uint32_t)((uint64_t)f.c[a] * inv % F.p);
+            if (basis_push(&F.B, f)) { free(f.m); free(f.c); goto done; }
+        }
+    }
+    /* smallest lead first, then installed one by one (f4.c:258-275) */
+    {
+        const int32_t n0 = F.B.n;
+        int32_t *ord = (int32_t *)malloc((size_t)(n0 ? n0 : 1) * sizeof(int32_t));
+        poly_t *tmp = (poly_t *)malloc((size_t)(n0 ? n0 : 1) * sizeof(poly_t));
+        if (!ord || !tmp) { free(ord); free(tmp); goto done; }
+        for (int32_t i = 0; i < n0; ++i) ord[i] = i;
+        for (int32_t a = 1; a < n0; ++a) {
+            const int32_t x = ord[a];
+            int32_t b = a - 1;
+            while (b >= 0 && mt_cmp(&F.T, F.B.e[ord[b]].m[0], F.B.e[x].m[0]) > 0) {
+                ord[b + 1] = ord[b];
+                --b;
+            }
+            ord[b + 1] = x;
+        }
+        for (int32_t i = 0; i < n0; ++i) tmp[i] = F.B.e[ord[i]];
+        memcpy(F.B.e, tmp, (size_t)n0 * sizeof(poly_t));
+        free(ord);
+        free(tmp);
+        for (int32_t k = 0; k < n0; ++k)
+            if (update_pairs(&F.T, &F.B, &F.P, k, 0)) goto done;
+    }
+    while (F.P.n > 0) {
+        if ((rc = f4_round(&F)) != GBGPU_OK) goto done;
+        out->rounds++;
+    }
+    /* drop the redundant elements (f4.c:330-338) */
+    {
+        int32_t w = 0;
+        for (int32_t q = 0; q < F.B.nlm; ++q)
+            if (!F.B.red[F.B.lm[q]]) F.B.lm[w++] = F.B.lm[q];
+        F.B.nlm = w;
+    }
+    if (in->reduce_gb == 1 && F.B.nlm > 0) {
+        if ((rc = f4_reduce(&F)) != GBGPU_OK) goto done;
+    }
+    /* export the non-redundant elements (in installation order; the
+     * reduced basis by increasing lead) */
+    {
+        rc = GBGPU_ENOMEM;
+        int32_t *lm = (int32_t *)malloc((size_t)(F.B.nlm ? F.B.nlm : 1) * sizeof(int32_t));
+        if (!lm) goto done;
+        memcpy(lm, F.B.lm, (size_t)F.B.nlm * sizeof(int32_t));
+        if (in->reduce_gb != 1) qsort(lm, (size_t)F.B.nlm, sizeof(int32_t), cmp_i32);
+        int64_t nterms = 0;
+        for (int32_t q = 0; q < F.B.nlm; ++q) nterms += F.B.e[lm[q]].len;
+        out->bld = F.B.nlm;
+        out->nterms = nterms;
+        const size_t nl = F.B.nlm > 0 ? (size_t)F.B.nlm : 1;
+        out->blen = (int32_t *)malloc(nl * sizeof(int32_t));
+        out->bexp = (int32_t *)malloc((size_t)(nterms ? nterms : 1) * (size_t)in->nr_vars * sizeof(int32_t));
+        out->bcf = (uint32_t *)malloc((size_t)(nterms ? nterms : 1) * sizeof(uint32_t));
+        if (!out->blen || !out->bexp || !out->bcf) { free(lm); gbgpu_f4_out_free(out); goto done; }
+        int64_t t = 0;
+        for (int32_t q = 0; q < F.B.nlm; ++q) {
+            const poly_t *f = &F.B.e[lm[q]];
+            out->blen[q] = f->len;
+            for (int32_t a = 0; a < f->len; ++a, ++t) {
+                out->bcf[t] = f->c[a];
+                const uint8_t *e = EV(&F.T, f->m[a]);
+                for (int i = 0; i < in->nr_vars; ++i) out->bexp[t * in->nr_vars + i] = e[i];
+            }
+        }
+        free(lm);
+    }
+    rc = GBGPU_OK;
+done:
+    for (int32_t i = 0; i < F.B.n; ++i) {
+        free(F.B.e[i].m);
+        free(F.B.e[i].c);
+    }
+    free(F.B.e); free(F.B.red); free(F.B.lm);
+    free(F.P.p);
+    free(F.M.stamp); free(F.M.col); free(F.M.piv);
+    mt_free(&F.T);
+    out->total_ms = now_ms() - t_start;
+    return rc;
+}
+
+void gbgpu_f4_out_free(gbgpu_f4_out_t *out) {
+    if (!out) return;
+    free(out->blen);
+    free(out->bexp);
+    free(out->bcf);
+    out->blen = NULL;
+    out->bexp = NULL;
+    out->bcf = NULL;
+    out->bld = 0;
+    out->nterms = 0;
+}
+
+/* ------------------------------------------------------- GPU back end */
+
+int gbgpu_f4_gpu_la(void *user, const gbgpu_csr_t *m, int interreduce, gbgpu_rows_t *out) {
+    gbgpu_f4_gpu_t *g = (gbgpu_f4_gpu_t *)user;
+    if (!g || !g->ctx) return GBGPU_EINVAL;
+    if (!interreduce) {
+        const int rc = gbgpu_la_ff(g->ctx, m, 1, g->form, out);
+        if (rc == GBGPU_OK) g->device_ms += gbgpu_phase_ms(g->ctx, 2);
+        return rc;
+    }
+    gbgpu_mat *dm = NULL;
+    int32_t np = 0;
+    int rc = gbgpu_mat_upload(g->ctx, m, &dm);
+    if (rc == GBGPU_OK) rc = gbgpu_interreduce(g->ctx, dm, g->form, &np);
+    if (rc == GBGPU_OK) rc = gbgpu_rows_download(g->ctx, dm, out);
+    gbgpu_mat_free(dm);
+    return rc;
+}
+
+int64_t gbgpu_f4_julia(int32_t *bld, int32_t **blen, int32_t **bexp, void **bcf,
+                       const int32_t *lens, const int32_t *exps, const void *cfs,
+                       const int32_t field_char, const int32_t mon_order,
+                       const int32_t nr_vars, const int32_t nr_gens, const int32_t ht_size,
+                       const int32_t nr_threads, const int32_t max_nr_pairs,
+                       const int32_t reset_ht, const int32_t la_option,
+                       const int32_t reduce_gb, const int32_t pbm_file,
+                       const int32_t info_level) {
+    (void)ht_size; (void)nr_threads; (void)max_nr_pairs; (void)reset_ht; (void)pbm_file;
+    if (!bld || !blen || !bexp || !bcf) return 0;
+    const char *dev = getenv("GBGPU_DEVICE");
+    gbgpu_ctx *ctx = NULL;
+    if (gbgpu_init(dev ? atoi(dev) : 0, &ctx) != GBGPU_OK) {
+        fprintf(stderr, "gbgpu_f4_julia: no usable GPU\n");
+        return 0;
+    }
+    gbgpu_f4_gpu_t g = {ctx, la_option == 101 ? GBGPU_FORM_PLAIN : GBGPU_FORM_MFMA, 0.0};
+    gbgpu_f4_in_t in = {nr_vars, nr_gens, lens, exps, (const int32_t *)cfs, field_char,
+                        mon_order, 1, reduce_gb, info_level};
+    gbgpu_f4_out_t out;
+    const int rc = gbgpu_f4(&in, gbgpu_f4_gpu_la, &g, &out);
+    if (rc != GBGPU_OK) {
+        fprintf(stderr, "gbgpu_f4_julia: failed (%d): %s\n", rc, gbgpu_last_error(ctx));
+        gbgpu_finalize(ctx);
+        return 0;
+    }
+    gbgpu_finalize(ctx);
+    *bld = out.bld;
+    *blen = out.blen;
+    *bexp = out.bexp;
+    *bcf = (void *)out.bcf;
+    return out.nterms;
+}

@@ -6,6 +6,7 @@ loading fails loudly when the library or the GPU is missing.
 from __future__ import annotations
 
 import ctypes as C
+from dataclasses import dataclass
 import os
 import re
 
@@ -16,7 +17,8 @@ from .matrix import F4Matrix, SparseRows
 HERE = os.path.dirname(os.path.abspath(__file__))
 # GBGPU_LIB may point at another build of the same C-ABI
 LIB_PATH = os.environ.get("GBGPU_LIB", os.path.join(HERE, "libgbgpu.so"))
-HEADERS = [os.path.join(os.path.dirname(HERE), "include", h) for h in ("gbgpu.h", "gbgpu_gb.h")]
+HEADERS = [os.path.join(os.path.dirname(HERE), "include", h)
+           for h in ("gbgpu.h", "gbgpu_gb.h", "gbgpu_f4.h")]
 
 FORM_PLAIN = 0
 FORM_MFMA = 1
@@ -37,6 +39,87 @@ class _Csr(C.Structure):
 class _Rows(C.Structure):
     _fields_ = [("np", C.c_int32), ("rp", C.POINTER(C.c_int64)),
                 ("ci", C.POINTER(C.c_int32)), ("cf", C.POINTER(C.c_uint32))]
+
+
+class _F4In(C.Structure):
+    _fields_ = [("nr_vars", C.c_int32), ("nr_gens", C.c_int32),
+                ("lens", C.POINTER(C.c_int32)), ("exps", C.POINTER(C.c_int32)),
+                ("cfs", C.POINTER(C.c_int32)), ("field_char", C.c_int32),
+                ("mon_order", C.c_int32), ("la_option", C.c_int32), ("reduce_gb", C.c_int32),
+                ("info_level", C.c_int32)]
+
+
+class _F4Out(C.Structure):
+    _fields_ = [("bld", C.c_int32), ("blen", C.POINTER(C.c_int32)),
+                ("bexp", C.POINTER(C.c_int32)), ("bcf", C.POINTER(C.c_uint32)),
+                ("nterms", C.c_int64), ("rounds", C.c_int32), ("max_rows", C.c_int64),
+                ("max_cols", C.c_int64), ("max_nnz", C.c_int64), ("rows_total", C.c_int64),
+                ("la_ms", C.c_double), ("total_ms", C.c_double)]
+
+
+class _F4Gpu(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("form", C.c_int32), ("device_ms", C.c_double)]
+
+
+@dataclass
+class PolySystem:
+    """A polynomial system in f4_julia's arrays: lens[i] terms per
+    generator, exps nv exponents per term, cfs one coefficient per term."""
+    nv: int
+    p: int
+    lens: np.ndarray
+    exps: np.ndarray
+    cfs: np.ndarray
+    order: int = 0  # 0 DRL, 1 LEX
+
+
+@dataclass
+class Basis:
+    """A basis returned by the F4 host, with the run's statistics."""
+    nv: int
+    lens: np.ndarray
+    exps: np.ndarray  # nterms x nv
+    cfs: np.ndarray
+    stats: dict
+
+    def polys(self):
+        """Elements as lists of (exponent tuple, coefficient), decreasing."""
+        out, t = [], 0
+        for L in self.lens:
+            out.append([(tuple(int(x) for x in self.exps[t + k]), int(self.cfs[t + k]))
+                        for k in range(int(L))])
+            t += int(L)
+        return out
+
+    def leads(self):
+        return sorted(p[0][0] for p in self.polys())
+
+
+def f4(system: PolySystem, la_fn: int, la_user, reduce_gb: int = 0, info_level: int = 0,
+       keep=None) -> Basis:
+    """Runs the F4 host with the linear-algebra callback at address la_fn
+    (a gbgpu_f4_la_t) and its user pointer."""
+    L = lib()
+    lens = np.ascontiguousarray(system.lens, np.int32)
+    exps = np.ascontiguousarray(system.exps, np.int32).reshape(-1)
+    cfs = np.ascontiguousarray(system.cfs, np.int32)
+    pi = C.POINTER(C.c_int32)
+    inp = _F4In(system.nv, len(lens), lens.ctypes.data_as(pi), exps.ctypes.data_as(pi),
+                cfs.ctypes.data_as(pi), system.p, system.order, 1, reduce_gb, info_level)
+    out = _F4Out()
+    rc = L.gbgpu_f4(C.byref(inp), C.c_void_p(la_fn), la_user, C.byref(out))
+    if rc != 0:
+        raise GpuError(f"gbgpu_f4 failed ({rc})")
+    nt = int(out.nterms)
+    b = Basis(nv=system.nv,
+              lens=np.ctypeslib.as_array(out.blen, shape=(out.bld,)).copy() if out.bld else np.zeros(0, np.int32),
+              exps=np.ctypeslib.as_array(out.bexp, shape=(nt * system.nv,)).copy().reshape(nt, system.nv) if nt else np.zeros((0, system.nv), np.int32),
+              cfs=np.ctypeslib.as_array(out.bcf, shape=(nt,)).copy() if nt else np.zeros(0, np.uint32),
+              stats={"rounds": out.rounds, "max_rows": out.max_rows, "max_cols": out.max_cols,
+                     "max_nnz": out.max_nnz, "rows_total": out.rows_total, "la_ms": out.la_ms,
+                     "total_ms": out.total_ms})
+    L.gbgpu_f4_out_free(C.byref(out))
+    return b
 
 
 def declared_symbols() -> list[str]:
@@ -85,6 +168,8 @@ def lib():
         "gbgpu_batch_rows_download": ([vp, vp, i32, C.POINTER(_Rows)], C.c_int),
         "gbgpu_batch_info": ([vp, pi32, i32], C.c_int),
         "gbgpu_batch_free": ([vp], None),
+        "gbgpu_f4": ([C.POINTER(_F4In), vp, vp, C.POINTER(_F4Out)], C.c_int),
+        "gbgpu_f4_out_free": ([C.POINTER(_F4Out)], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -180,6 +265,16 @@ class Context:
         self._check(lib().gbgpu_batch_upload(self.h, arr, len(structs), C.byref(h)),
                     "batch upload")
         return DeviceBatch(self, h, len(mats))
+
+    def f4(self, system: PolySystem, form: int = FORM_MFMA, reduce_gb: int = 0,
+           info_level: int = 0) -> Basis:
+        """F4 on this device: every round's linear algebra on the GPU."""
+        L = lib()
+        g = _F4Gpu(self.h, form, 0.0)
+        fn = C.cast(L.gbgpu_f4_gpu_la, C.c_void_p).value
+        b = f4(system, fn, C.cast(C.pointer(g), C.c_void_p), reduce_gb, info_level)
+        b.stats["device_ms"] = g.device_ms
+        return b
 
     def dense(self, a: np.ndarray, p: int) -> "DeviceDense":
         a = np.ascontiguousarray(a, dtype=np.uint32)

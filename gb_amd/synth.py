@@ -145,3 +145,41 @@ def batch_31bit(count: int, seed: int, rows=(500, 20000), cols_factor: float = 1
                              piv_mu=float(np.log(plen)), piv_sigma=0.5,
                              red_density=dens))
     return out
+
+
+def splitmix64(seed: int):
+    """The splitmix64 generator (Steele, Lea, Flood 2014) as an iterator."""
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    while True:
+        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        yield z ^ (z >> 31)
+
+
+def dense_quadratic_system(n: int, p: int, seed: int, order: int = 0):
+    """n quadratics in n variables with every monomial of degree <= 2
+    present, coefficients uniform in [1, p - 1] from splitmix64(seed)."""
+    from .native import PolySystem
+    mons = [(i, j) for i in range(n) for j in range(i, n)]  # x_i x_j
+    terms = []
+    for i, j in mons:
+        e = [0] * n
+        e[i] += 1
+        e[j] += 1
+        terms.append(e)
+    for i in range(n):
+        e = [0] * n
+        e[i] = 1
+        terms.append(e)
+    terms.append([0] * n)
+    g = splitmix64(seed)
+    lens, exps, cfs = [], [], []
+    for _ in range(n):
+        lens.append(len(terms))
+        for e in terms:
+            exps.extend(e)
+            cfs.append(1 + next(g) % (p - 1))
+    return PolySystem(nv=n, p=p, lens=np.array(lens, np.int32), exps=np.array(exps, np.int32),
+                      cfs=np.array(cfs, np.int64).astype(np.int32), order=order)

@@ -156,3 +156,13 @@ def interreduce(m: F4Matrix) -> SparseRows:
     if lib().orc_interreduce(C.byref(s), m.p, C.byref(r)) < 0:
         raise ValueError("row lead outside the matrix")
     return _take_rows(r)
+
+
+def f4(system, nthreads: int = 1, reduce_gb: int = 0, info_level: int = 0):
+    """The F4 host (libgbgpu's gbgpu_f4) with the oracle's linear algebra
+    in every round (orc_f4_la): an end-to-end run on the CPU."""
+    from gb_amd import native
+    L = lib()
+    fn = C.cast(L.orc_f4_la, C.c_void_p).value
+    nth = C.c_int(nthreads)
+    return native.f4(system, fn, C.cast(C.pointer(nth), C.c_void_p), reduce_gb, info_level)
