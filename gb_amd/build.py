@@ -27,8 +27,8 @@ def _compile(src: str) -> str:
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
     path = os.path.join(CSRC, src)
     if src.endswith(".c"):
-        cmd = ["gcc", "-O2", "-fPIC", "-std=gnu11", "-Wall", "-I", os.path.join(ROOT, "include"),
-               "-c", path, "-o", obj]
+        cmd = ["gcc", "-O2", "-fPIC", "-std=gnu11", "-Wall", "-fopenmp", "-I",
+               os.path.join(ROOT, "include"), "-c", path, "-o", obj]
     else:
         cmd = [HIPCC] + FLAGS + ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -44,7 +44,10 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    # the C host code uses OpenMP through gcc's runtime
+    gomp = subprocess.run(["gcc", "-print-file-name=libgomp.so"], capture_output=True,
+                          text=True).stdout.strip()
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + [gomp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -146,6 +146,20 @@ static int32_t mt_get(mt_t *t, const uint8_t *e, uint64_t h) {
     return id;
 }
 
+/* id of an existing monomial, -1 when absent (no insertion: safe to call
+ * from many threads while nothing is inserted) */
+static int32_t mt_find(const mt_t *t, const uint8_t *e, uint64_t h) {
+    uint64_t p = mt_pos(h) & t->smask;
+    for (;;) {
+        const int32_t s = t->slot[p];
+        if (!s) return -1;
+        const int32_t id = s - 1;
+        if (t->hv[id] == h && memcmp(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp) == 0)
+            return id;
+        p = (p + 1) & t->smask;
+    }
+}
+
 static uint64_t mt_hash(const mt_t *t, const uint8_t *e) {
     uint64_t h = 0;
     for (int i = 0; i < t->nv; ++i) h += t->rnd[i] * e[i];
@@ -270,9 +284,24 @@ static uint32_t inv_mod(uint32_t a, uint32_t p) {
  * them divides k's lead, k is redundant from the start and only the pair
  * with that element is kept (its S-polynomial reduces k further) -- gb's
  * rule in insert_and_update_spairs (update.c:96-112). */
+static inline int lcm_is(const mt_t *T, int32_t a, int32_t b, int32_t l) {
+    /* lcm(a, b) == l, on the exponents */
+    const uint8_t *ea = EV(T, a), *eb = EV(T, b), *el = EV(T, l);
+    for (int v = 0; v < T->nv; ++v)
+        if ((ea[v] > eb[v] ? ea[v] : eb[v]) != el[v]) return 0;
+    return 1;
+}
+
+static inline int ediv(const uint8_t *a, const uint8_t *b, int nv) {
+    for (int v = 0; v < nv; ++v)
+        if (a[v] > b[v]) return 0;
+    return 1;
+}
+
 static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) {
     const int32_t lk = B->e[k].m[0];
     const int32_t nl = B->nlm;
+    const int nvp = T->nvp, nv = T->nv;
     int rc = 0;
     for (int32_t i = lo; i < k; ++i) {
         if (B->red[i] || !mt_divides(T, B->e[i].m[0], lk)) continue;
@@ -280,17 +309,33 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
         B->red[k] = 1;
         return pairs_push(P, x);
     }
-    /* candidate pairs (i, k) */
-    int32_t *lc = (int32_t *)malloc((size_t)(nl ? nl : 1) * sizeof(int32_t));
-    uint8_t *st = (uint8_t *)malloc((size_t)(nl ? nl : 1));  /* 0 dropped, 1 kept, 2 coprime */
-    if (!lc || !st) { free(lc); free(st); return -1; }
-    for (int32_t q = 0; q < nl; ++q) {
-        const int32_t i = B->lm[q];
-        const int32_t li = B->e[i].m[0];
-        lc[q] = mt_lcm(T, li, lk);
-        if (lc[q] < 0) { rc = -1; goto out; }
-        /* pairs with redundant elements (kept as reducers in lex) are not formed */
-        st[q] = B->red[i] ? 0 : (T->deg[lc[q]] == T->deg[li] + T->deg[lk]) ? 2 : 1;
+    /* candidate pairs (i, k): lcm exponents, degree and mask kept locally;
+     * only the pairs that survive get a monomial id */
+    const size_t nn = nl ? (size_t)nl : 1;
+    uint8_t *le = (uint8_t *)malloc(nn * (size_t)nvp);
+    uint32_t *ld = (uint32_t *)malloc(nn * sizeof(uint32_t));
+    uint64_t *lm = (uint64_t *)malloc(nn * sizeof(uint64_t));
+    uint8_t *st = (uint8_t *)malloc(nn);  /* 0 dropped, 1 kept, 2 coprime */
+    if (!le || !ld || !lm || !st) { rc = -1; goto out; }
+    {
+        const uint8_t *ek = EV(T, lk);
+        for (int32_t q = 0; q < nl; ++q) {
+            const int32_t i = B->lm[q];
+            const int32_t li = B->e[i].m[0];
+            const uint8_t *ei = EV(T, li);
+            uint8_t *e = le + (size_t)q * nvp;
+            uint32_t d = 0;
+            uint64_t m = 0;
+            for (int v = 0; v < nvp; ++v) {
+                e[v] = ei[v] > ek[v] ? ei[v] : ek[v];
+                d += e[v];
+                if (e[v]) m |= 1ull << v;
+            }
+            ld[q] = d;
+            lm[q] = m;
+            /* pairs with redundant elements (kept as reducers in lex) are not formed */
+            st[q] = B->red[i] ? 0 : (d == T->deg[li] + T->deg[lk]) ? 2 : 1;
+        }
     }
     /* old pairs: chain criterion */
     {
@@ -298,12 +343,9 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
         for (int64_t q = 0; q < P->n; ++q) {
             const pair_t x = P->p[q];
             int drop = 0;
-            if (mt_divides(T, lk, x.lcm)) {
-                const int32_t a = mt_lcm(T, B->e[x.i].m[0], lk);
-                const int32_t b = mt_lcm(T, B->e[x.j].m[0], lk);
-                if (a < 0 || b < 0) { rc = -1; goto out; }
-                if (a != x.lcm && b != x.lcm) drop = 1;
-            }
+            if (mt_divides(T, lk, x.lcm) && !lcm_is(T, B->e[x.i].m[0], lk, x.lcm) &&
+                !lcm_is(T, B->e[x.j].m[0], lk, x.lcm))
+                drop = 1;
             if (!drop) P->p[w++] = x;
         }
         P->n = w;
@@ -312,10 +354,12 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
      * divides its lcm (for equal lcms the later one survives); coprime
      * pairs take part in this and are dropped afterwards */
     for (int32_t q = 0; q < nl; ++q) {
-        if (st[q] == 2) continue;
+        if (st[q] == 2 || st[q] == 0) continue;
+        const uint8_t *eq = le + (size_t)q * nvp;
         for (int32_t r = 0; r < nl; ++r) {
-            if (r == q || st[r] == 0) continue;
-            if (lc[r] == lc[q] ? r > q : mt_divides(T, lc[r], lc[q])) {
+            if (r == q || st[r] == 0 || ld[r] > ld[q] || (lm[r] & ~lm[q])) continue;
+            const uint8_t *er = le + (size_t)r * nvp;
+            if (ld[r] == ld[q] ? (r > q && memcmp(er, eq, (size_t)nvp) == 0) : ediv(er, eq, nv)) {
                 st[q] = 0;
                 break;
             }
@@ -325,11 +369,17 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
         if (st[q] != 1) continue;
         /* a coprime pair with the same lcm removes the whole group */
         int cop = 0;
+        const uint8_t *eq = le + (size_t)q * nvp;
         for (int32_t r = 0; r < nl; ++r)
-            if (st[r] == 2 && lc[r] == lc[q]) { cop = 1; break; }
+            if (st[r] == 2 && ld[r] == ld[q] && memcmp(le + (size_t)r * nvp, eq, (size_t)nvp) == 0) {
+                cop = 1;
+                break;
+            }
         if (cop) continue;
         const int32_t i = B->lm[q];
-        pair_t x = {i < k ? i : k, i < k ? k : i, lc[q], T->deg[lc[q]]};
+        const int32_t l = mt_get(T, eq, mt_hash(T, eq));
+        if (l < 0) { rc = -1; goto out; }
+        pair_t x = {i < k ? i : k, i < k ? k : i, l, T->deg[l]};
         if (pairs_push(P, x)) { rc = -1; goto out; }
     }
     /* redundant elements: dropped from the reducer list for DRL; for lex
@@ -346,7 +396,9 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
         B->lm[B->nlm++] = k;
     }
 out:
-    free(lc);
+    free(le);
+    free(ld);
+    free(lm);
     free(st);
     return rc;
 }
@@ -361,11 +413,14 @@ typedef struct {
     int32_t mul;  /* multiplier monomial */
     int32_t lead; /* lead monomial of the row */
     int32_t len;
+    int64_t off;  /* its monomials: rows_t.mon[off .. off + len) */
 } row_t;
 
 typedef struct {
     row_t *r;
     int64_t n, cap;
+    int32_t *mon;  /* the rows' monomials, found once by symbolic preprocessing */
+    int64_t nmon, capmon;
 } rows_t;
 
 static int rows_push(rows_t *R, row_t x) {
@@ -481,15 +536,16 @@ static int build_csr(f4_t *F, rows_t *R, int32_t *cols, int32_t nc, int32_t ncl,
     rp[0] = 0;
     for (int64_t q = 0; q < R->n; ++q) {
         const row_t *x = &R->r[key[q].idx];
-        const poly_t *f = &F->B.e[x->e];
-        int32_t *o = ci + rp[q];
-        for (int32_t t = 0; t < f->len; ++t) {
-            const int32_t mon = mt_mul(T, x->mul, f->m[t]);
-            if (mon < 0) { free(cp); free(cf); goto fail; }
-            o[t] = F->M.col[mon];
-        }
-        rp[q + 1] = rp[q] + f->len;
+        rp[q + 1] = rp[q] + x->len;
         rcf[q] = eid[x->e];
+    }
+    const int32_t *colmap = F->M.col;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t q = 0; q < R->n; ++q) {
+        const row_t *x = &R->r[key[q].idx];
+        int32_t *o = ci + rp[q];
+        const int32_t *mon = R->mon + x->off;
+        for (int32_t t = 0; t < x->len; ++t) o[t] = colmap[mon[t]];
     }
     free(key);
     free(eid);
@@ -516,20 +572,73 @@ static int find_reducer(f4_t *F, int32_t mon) {
     return -1;
 }
 
-/* add row (e, mul) if new; marks its monomials as columns, queueing the
- * ones not seen before */
-static int add_row(f4_t *F, rows_t *R, int32_t e, int32_t mul, int32_t **queue, int64_t *nq,
-                   int64_t *capq) {
+/* A round's rows are added in batches: stubs (element, multiplier) first,
+ * then the monomials of the whole batch -- looked up in parallel while the
+ * table is read only, the few new ones inserted afterwards in row order --
+ * then one pass marking the columns (new ones queued) and the leads. */
+static int row_stub(rows_t *R, const basis_t *B, int32_t e, int32_t mul) {
+    const int32_t len = B->e[e].len;
+    if (R->nmon + len > R->capmon) {
+        int64_t nc = R->capmon ? R->capmon : 1 << 16;
+        while (nc < R->nmon + len) nc *= 2;
+        int32_t *nm = (int32_t *)realloc(R->mon, (size_t)nc * sizeof(int32_t));
+        if (!nm) return -1;
+        R->mon = nm;
+        R->capmon = nc;
+    }
+    row_t x = {e, mul, -1, len, R->nmon};
+    R->nmon += len;
+    return rows_push(R, x);
+}
+
+static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
     mt_t *T = &F->T;
-    const poly_t *f = &F->B.e[e];
-    for (int32_t t = 0; t < f->len; ++t) {
-        const int32_t mon = mt_mul(T, mul, f->m[t]);
-        if (mon < 0) return -1;
-        if (marks_fit(&F->M, T->n)) return -1;
-        if (F->M.stamp[mon] != F->round) {
-            F->M.stamp[mon] = F->round;
-            F->M.col[mon] = -1;
-            F->M.piv[mon] = 0;
+    const basis_t *B = &F->B;
+    const int nw = T->nvp / 8;
+    int64_t nmiss = 0;
+#pragma omp parallel reduction(+ : nmiss)
+    {
+        uint64_t tmp[8];
+#pragma omp for schedule(dynamic, 8)
+        for (int64_t r = r0; r < r1; ++r) {
+            const row_t *x = &R->r[r];
+            const poly_t *f = &B->e[x->e];
+            const uint64_t *em = (const uint64_t *)EV(T, x->mul);
+            const uint64_t hm = T->hv[x->mul];
+            int32_t *o = R->mon + x->off;
+            for (int32_t t = 0; t < f->len; ++t) {
+                const uint64_t *et = (const uint64_t *)EV(T, f->m[t]);
+                for (int w = 0; w < nw; ++w) tmp[w] = em[w] + et[w];  /* no carries: exponents < 256 */
+                o[t] = mt_find(T, (const uint8_t *)tmp, hm + T->hv[f->m[t]]);
+                nmiss += o[t] < 0;
+            }
+        }
+    }
+    if (nmiss) {
+        for (int64_t r = r0; r < r1; ++r) {
+            const row_t *x = &R->r[r];
+            const poly_t *f = &B->e[x->e];
+            int32_t *o = R->mon + x->off;
+            for (int32_t t = 0; t < f->len; ++t)
+                if (o[t] < 0 && (o[t] = mt_mul(T, x->mul, f->m[t])) < 0) return -1;
+        }
+    }
+    for (int64_t r = r0; r < r1; ++r) R->r[r].lead = R->mon[R->r[r].off];
+    return 0;
+}
+
+static int rows_mark(f4_t *F, rows_t *R, int64_t r0, int64_t r1, int32_t **queue, int64_t *nq,
+                     int64_t *capq) {
+    if (marks_fit(&F->M, F->T.n)) return -1;
+    for (int64_t r = r0; r < r1; ++r) {
+        const row_t *x = &R->r[r];
+        const int32_t *mon = R->mon + x->off;
+        for (int32_t t = 0; t < x->len; ++t) {
+            const int32_t m = mon[t];
+            if (F->M.stamp[m] == F->round) continue;
+            F->M.stamp[m] = F->round;
+            F->M.col[m] = -1;
+            F->M.piv[m] = 0;
             if (*nq == *capq) {
                 int64_t nc = *capq ? 2 * *capq : 4096;
                 int32_t *nqp = (int32_t *)realloc(*queue, (size_t)nc * sizeof(int32_t));
@@ -537,13 +646,41 @@ static int add_row(f4_t *F, rows_t *R, int32_t e, int32_t mul, int32_t **queue, 
                 *queue = nqp;
                 *capq = nc;
             }
-            (*queue)[(*nq)++] = mon;
+            (*queue)[(*nq)++] = m;
         }
-        if (t == 0) {
-            F->M.piv[mon] = 1;
-            row_t x = {e, mul, mon, f->len};
-            if (rows_push(R, x)) return -1;
+        F->M.piv[x->lead] = 1;
+    }
+    return 0;
+}
+
+/* symbolic preprocessing: a reducer for every queued column monomial that
+ * is no lead yet and is divisible by a basis lead, until no new column
+ * appears; the reducers of one generation of columns are searched in
+ * parallel (the result does not depend on the order) */
+static int symbolic(f4_t *F, rows_t *R, int32_t **queue, int64_t *nq, int64_t *capq) {
+    mt_t *T = &F->T;
+    const basis_t *B = &F->B;
+    int64_t qdone = 0;
+    while (qdone < *nq) {
+        const int64_t qend = *nq;
+        int32_t *red = (int32_t *)malloc((size_t)(qend - qdone) * sizeof(int32_t));
+        if (!red) return -1;
+        const int32_t *qq = *queue;
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int64_t q = qdone; q < qend; ++q) {
+            const int32_t mon = qq[q];
+            red[q - qdone] = F->M.piv[mon] ? -1 : find_reducer(F, mon);
         }
+        const int64_t r0 = R->n;
+        for (int64_t q = qdone; q < qend; ++q) {
+            const int32_t e = red[q - qdone];
+            if (e < 0) continue;
+            const int32_t mul = mt_div(T, qq[q], B->e[e].m[0]);
+            if (mul < 0 || row_stub(R, B, e, mul)) { free(red); return -1; }
+        }
+        free(red);
+        if (rows_products(F, R, r0, R->n) || rows_mark(F, R, r0, R->n, queue, nq, capq)) return -1;
+        qdone = qend;
     }
     return 0;
 }
@@ -591,6 +728,7 @@ static int f4_round(f4_t *F) {
     pairs_t *P = &F->P;
     int rc = GBGPU_ENOMEM;
     ++F->round;
+    double tp = now_ms();
     /* selection: all pairs of minimal degree */
     qsort(P->p, (size_t)P->n, sizeof(pair_t), cmp_pair_deg);
     const uint32_t dmin = P->p[0].deg;
@@ -620,20 +758,15 @@ static int f4_round(f4_t *F) {
     qsort(gen, (size_t)(2 * nsel), sizeof(em_t), cmp_em);
     for (int64_t q = 0; q < 2 * nsel; ++q) {
         if (q > 0 && gen[q].e == gen[q - 1].e && gen[q].mul == gen[q - 1].mul) continue;
-        if (add_row(F, &R, gen[q].e, gen[q].mul, &queue, &nq, &capq)) goto done;
+        if (row_stub(&R, B, gen[q].e, gen[q].mul)) goto done;
     }
+    if (rows_products(F, &R, 0, R.n) || rows_mark(F, &R, 0, R.n, &queue, &nq, &capq)) goto done;
     const int64_t nspair_rows = R.n;
-    /* symbolic preprocessing: a reducer for every column monomial that is
-     * no lead yet and is divisible by a basis lead */
-    for (int64_t q = 0; q < nq; ++q) {
-        const int32_t mon = queue[q];
-        if (F->M.piv[mon]) continue;
-        const int32_t e = find_reducer(F, mon);
-        if (e < 0) continue;
-        const int32_t mul = mt_div(T, mon, B->e[e].m[0]);
-        if (mul < 0) goto done;
-        if (add_row(F, &R, e, mul, &queue, &nq, &capq)) goto done;
-    }
+    F->out->select_ms += now_ms() - tp;
+    tp = now_ms();
+    if (symbolic(F, &R, &queue, &nq, &capq)) goto done;
+    F->out->symbolic_ms += now_ms() - tp;
+    tp = now_ms();
     /* columns: pivot columns, then the rest, each decreasing */
     cols = (int32_t *)malloc((size_t)(nq ? nq : 1) * sizeof(int32_t));
     if (!cols) goto done;
@@ -652,6 +785,7 @@ static int f4_round(f4_t *F) {
     gbgpu_csr_t m;
     if (build_csr(F, &R, cols, nc, ncl, &m, &rp, &ci, &rcf, &cp, &cf)) goto done;
     int64_t nnz = rp[m.nr];
+    F->out->build_ms += now_ms() - tp;
     if (m.nr > F->out->max_rows) F->out->max_rows = m.nr;
     if (nc > F->out->max_cols) F->out->max_cols = nc;
     if (nnz > F->out->max_nnz) F->out->max_nnz = nnz;
@@ -661,6 +795,7 @@ static int f4_round(f4_t *F) {
     const double tla = now_ms() - t0;
     /* new elements in the order the LA returns them (decreasing lead) */
     rc = GBGPU_ENOMEM;
+    tp = now_ms();
     const int32_t lo = B->n;
     for (int32_t k = 0; k < out.np; ++k) {
         poly_t f;
@@ -668,6 +803,7 @@ static int f4_round(f4_t *F) {
         if (basis_push(B, f)) goto done;
         if (update_pairs(T, B, P, B->n - 1, lo)) goto done;
     }
+    F->out->update_ms += now_ms() - tp;
     if (F->info > 1)
         printf("%3u  %6lld sel  %8lld pairs  %7d x %-7d (%lld spair rows)  %6.2f%%  %5d new  %8.1f ms LA\n",
                dmin, (long long)nsel, (long long)P->n, m.nr, nc, (long long)nspair_rows,
@@ -676,7 +812,7 @@ static int f4_round(f4_t *F) {
     rc = GBGPU_OK;
 done:
     gbgpu_rows_free(&out);
-    free(gen); free(R.r); free(queue); free(cols);
+    free(gen); free(R.r); free(R.mon); free(queue); free(cols);
     free(rp); free(ci); free(rcf); free(cp); free(cf);
     return rc;
 }
@@ -703,18 +839,11 @@ static int f4_reduce(f4_t *F) {
         if (one < 0) goto done;
     }
     for (int32_t q = 0; q < B->nlm; ++q)
-        if (add_row(F, &R, B->lm[q], one, &queue, &nq, &capq)) goto done;
+        if (row_stub(&R, B, B->lm[q], one)) goto done;
+    if (rows_products(F, &R, 0, R.n) || rows_mark(F, &R, 0, R.n, &queue, &nq, &capq)) goto done;
     /* reducers for the tail monomials divisible by a lead, so that the
      * reduced rows have no such monomial left */
-    for (int64_t q = 0; q < nq; ++q) {
-        const int32_t mon = queue[q];
-        if (F->M.piv[mon]) continue;
-        const int32_t e = find_reducer(F, mon);
-        if (e < 0) continue;
-        const int32_t mul = mt_div(T, mon, B->e[e].m[0]);
-        if (mul < 0) goto done;
-        if (add_row(F, &R, e, mul, &queue, &nq, &capq)) goto done;
-    }
+    if (symbolic(F, &R, &queue, &nq, &capq)) goto done;
     cols = (int32_t *)malloc((size_t)(nq ? nq : 1) * sizeof(int32_t));
     if (!cols) goto done;
     memcpy(cols, queue, (size_t)nq * sizeof(int32_t));
@@ -743,7 +872,7 @@ static int f4_reduce(f4_t *F) {
     rc = GBGPU_OK;
 done:
     gbgpu_rows_free(&out);
-    free(R.r); free(queue); free(cols);
+    free(R.r); free(R.mon); free(queue); free(cols);
     free(rp); free(ci); free(rcf); free(cp); free(cf);
     return rc;
 }

@@ -54,7 +54,8 @@ class _F4Out(C.Structure):
                 ("bexp", C.POINTER(C.c_int32)), ("bcf", C.POINTER(C.c_uint32)),
                 ("nterms", C.c_int64), ("rounds", C.c_int32), ("max_rows", C.c_int64),
                 ("max_cols", C.c_int64), ("max_nnz", C.c_int64), ("rows_total", C.c_int64),
-                ("la_ms", C.c_double), ("total_ms", C.c_double)]
+                ("la_ms", C.c_double), ("total_ms", C.c_double), ("select_ms", C.c_double),
+                ("symbolic_ms", C.c_double), ("build_ms", C.c_double), ("update_ms", C.c_double)]
 
 
 class _F4Gpu(C.Structure):
@@ -117,7 +118,9 @@ def f4(system: PolySystem, la_fn: int, la_user, reduce_gb: int = 0, info_level: 
               cfs=np.ctypeslib.as_array(out.bcf, shape=(nt,)).copy() if nt else np.zeros(0, np.uint32),
               stats={"rounds": out.rounds, "max_rows": out.max_rows, "max_cols": out.max_cols,
                      "max_nnz": out.max_nnz, "rows_total": out.rows_total, "la_ms": out.la_ms,
-                     "total_ms": out.total_ms})
+                     "total_ms": out.total_ms, "select_ms": out.select_ms,
+                     "symbolic_ms": out.symbolic_ms, "build_ms": out.build_ms,
+                     "update_ms": out.update_ms})
     L.gbgpu_f4_out_free(C.byref(out))
     return b
 

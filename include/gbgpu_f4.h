@@ -50,6 +50,7 @@ typedef struct {
     int64_t rows_total;                   /* rows to reduce, summed over rounds */
     double la_ms;                         /* wall time inside the callback */
     double total_ms;
+    double select_ms, symbolic_ms, build_ms, update_ms;  /* host phases */
 } gbgpu_f4_out_t;
 
 /* Linear algebra of one round: interreduce == 0 is gb's linear_algebra on
