@@ -25,6 +25,7 @@
  *  * Columns: pivot columns (leads of rows) first, then the others, each
  *    block in decreasing monomial order; rows sorted by (lead, length)
  *    (order.c:98-131). */
+#include <limits.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -153,6 +154,84 @@ static int32_t mt_find(const mt_t *t, const uint8_t *e, uint64_t h) {
     for (;;) {
         const int32_t s = t->slot[p];
         if (!s) return -1;
+        const int32_t id = s - 1;
+        if (t->hv[id] == h && memcmp(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp) == 0)
+            return id;
+        p = (p + 1) & t->smask;
+    }
+}
+
+/* room for `extra` more monomials with neither reallocation nor rehash */
+static int mt_reserve(mt_t *t, int64_t extra) {
+    const int64_t need = (int64_t)t->n + extra + 1;
+    if (need > INT32_MAX / 2) return -1;
+    if (need > t->cap) {
+        int32_t nc = t->cap;
+        while (nc < need) nc *= 2;
+        uint8_t *ev = (uint8_t *)realloc(t->ev, (size_t)nc * (size_t)t->nvp);
+        uint32_t *dg = (uint32_t *)realloc(t->deg, (size_t)nc * sizeof(uint32_t));
+        uint64_t *hv = (uint64_t *)realloc(t->hv, (size_t)nc * sizeof(uint64_t));
+        uint64_t *dm = (uint64_t *)realloc(t->dm, (size_t)nc * sizeof(uint64_t));
+        if (ev) t->ev = ev;
+        if (dg) t->deg = dg;
+        if (hv) t->hv = hv;
+        if (dm) t->dm = dm;
+        if (!ev || !dg || !hv || !dm) return -1;
+        t->cap = nc;
+    }
+    if ((uint64_t)need * 2 > t->smask) {
+        uint64_t nm = t->smask;
+        while ((uint64_t)need * 2 > nm) nm = nm * 2 + 1;
+        int32_t *sl = (int32_t *)calloc((size_t)nm + 1, sizeof(int32_t));
+        if (!sl) return -1;
+        for (int32_t i = 0; i < t->n; ++i) {
+            uint64_t p = mt_pos(t->hv[i]) & nm;
+            while (sl[p]) p = (p + 1) & nm;
+            sl[p] = i + 1;
+        }
+        free(t->slot);
+        t->slot = sl;
+        t->smask = nm;
+    }
+    return 0;
+}
+
+/* Insertion from many threads at once, after mt_reserve: a slot is
+ * claimed by compare-and-swap (0 -> -1), filled, then published as id + 1;
+ * a thread meeting a claimed slot waits for it.  Ids come from an atomic
+ * counter; one at or above lim is not stored (the slot is released and
+ * *over set) -- the caller falls back to the serial path. */
+static int32_t mt_get_par(mt_t *t, const uint8_t *e, uint64_t h, int32_t lim, int *over) {
+    uint64_t p = mt_pos(h) & t->smask;
+    for (;;) {
+        int32_t s = __atomic_load_n(&t->slot[p], __ATOMIC_ACQUIRE);
+        if (s == 0) {
+            int32_t expect = 0;
+            if (__atomic_compare_exchange_n(&t->slot[p], &expect, -1, 0, __ATOMIC_ACQ_REL,
+                                            __ATOMIC_ACQUIRE)) {
+                const int32_t id = __atomic_fetch_add(&t->n, 1, __ATOMIC_RELAXED);
+                if (id >= lim) {
+                    __atomic_store_n(&t->slot[p], 0, __ATOMIC_RELEASE);
+                    __atomic_store_n(over, 1, __ATOMIC_RELAXED);
+                    return -1;
+                }
+                memcpy(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp);
+                uint32_t d = 0;
+                uint64_t m = 0;
+                for (int i = 0; i < t->nv; ++i) {
+                    d += e[i];
+                    if (e[i]) m |= 1ull << i;
+                }
+                t->deg[id] = d;
+                t->hv[id] = h;
+                t->dm[id] = m;
+                __atomic_store_n(&t->slot[p], id + 1, __ATOMIC_RELEASE);
+                return id;
+            }
+            s = expect;
+        }
+        while (s == -1) s = __atomic_load_n(&t->slot[p], __ATOMIC_ACQUIRE);
+        if (s == 0) continue;  /* released by an overflowing claim: look again */
         const int32_t id = s - 1;
         if (t->hv[id] == h && memcmp(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp) == 0)
             return id;
@@ -337,33 +416,51 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
             st[q] = B->red[i] ? 0 : (d == T->deg[li] + T->deg[lk]) ? 2 : 1;
         }
     }
-    /* old pairs: chain criterion */
+    /* old pairs: chain criterion (marks in parallel, then compaction) */
     {
-        int64_t w = 0;
-        for (int64_t q = 0; q < P->n; ++q) {
-            const pair_t x = P->p[q];
-            int drop = 0;
-            if (mt_divides(T, lk, x.lcm) && !lcm_is(T, B->e[x.i].m[0], lk, x.lcm) &&
-                !lcm_is(T, B->e[x.j].m[0], lk, x.lcm))
-                drop = 1;
-            if (!drop) P->p[w++] = x;
+        const int64_t np = P->n;
+        pair_t *pp = P->p;
+        if (np > 0) {
+            uint8_t *drop = (uint8_t *)malloc((size_t)np);
+            if (!drop) { rc = -1; goto out; }
+#pragma omp parallel for schedule(static) if (np > 4096)
+            for (int64_t q = 0; q < np; ++q) {
+                const pair_t x = pp[q];
+                drop[q] = mt_divides(T, lk, x.lcm) && !lcm_is(T, B->e[x.i].m[0], lk, x.lcm) &&
+                          !lcm_is(T, B->e[x.j].m[0], lk, x.lcm);
+            }
+            int64_t w = 0;
+            for (int64_t q = 0; q < np; ++q)
+                if (!drop[q]) pp[w++] = pp[q];
+            P->n = w;
+            free(drop);
         }
-        P->n = w;
     }
-    /* new pairs: a pair is dropped when another remaining pair's lcm
-     * divides its lcm (for equal lcms the later one survives); coprime
-     * pairs take part in this and are dropped afterwards */
-    for (int32_t q = 0; q < nl; ++q) {
-        if (st[q] == 2 || st[q] == 0) continue;
-        const uint8_t *eq = le + (size_t)q * nvp;
-        for (int32_t r = 0; r < nl; ++r) {
-            if (r == q || st[r] == 0 || ld[r] > ld[q] || (lm[r] & ~lm[q])) continue;
-            const uint8_t *er = le + (size_t)r * nvp;
-            if (ld[r] == ld[q] ? (r > q && memcmp(er, eq, (size_t)nvp) == 0) : ediv(er, eq, nv)) {
-                st[q] = 0;
-                break;
+    /* new pairs: a pair is dropped when another pair's lcm properly
+     * divides its lcm, or equals it and the other pair comes later;
+     * coprime pairs take part in this and are dropped afterwards.  (Taken
+     * in order with dropped pairs left out, the Gebauer-Moeller loop drops
+     * exactly these pairs: a divisor that is itself dropped has a divisor
+     * that is not.)  Each candidate is checked on its own, in parallel. */
+    {
+        uint8_t *keep = (uint8_t *)malloc(nn);
+        if (!keep) { rc = -1; goto out; }
+#pragma omp parallel for schedule(dynamic, 16) if (nl > 256)
+        for (int32_t q = 0; q < nl; ++q) {
+            keep[q] = st[q];
+            if (st[q] != 1) continue;
+            const uint8_t *eq = le + (size_t)q * nvp;
+            for (int32_t r = 0; r < nl; ++r) {
+                if (r == q || st[r] == 0 || ld[r] > ld[q] || (lm[r] & ~lm[q])) continue;
+                const uint8_t *er = le + (size_t)r * nvp;
+                if (ld[r] == ld[q] ? (r > q && memcmp(er, eq, (size_t)nvp) == 0) : ediv(er, eq, nv)) {
+                    keep[q] = 0;
+                    break;
+                }
             }
         }
+        memcpy(st, keep, nn);
+        free(keep);
     }
     for (int32_t q = 0; q < nl; ++q) {
         if (st[q] != 1) continue;
@@ -615,6 +712,35 @@ static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
         }
     }
     if (nmiss) {
+        /* new monomials: inserted by all threads into reserved room (the
+         * misses count repeats, so the room is capped) */
+        int64_t room = nmiss;
+        const int64_t cap_room = (int64_t)T->n * 2 > (1 << 22) ? (int64_t)T->n * 2 : (1 << 22);
+        if (room > cap_room) room = cap_room;
+        int over = 0;
+        if (mt_reserve(T, room) == 0) {
+            const int32_t lim = (int32_t)(T->n + room);
+#pragma omp parallel
+            {
+                uint64_t tmp[8];
+#pragma omp for schedule(dynamic, 8)
+                for (int64_t r = r0; r < r1; ++r) {
+                    const row_t *x = &R->r[r];
+                    const poly_t *f = &B->e[x->e];
+                    const uint64_t *em = (const uint64_t *)EV(T, x->mul);
+                    const uint64_t hm = T->hv[x->mul];
+                    int32_t *o = R->mon + x->off;
+                    for (int32_t t = 0; t < f->len; ++t) {
+                        if (o[t] >= 0) continue;
+                        const uint64_t *et = (const uint64_t *)EV(T, f->m[t]);
+                        for (int w = 0; w < nw; ++w) tmp[w] = em[w] + et[w];
+                        o[t] = mt_get_par(T, (const uint8_t *)tmp, hm + T->hv[f->m[t]], lim, &over);
+                    }
+                }
+            }
+            if (T->n > lim) T->n = lim;
+        }
+        /* whatever is left (no room, or the room ran out): one thread */
         for (int64_t r = r0; r < r1; ++r) {
             const row_t *x = &R->r[r];
             const poly_t *f = &B->e[x->e];
@@ -627,29 +753,60 @@ static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
     return 0;
 }
 
+static int queue_push(int32_t **queue, int64_t *nq, int64_t *capq, const int32_t *v, int64_t n) {
+    if (*nq + n > *capq) {
+        int64_t nc = *capq ? *capq : 4096;
+        while (nc < *nq + n) nc *= 2;
+        int32_t *nqp = (int32_t *)realloc(*queue, (size_t)nc * sizeof(int32_t));
+        if (!nqp) return -1;
+        *queue = nqp;
+        *capq = nc;
+    }
+    memcpy(*queue + *nq, v, (size_t)n * sizeof(int32_t));
+    *nq += n;
+    return 0;
+}
+
+/* first sight of a monomial in this round: whichever thread swaps the
+ * stamp queues it (the queue order does not matter: columns are sorted,
+ * reducers do not depend on the order) */
 static int rows_mark(f4_t *F, rows_t *R, int64_t r0, int64_t r1, int32_t **queue, int64_t *nq,
                      int64_t *capq) {
     if (marks_fit(&F->M, F->T.n)) return -1;
-    for (int64_t r = r0; r < r1; ++r) {
-        const row_t *x = &R->r[r];
-        const int32_t *mon = R->mon + x->off;
-        for (int32_t t = 0; t < x->len; ++t) {
-            const int32_t m = mon[t];
-            if (F->M.stamp[m] == F->round) continue;
-            F->M.stamp[m] = F->round;
-            F->M.col[m] = -1;
-            F->M.piv[m] = 0;
-            if (*nq == *capq) {
-                int64_t nc = *capq ? 2 * *capq : 4096;
-                int32_t *nqp = (int32_t *)realloc(*queue, (size_t)nc * sizeof(int32_t));
-                if (!nqp) return -1;
-                *queue = nqp;
-                *capq = nc;
+    const int32_t round = F->round;
+    int32_t *stamp = F->M.stamp;
+    int bad = 0;
+#pragma omp parallel
+    {
+        int32_t *loc = NULL;
+        int64_t nloc = 0, cap = 0;
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t r = r0; r < r1; ++r) {
+            const row_t *x = &R->r[r];
+            const int32_t *mon = R->mon + x->off;
+            for (int32_t t = 0; t < x->len; ++t) {
+                const int32_t m = mon[t];
+                if (__atomic_load_n(&stamp[m], __ATOMIC_RELAXED) == round) continue;
+                if (__atomic_exchange_n(&stamp[m], round, __ATOMIC_RELAXED) == round) continue;
+                F->M.col[m] = -1;
+                F->M.piv[m] = 0;
+                if (nloc == cap) {
+                    cap = cap ? 2 * cap : 1024;
+                    int32_t *nl2 = (int32_t *)realloc(loc, (size_t)cap * sizeof(int32_t));
+                    if (!nl2) { bad = 1; cap = nloc; continue; }
+                    loc = nl2;
+                }
+                loc[nloc++] = m;
             }
-            (*queue)[(*nq)++] = m;
         }
-        F->M.piv[x->lead] = 1;
+#pragma omp critical
+        {
+            if (!bad && nloc && queue_push(queue, nq, capq, loc, nloc)) bad = 1;
+        }
+        free(loc);
     }
+    if (bad) return -1;
+    for (int64_t r = r0; r < r1; ++r) F->M.piv[R->r[r].lead] = 1;
     return 0;
 }
 
