@@ -72,6 +72,65 @@ def cpu_baseline(m, nthreads: int):
     return t
 
 
+def f4_runs(nvars: int, ctx, with_cpu: bool, forms=("mfma", "plain"), cpu_vars: int = 12,
+            info: int = 0) -> dict:
+    """End to end F4 (the F4 host, every round's linear algebra on the GPU)
+    on a seeded dense quadratic system: nvars quadratics in nvars
+    variables, every monomial of degree <= 2, coefficients uniform in
+    [1, p - 1] (splitmix64, seed 1), p = 65521, DRL; plain and MFMA forms;
+    and, with with_cpu, the same run with the oracle's linear algebra on
+    this host's cores (on the cpu_vars-variable system: the same one when
+    cpu_vars >= nvars, otherwise a smaller system as the bounded sample,
+    with the GPU run on it beside)."""
+    import numpy as np
+    from gb_amd import native, synth
+    res = {"system": f"{nvars} dense quadratics in {nvars} variables mod 65521, DRL, seed 1"}
+    s = synth.dense_quadratic_system(nvars, 65521, seed=1)
+    runs = {}
+    for form, name in ((native.FORM_MFMA, "mfma"), (native.FORM_PLAIN, "plain")):
+        if name not in forms:
+            continue
+        _sync()
+        t0 = time.perf_counter()
+        b = ctx.f4(s, form=form, info_level=info)
+        _sync()
+        runs[name] = b
+        st = b.stats
+        res[name] = {"total_s": time.perf_counter() - t0, "la_s": st["la_ms"] / 1e3,
+                     "la_device_s": st["device_ms"] / 1e3,
+                     "host_s": (st["total_ms"] - st["la_ms"]) / 1e3, "rounds": st["rounds"],
+                     "basis_elements": len(b.lens), "max_matrix": [st["max_rows"], st["max_cols"]],
+                     "max_nnz": st["max_nnz"], "rows_reduced": st["rows_total"],
+                     "rows_per_s_la": st["rows_total"] / (st["la_ms"] / 1e3)}
+    a = runs.get("mfma") or runs.get("plain")
+    if len(runs) == 2:
+        b2 = runs["plain"]
+        res["forms_identical"] = bool(np.array_equal(a.lens, b2.lens) and np.array_equal(a.exps, b2.exps)
+                                      and np.array_equal(a.cfs, b2.cfs))
+        res["mfma_vs_plain_total"] = res["plain"]["total_s"] / res["mfma"]["total_s"]
+        res["mfma_vs_plain_la"] = res["plain"]["la_s"] / res["mfma"]["la_s"]
+    if with_cpu:
+        from oracle import oracle
+        n_cpu = min(nvars, cpu_vars)
+        s_cpu = s if n_cpu == nvars else synth.dense_quadratic_system(n_cpu, 65521, seed=1)
+        threads = int(os.environ.get("BENCH_CPU_THREADS", min(16, os.cpu_count() or 1)))
+        t0 = time.perf_counter()
+        c = oracle.f4(s_cpu, nthreads=threads, info_level=info)
+        tc = time.perf_counter() - t0
+        cpu = {"vars": n_cpu, "threads": threads, "total_s": tc, "la_s": c.stats["la_ms"] / 1e3}
+        if n_cpu == nvars:
+            cpu["identical_to_gpu"] = bool(np.array_equal(c.lens, a.lens) and np.array_equal(c.exps, a.exps)
+                                           and np.array_equal(c.cfs, a.cfs))
+        else:
+            g = ctx.f4(s_cpu, form=native.FORM_MFMA)
+            cpu["gpu_mfma_total_s_same_system"] = g.stats["total_ms"] / 1e3
+            cpu["gpu_mfma_la_s_same_system"] = g.stats["la_ms"] / 1e3
+            cpu["identical_to_gpu"] = bool(np.array_equal(c.lens, g.lens) and np.array_equal(c.exps, g.exps)
+                                           and np.array_equal(c.cfs, g.cfs))
+        res["cpu_oracle"] = cpu
+    return res
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +138,14 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the extra workloads")
+    ap.add_argument("--f4-vars", type=int, default=12,
+                    help="variables of the dense quadratic system run end to end by the F4 host "
+                         "(14 = the 14-variable configuration; plain and MFMA forms)")
+    ap.add_argument("--only-f4", action="store_true", help="only the end-to-end F4 runs")
+    ap.add_argument("--f4-forms", default="mfma,plain", help="forms of the F4 runs")
+    ap.add_argument("--f4-info", action="store_true", help="print one line per F4 round")
+    ap.add_argument("--f4-cpu-vars", type=int, default=12,
+                    help="variables of the system the CPU (oracle LA) F4 run uses")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: the F4 step runs on a single MI355X "
@@ -88,8 +155,13 @@ def main(argv=None) -> int:
     import numpy as np
     from gb_amd import native, synth
 
-    m = synth.f4_shaped(NR, NC, NPIV, P, seed=SEED)
     ctx = native.Context(0)
+    if args.only_f4:
+        print(json.dumps({"f4": f4_runs(args.f4_vars, ctx, not args.no_cpu,
+                                        tuple(args.f4_forms.split(",")), args.f4_cpu_vars,
+                                        2 if args.f4_info else 0)}))
+        return 0
+    m = synth.f4_shaped(NR, NC, NPIV, P, seed=SEED)
     d = ctx.upload(m)
     nrl = m.nr - len(np.unique(m.leads()))
 
@@ -160,6 +232,8 @@ def main(argv=None) -> int:
 
     if not args.no_extra:
         extra = {}
+        extra["f4"] = f4_runs(args.f4_vars, ctx, not args.no_cpu, tuple(args.f4_forms.split(",")),
+                              args.f4_cpu_vars, 2 if args.f4_info else 0)
         for form, name in ((native.FORM_PLAIN, "plain"), (native.FORM_MFMA, "mfma")):
             d2 = ctx.upload(m)
             d2.la_sparse_dense(form)  # warm-up
