@@ -69,17 +69,6 @@ __device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p) 
     return a >= b ? a - b : a + p - b;
 }
 
-__device__ __forceinline__ uint32_t digit_word(uint32_t x, uint32_t p, uint32_t hi, int i) {
-    // digit i of x's representative (as an unsigned byte)
-    int64_t v = x <= hi ? (int64_t)x : (int64_t)x - (int64_t)p;
-    int8_t d = 0;
-    for (int k = 0; k <= i; ++k) {
-        d = (int8_t)(uint8_t)(v & 0xFF);
-        v = (v - d) >> 8;
-    }
-    return (uint8_t)d;
-}
-
 // ------------------------------------------------------------ select
 //
 // One workgroup finds the panel's pivots.  Candidate rows (rows that are no
@@ -581,7 +570,7 @@ __global__ __launch_bounds__(256) void k_update_plain(uint32_t *a, int64_t ld, i
 // of the digit planes, which stay in L2 across the grid.
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-template <int ND>
+template <int ND, int KC>
 __device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t m, int32_t cb,
                                                int32_t ce, int32_t c0, const Panel *pan,
                                                const int32_t *active, Mod md, const Digits &dg,
@@ -595,18 +584,6 @@ __device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t 
     const bool act = lane < 32 && r0 + lane < m && active[r0 + lane] != 0;
     if (!__any(act) || q0 >= ce) return;
     const int lr = lane & 15, lk = lane >> 4;
-    v4i A[ND][2], B[ND][2];
-#pragma unroll
-    for (int i = 0; i < ND; ++i)
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int32_t row = r0 + 16 * h + lr;
-            const int32_t col = q0 + 16 * h + lr;
-            A[i][h] = row < m ? *(const v4i *)(dg.L + i * dg.lplane + (int64_t)row * 64 + 16 * lk)
-                              : v4i{0, 0, 0, 0};
-            B[i][h] = col < ce ? *(const v4i *)(dg.R + i * dg.rplane + (int64_t)(col - c0) * 64 + 16 * lk)
-                               : v4i{0, 0, 0, 0};
-        }
     v4i acc[2][2][2 * ND - 1];
 #pragma unroll
     for (int rs = 0; rs < 2; ++rs)
@@ -614,16 +591,33 @@ __device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t 
         for (int cs = 0; cs < 2; ++cs)
 #pragma unroll
             for (int q = 0; q < 2 * ND - 1; ++q) acc[rs][cs][q] = v4i{0, 0, 0, 0};
+    // KC chunks of 64 pivot slots; the digit planes hold 64 KC bytes per row
+    for (int kc = 0; kc < KC; ++kc) {
+        v4i A[ND][2], B[ND][2];
 #pragma unroll
-    for (int i = 0; i < ND; ++i)
+        for (int i = 0; i < ND; ++i)
 #pragma unroll
-        for (int j = 0; j < ND; ++j)
+            for (int h = 0; h < 2; ++h) {
+                const int32_t row = r0 + 16 * h + lr;
+                const int32_t col = q0 + 16 * h + lr;
+                A[i][h] = row < m ? *(const v4i *)(dg.L + i * dg.lplane + (int64_t)row * (64 * KC) +
+                                                   64 * kc + 16 * lk)
+                                  : v4i{0, 0, 0, 0};
+                B[i][h] = col < ce ? *(const v4i *)(dg.R + i * dg.rplane +
+                                                    (int64_t)(col - c0) * (64 * KC) + 64 * kc + 16 * lk)
+                                   : v4i{0, 0, 0, 0};
+            }
 #pragma unroll
-            for (int rs = 0; rs < 2; ++rs)
+        for (int i = 0; i < ND; ++i)
 #pragma unroll
-                for (int cs = 0; cs < 2; ++cs)
-                    acc[rs][cs][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                        A[i][rs], B[j][cs], acc[rs][cs][i + j], 0, 0, 0);
+            for (int j = 0; j < ND; ++j)
+#pragma unroll
+                for (int rs = 0; rs < 2; ++rs)
+#pragma unroll
+                    for (int cs = 0; cs < 2; ++cs)
+                        acc[rs][cs][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                            A[i][rs], B[j][cs], acc[rs][cs][i + j], 0, 0, 0);
+    }
 #pragma unroll
     for (int rs = 0; rs < 2; ++rs)
 #pragma unroll
@@ -644,12 +638,34 @@ __device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t 
         }
 }
 
-template <int ND>
+template <int ND, int KC = 1>
 __global__ __launch_bounds__(256) void k_update_mfma(uint32_t *a, int64_t ld, int32_t m,
                                                      int32_t cb, int32_t ce, int32_t c0,
                                                      const Panel *pan, const int32_t *active,
                                                      Mod md, Digits dg) {
-    update_i8_body<ND>(a, ld, m, cb, ce, c0, pan, active, md, dg, blockIdx.x, blockIdx.y);
+    update_i8_body<ND, KC>(a, ld, m, cb, ce, c0, pan, active, md, dg, blockIdx.x, blockIdx.y);
+}
+
+void launch_update_i8(int nd, int kc, dim3 grid, hipStream_t s, uint32_t *a, int64_t ld,
+                      int32_t m, int32_t cb, int32_t ce, int32_t c0, const Panel *pan,
+                      const int32_t *active, Mod md, const Digits &dg) {
+#define GBGPU_UPD(ND, KC) k_update_mfma<ND, KC><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg)
+    if (kc == 4) {
+        switch (nd) {
+            case 1: GBGPU_UPD(1, 4); break;
+            case 2: GBGPU_UPD(2, 4); break;
+            case 3: GBGPU_UPD(3, 4); break;
+            default: GBGPU_UPD(4, 4); break;
+        }
+    } else {
+        switch (nd) {
+            case 1: GBGPU_UPD(1, 1); break;
+            case 2: GBGPU_UPD(2, 1); break;
+            case 3: GBGPU_UPD(3, 1); break;
+            default: GBGPU_UPD(4, 1); break;
+        }
+    }
+#undef GBGPU_UPD
 }
 
 // ------------------------------------------------------------ output
@@ -926,7 +942,7 @@ __global__ __launch_bounds__(256) void k_batch_update_mfma(const DenseDesc *D, c
     const DenseDesc &d = D[i];
     const int32_t l = blockIdx.x - pre[i];
     const int32_t ct = (d.n - c0 + 63) / 64;
-    update_i8_body<ND>(d.a, d.ld, d.m, c0, d.n, c0, d.pan, d.active, d.md, d.dg, l % ct, l / ct);
+    update_i8_body<ND, 1>(d.a, d.ld, d.m, c0, d.n, c0, d.pan, d.active, d.md, d.dg, l % ct, l / ct);
 }
 
 template <bool BIG>

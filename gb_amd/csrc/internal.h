@@ -1,6 +1,8 @@
 // internal.h -- device-resident objects behind the opaque C-ABI handles.
 #pragma once
 
+#include <climits>
+
 #include "common.h"
 
 // One F4 matrix on the device plus everything derived from it.
@@ -117,6 +119,8 @@ int dense_rref(gbgpu_dense *d, int form);
 // eliminated block (decreasing lead order)
 int dense_output(gbgpu_dense *d, const int32_t *colpiv);
 
+struct Panel;
+
 // Signed base-256 digits for the int8 matrix cores.  With nd digits in
 // [-128, 127] the integers 127 S .. -128 S are reachable (S = 1 + 256 + ...
 // + 256^(nd-1)), a window of 256^nd consecutive integers; so for
@@ -147,12 +151,35 @@ __host__ inline void digits_setup(Digits &g, uint32_t p, int nd) {
     g.hi = (uint32_t)(127 * S);
     uint64_t w = 1 % p;
     for (int s_ = 0; s_ < 7; ++s_, w = (w * 256) % p) g.w[s_] = (uint32_t)w;
-    // |sum_s C_s w_s| < (2 nd - 1) nd 64 128^2 p < 2^56
-    g.off = ((1ull << 56) / p + 1) * p;
+    // |sum_s C_s w_s| < (2 nd - 1) nd 256 128^2 p < 2^58 (up to 256 pivot
+    // slots per product)
+    g.off = ((1ull << 59) / p + 1) * p;
 }
 
+__device__ __forceinline__ uint32_t digit_word(uint32_t x, uint32_t p, uint32_t hi, int i) {
+    // digit i of x's representative (as an unsigned byte)
+    int64_t v = x <= hi ? (int64_t)x : (int64_t)x - (int64_t)p;
+    int8_t d = 0;
+    for (int k = 0; k <= i; ++k) {
+        d = (int8_t)(uint8_t)(v & 0xFF);
+        v = (v - d) >> 8;
+    }
+    return (uint8_t)d;
+}
+
+// the int8-digit MFMA update A[r][cb, ce) -= L[r] Rp (dense.hip) over kc
+// chunks of 64 pivot slots (kc = 1 or 4): L and Rp^T as digit planes of dg
+// with 64 kc bytes per row / column (Rp^T indexed from column c0), rows
+// with active[r]
+void launch_update_i8(int nd, int kc, dim3 grid, hipStream_t s, uint32_t *a, int64_t ld,
+                      int32_t m, int32_t cb, int32_t ce, int32_t c0, const Panel *pan,
+                      const int32_t *active, Mod md, const Digits &dg);
+
+// dsparse.hip: the sparse pass as blocked elimination on the matrix cores
+bool dense_route_pays(const gbgpu_mat *m);
+int run_dense_route(gbgpu_mat *m, const uint32_t *tinv);
+
 // dense.hip, batches: one dense block of a batch and its elimination state
-struct Panel;
 struct ListState;
 struct DenseDesc {
     uint32_t *a;

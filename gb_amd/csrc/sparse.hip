@@ -831,11 +831,13 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
     if ((rc = dalloc(ctx, &m->dprime, (size_t)m->nrl * (size_t)ncr)) ||
         (rc = dalloc(ctx, &m->dnz, (size_t)m->nrl)))
         return rc;
-    // the blocked form needs the inverses of the groups' pivot blocks
+    // the blocked form needs the inverses of the groups' pivot blocks; on
+    // matrices with dense enough pivot rows it runs on the matrix cores
     const bool blocked = form == GBGPU_FORM_MFMA && work == nullptr;
+    const bool dense = blocked && dense_route_pays(m);
     uint32_t *tinv = nullptr;
     const int32_t ngroups = (m->ncl + 63) / 64;
-    if (blocked && (rc = prep_group_buckets(m))) return rc;
+    if (blocked && !dense && (rc = prep_group_buckets(m))) return rc;
     if (blocked && ngroups > 0) {
         if ((rc = dalloc(ctx, &tinv, (size_t)ngroups * 4096))) return rc;
         k_group_inverse<<<ngroups, 64, 0, s>>>(m->ncl, m->wshift, m->pbits, m->seg, m->bucket,
@@ -861,7 +863,9 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
     a.md = Mod::make(m->p);
     const bool big = m->p >= (1u << 17);
     GBGPU_HIP(ctx, hipEventRecord(ctx->ev[0], s));
-    if (m->nrl > 0 && blocked) {
+    if (m->nrl > 0 && dense) {
+        if ((rc = run_dense_route(m, tinv))) { dfree(xs); dfree(queue); dfree(tinv); return rc; }
+    } else if (m->nrl > 0 && blocked) {
         switch (m->wshift) {
             case 11: launch_blocked<11>(a, tinv, m->gseg, m->gbucket, ngroups, big, grid, s); break;
             case 12: launch_blocked<12>(a, tinv, m->gseg, m->gbucket, ngroups, big, grid, s); break;

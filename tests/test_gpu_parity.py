@@ -45,6 +45,26 @@ def test_sparse_pass_many_windows(gpu_lib, p, form):
     assert np.array_equal(got, exp)
 
 
+@pytest.mark.parametrize("p", [251, 65521, 131071, 2147483647])
+@pytest.mark.parametrize("shape", [(300, 420, 200), (1500, 2600, 900)])
+def test_sparse_pass_dense_route(gpu_lib, p, shape):
+    """Pivot rows carrying ~10 % of the columns: the MFMA form reduces on
+    the matrix cores (blocked elimination on a dense copy); same D' rows,
+    same order, as the oracle, and the same la_option 1 result."""
+    nr, nc, npiv = shape
+    m = _f4(nr, nc, npiv, p, seed=7, piv_len=(nc // 30, nc // 5), piv_mu=np.log(nc / 10),
+            red_nnz=nc // 10)
+    d = gpu_lib.upload(m)
+    nd = d.sparse_pass(FORM_MFMA)
+    got, src = d.sparse_pass_result(nd)
+    exp, esrc = oracle.sparse_pass(m, nthreads=8)
+    assert np.array_equal(src, esrc)
+    assert np.array_equal(got, exp)
+    exp1 = oracle.la_sparse_dense(m, nthreads=8)
+    assert d.la_sparse_dense(FORM_MFMA) == exp1.np
+    assert d.rows().equals(exp1)
+
+
 @pytest.mark.parametrize("p", PRIMES)
 @pytest.mark.parametrize("form", FORMS)
 def test_la_option1_medium(gpu_lib, p, form):
