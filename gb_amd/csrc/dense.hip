@@ -570,7 +570,28 @@ __global__ __launch_bounds__(256) void k_update_plain(uint32_t *a, int64_t ld, i
 // of the digit planes, which stay in L2 across the grid.
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-template <int ND, int KC>
+// combination of the digit products of one element:
+// sum_s C_s 256^s mod p.  For nd <= 3 the shifted sum fits in 57 bits; for
+// nd = 4 the terms s >= 4 are reduced first and enter as (.. mod p)
+// (2^32 mod p).  |C_s| < nd 256 128^2 <= 2^24.
+template <int ND>
+__device__ __forceinline__ uint32_t digits_combine(const v4i *acc, int e, const Mod &md,
+                                                   const Digits &dg) {
+    int64_t lo = 0;
+#pragma unroll
+    for (int q = 0; q < (2 * ND - 1 < 4 ? 2 * ND - 1 : 4); ++q) lo += (int64_t)acc[q][e] << (8 * q);
+    if constexpr (ND == 3) lo += (int64_t)acc[4][e] << 32;
+    if constexpr (ND == 4) {
+        int64_t hi = 0;
+#pragma unroll
+        for (int q = 4; q < 2 * ND - 1; ++q) hi += (int64_t)acc[q][e] << (8 * (q - 4));
+        const uint32_t hr = md.reduce((uint64_t)(hi + (int64_t)dg.off2));
+        lo += (int64_t)md.mul(hr, dg.w32);
+    }
+    return md.reduce((uint64_t)(lo + (int64_t)dg.off));
+}
+
+template <int ND, int KC, int RS = 2, int CS = 2>
 __device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t m, int32_t cb,
                                                int32_t ce, int32_t c0, const Panel *pan,
                                                const int32_t *active, Mod md, const Digits &dg,
@@ -578,91 +599,125 @@ __device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t 
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int32_t r0 = by * 64 + (wv >> 1) * 32;
-    const int32_t q0 = cb + bx * 64 + (wv & 1) * 32;
+    // the block's tile is 32 RS x 32 CS, each of its four waves 16 RS x 16 CS
+    const int32_t r0 = by * (32 * RS) + (wv >> 1) * (16 * RS);
+    const int32_t q0 = cb + bx * (32 * CS) + (wv & 1) * (16 * CS);
     // rows of this wave with a nonzero multiplier (wave-uniform exit)
-    const bool act = lane < 32 && r0 + lane < m && active[r0 + lane] != 0;
+    const bool act = lane < 16 * RS && r0 + lane < m && active[r0 + lane] != 0;
     if (!__any(act) || q0 >= ce) return;
     const int lr = lane & 15, lk = lane >> 4;
-    v4i acc[2][2][2 * ND - 1];
+    // this lane's values of A (rows r0 + 16 rs + 4 (l / 16) + e, columns
+    // q0 + 16 cs + l % 16), loaded before the products so that their
+    // latency hides behind them
+    uint32_t xv[RS][4][CS];
+    bool xa[RS][4];
 #pragma unroll
-    for (int rs = 0; rs < 2; ++rs)
+    for (int rs = 0; rs < RS; ++rs)
 #pragma unroll
-        for (int cs = 0; cs < 2; ++cs)
+        for (int e = 0; e < 4; ++e) {
+            const int32_t row = r0 + 16 * rs + 4 * lk + e;
+            xa[rs][e] = row < m && active[row] != 0;
+#pragma unroll
+            for (int cs = 0; cs < CS; ++cs) {
+                const int32_t col = q0 + 16 * cs + lr;
+                xv[rs][e][cs] = (xa[rs][e] && col < ce) ? a[(int64_t)row * ld + col] : 0u;
+            }
+        }
+    v4i acc[RS][CS][2 * ND - 1];
+#pragma unroll
+    for (int rs = 0; rs < RS; ++rs)
+#pragma unroll
+        for (int cs = 0; cs < CS; ++cs)
 #pragma unroll
             for (int q = 0; q < 2 * ND - 1; ++q) acc[rs][cs][q] = v4i{0, 0, 0, 0};
-    // KC chunks of 64 pivot slots; the digit planes hold 64 KC bytes per row
+    // KC chunks of 64 pivot slots from chunk k0; the digit planes hold
+    // dg.stride bytes per row (L) and per column (R)
+#pragma unroll 1
     for (int kc = 0; kc < KC; ++kc) {
-        v4i A[ND][2], B[ND][2];
+        const int off = 64 * (dg.k0 + kc) + 16 * lk;
+        v4i A[ND][RS], B[ND][CS];
 #pragma unroll
-        for (int i = 0; i < ND; ++i)
+        for (int i = 0; i < ND; ++i) {
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < RS; ++h) {
                 const int32_t row = r0 + 16 * h + lr;
-                const int32_t col = q0 + 16 * h + lr;
-                A[i][h] = row < m ? *(const v4i *)(dg.L + i * dg.lplane + (int64_t)row * (64 * KC) +
-                                                   64 * kc + 16 * lk)
+                A[i][h] = row < m ? *(const v4i *)(dg.L + i * dg.lplane + (int64_t)row * dg.stride + off)
                                   : v4i{0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int h = 0; h < CS; ++h) {
+                const int32_t col = q0 + 16 * h + lr;
                 B[i][h] = col < ce ? *(const v4i *)(dg.R + i * dg.rplane +
-                                                    (int64_t)(col - c0) * (64 * KC) + 64 * kc + 16 * lk)
+                                                    (int64_t)(col - c0) * dg.stride + off)
                                    : v4i{0, 0, 0, 0};
             }
+        }
 #pragma unroll
         for (int i = 0; i < ND; ++i)
 #pragma unroll
             for (int j = 0; j < ND; ++j)
 #pragma unroll
-                for (int rs = 0; rs < 2; ++rs)
+                for (int rs = 0; rs < RS; ++rs)
 #pragma unroll
-                    for (int cs = 0; cs < 2; ++cs)
+                    for (int cs = 0; cs < CS; ++cs)
                         acc[rs][cs][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
                             A[i][rs], B[j][cs], acc[rs][cs][i + j], 0, 0, 0);
     }
 #pragma unroll
-    for (int rs = 0; rs < 2; ++rs)
+    for (int rs = 0; rs < RS; ++rs)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
+            if (!xa[rs][e]) continue;
             const int32_t row = r0 + 16 * rs + 4 * lk + e;
-            if (row >= m || !active[row]) continue;
 #pragma unroll
-            for (int cs = 0; cs < 2; ++cs) {
+            for (int cs = 0; cs < CS; ++cs) {
                 const int32_t col = q0 + 16 * cs + lr;
                 if (col >= ce) continue;
-                int64_t v = 0;
-#pragma unroll
-                for (int q = 0; q < 2 * ND - 1; ++q) v += (int64_t)acc[rs][cs][q][e] * dg.w[q];
-                const uint32_t u = md.reduce((uint64_t)(v + (int64_t)dg.off));
-                uint32_t *x = a + (int64_t)row * ld + col;
-                *x = sub_mod(*x, u, md.p);
+                const uint32_t u = digits_combine<ND>(acc[rs][cs], e, md, dg);
+                a[(int64_t)row * ld + col] = sub_mod(xv[rs][e][cs], u, md.p);
             }
         }
 }
 
-template <int ND, int KC = 1>
+template <int ND, int KC = 1, int RS = 2, int CS = 2>
 __global__ __launch_bounds__(256) void k_update_mfma(uint32_t *a, int64_t ld, int32_t m,
                                                      int32_t cb, int32_t ce, int32_t c0,
                                                      const Panel *pan, const int32_t *active,
                                                      Mod md, Digits dg) {
-    update_i8_body<ND, KC>(a, ld, m, cb, ce, c0, pan, active, md, dg, blockIdx.x, blockIdx.y);
+    update_i8_body<ND, KC, RS, CS>(a, ld, m, cb, ce, c0, pan, active, md, dg, blockIdx.x,
+                                   blockIdx.y);
 }
 
-void launch_update_i8(int nd, int kc, dim3 grid, hipStream_t s, uint32_t *a, int64_t ld,
-                      int32_t m, int32_t cb, int32_t ce, int32_t c0, const Panel *pan,
+// block tile rows x columns (32 RS x 32 CS)
+static void update_i8_tile(int nd, int kc, int *rows, int *cols) {
+    *rows = 64;
+    (void)nd;
+    (void)kc;
+    *cols = 64;
+}
+
+void launch_update_i8(int nd, int kc, hipStream_t s, uint32_t *a, int64_t ld, int32_t m,
+                      int32_t cb, int32_t ce, int32_t c0, const Panel *pan,
                       const int32_t *active, Mod md, const Digits &dg) {
-#define GBGPU_UPD(ND, KC) k_update_mfma<ND, KC><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg)
+    if (cb >= ce || m <= 0) return;
+    int tr, tc;
+    update_i8_tile(nd, kc, &tr, &tc);
+    dim3 grid((unsigned)((ce - cb + tc - 1) / tc), (unsigned)((m + tr - 1) / tr));
+#define GBGPU_UPD(ND, KC, R, C) \
+    k_update_mfma<ND, KC, R, C><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg)
     if (kc == 4) {
         switch (nd) {
-            case 1: GBGPU_UPD(1, 4); break;
-            case 2: GBGPU_UPD(2, 4); break;
-            case 3: GBGPU_UPD(3, 4); break;
-            default: GBGPU_UPD(4, 4); break;
+            case 1: GBGPU_UPD(1, 4, 2, 2); break;
+            case 2: GBGPU_UPD(2, 4, 2, 2); break;
+            case 3: GBGPU_UPD(3, 4, 2, 2); break;
+            default: GBGPU_UPD(4, 4, 2, 2); break;
         }
     } else {
         switch (nd) {
-            case 1: GBGPU_UPD(1, 1); break;
-            case 2: GBGPU_UPD(2, 1); break;
-            case 3: GBGPU_UPD(3, 1); break;
-            default: GBGPU_UPD(4, 1); break;
+            case 1: GBGPU_UPD(1, 1, 2, 2); break;
+            case 2: GBGPU_UPD(2, 1, 2, 2); break;
+            case 3: GBGPU_UPD(3, 1, 2, 2); break;
+            default: GBGPU_UPD(4, 1, 2, 2); break;
         }
     }
 #undef GBGPU_UPD
@@ -942,7 +997,7 @@ __global__ __launch_bounds__(256) void k_batch_update_mfma(const DenseDesc *D, c
     const DenseDesc &d = D[i];
     const int32_t l = blockIdx.x - pre[i];
     const int32_t ct = (d.n - c0 + 63) / 64;
-    update_i8_body<ND, 1>(d.a, d.ld, d.m, c0, d.n, c0, d.pan, d.active, d.md, d.dg, l % ct, l / ct);
+    update_i8_body<ND, 1, 2, 2>(d.a, d.ld, d.m, c0, d.n, c0, d.pan, d.active, d.md, d.dg, l % ct, l / ct);
 }
 
 template <bool BIG>

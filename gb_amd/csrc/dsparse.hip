@@ -11,14 +11,14 @@
 //
 //   P     = the panel's pivot rows, dense from the panel on
 //   Rp_a  = T_a^-1 P_a per group a (T_a the group's unit upper-triangular
-//           block, the inverses of the blocked pass), then block back
-//           substitution Rp_a -= Rp_a[:, group b] Rp_b (b > a), so that Rp
-//           is the identity on all the panel's pivot columns
-//   M    -= M[:, panel] Rp      over the columns right of the panel
+//           block, the inverses of the blocked pass): identity on the
+//           group's pivot columns
+//   for a = 0..3:  L_a = M[:, group a] (current values)
+//                  M[:, later groups of the panel] -= L_a Rp_a   (K = 64)
+//   M[:, right of the panel] -= [L_0 .. L_3] [Rp_0; ..; Rp_3]    (K = 256)
 //
-// The update is the int8-digit MFMA kernel of the dense echelon form with
-// K = 256 (four chunks of 64 pivot slots), so that each pass over M covers
-// four groups.
+// The updates are the int8-digit MFMA kernel of the dense echelon form; the
+// wide one covers four groups per pass over M.
 // Columns without a pivot get identity rows in T_g and zero rows in P_g,
 // so a row's entry there is left alone, as the reference skips them
 // (la_ff_32.c:517-519); only the right-hand part is kept.
@@ -96,50 +96,8 @@ __global__ __launch_bounds__(256) void k_ds_rp(int32_t g0, int32_t n, const uint
     }
 }
 
-// block back substitution inside a panel: X[t][c] -= sum_k C[t][k] Y[k][c]
-// for the 64 rows t of one group, K <= 192 rows k of the later groups
-// (already final), over the columns c < w
-constexpr int KB = 192;
-__global__ __launch_bounds__(256) void k_ds_backsub(uint32_t *X, int64_t ldr, const uint32_t *C,
-                                                    int K, const uint32_t *Y, int32_t w, Mod md,
-                                                    uint32_t r64) {
-    __shared__ uint32_t Cs[GB][KB + 1];
-    __shared__ uint32_t Ys[KB][GB + 1];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int32_t jb = blockIdx.x * 64;
-    for (int i = tid; i < GB * K; i += 256) Cs[i / K][i % K] = C[i];
-    for (int i = tid; i < K * 64; i += 256) {
-        const int k = i >> 6, c = i & 63;
-        Ys[k][c] = (jb + c < w) ? Y[(int64_t)k * ldr + jb + c] : 0u;
-    }
-    __syncthreads();
-    const int32_t j = jb + lane;
-    if (j >= w) return;
-    for (int q = 0; q < 16; ++q) {
-        const int t = 16 * wv + q;
-        uint64_t lo = 0;
-        uint32_t hi = 0;
-        for (int k = 0; k < K; ++k) {
-            const uint64_t pr = (uint64_t)Cs[t][k] * (uint64_t)Ys[k][lane];
-            const uint64_t o = lo;
-            lo += pr;
-            hi += lo < o;
-        }
-        uint32_t r = md.reduce(lo);
-        if (hi) r = (uint32_t)((r + (uint64_t)md.mul(hi, r64)) % md.p);
-        uint32_t *x = X + (int64_t)t * ldr + j;
-        *x = *x >= r ? *x - r : *x + md.p - r;
-    }
-}
-
-// C[t][k] = X[t][c0 + k] for the 64 rows of one group (the coefficients of
-// the back substitution, copied before X changes)
-__global__ void k_ds_coef(const uint32_t *X, int64_t ldr, int32_t c0, int K, uint32_t *C) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < GB * K; i += gridDim.x * blockDim.x)
-        C[i] = X[(int64_t)(i / K) * ldr + c0 + i % K];
-}
-
-// digit planes of Rp^T: column c holds 64 kc bytes per digit, byte t = row t
+// digit planes of Rp^T: column c holds dg.stride bytes per digit, byte t =
+// row t (zero from row `rows` on)
 __global__ void k_ds_rpdigits(const uint32_t *Rp, int64_t ldr, int rows, int32_t w, int kc,
                               uint32_t p, Digits dg) {
     const int lane = threadIdx.x & 63;
@@ -148,26 +106,24 @@ __global__ void k_ds_rpdigits(const uint32_t *Rp, int64_t ldr, int rows, int32_t
         for (int t = lane; t < 64 * kc; t += 64) {
             const uint32_t v = t < rows ? Rp[(int64_t)t * ldr + c] : 0u;
             for (int i = 0; i < dg.nd; ++i)
-                dg.R[i * dg.rplane + (int64_t)c * (64 * kc) + t] = (int8_t)digit_word(v, p, dg.hi, i);
+                dg.R[i * dg.rplane + (int64_t)c * dg.stride + t] = (int8_t)digit_word(v, p, dg.hi, i);
         }
     }
 }
 
-// L = M[q][g0 .. g0 + bw) as digit planes of 64 kc bytes per row;
-// active[q] = any nonzero
-__global__ void k_ds_gather(int32_t nrl, int32_t g0, int32_t bw, int kc, const uint32_t *M,
-                            int64_t ldm, int32_t *active, uint32_t p, Digits dg) {
+// chunk a of L: M[q][c0 .. c0 + bw) (bw <= 64) as digits at bytes 64 a ..
+// 64 a + 63 of row q; act[q] = any nonzero, any[q] = OR over the chunks
+__global__ void k_ds_gather(int32_t nrl, int32_t c0, int32_t bw, int a, const uint32_t *M,
+                            int64_t ldm, int32_t *act, int32_t *any, uint32_t p, Digits dg) {
     const int lane = threadIdx.x & 63;
     const int32_t step = gridDim.x * (blockDim.x >> 6);
     for (int32_t q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < nrl; q += step) {
-        bool nz = false;
-        for (int t = lane; t < 64 * kc; t += 64) {
-            const uint32_t v = t < bw ? M[(int64_t)q * ldm + g0 + t] : 0u;
-            nz = nz || v != 0u;
-            for (int i = 0; i < dg.nd; ++i)
-                dg.L[i * dg.lplane + (int64_t)q * (64 * kc) + t] = (int8_t)digit_word(v, p, dg.hi, i);
-        }
-        active[q] = __any(nz) ? 1 : 0;  // every lane stores the same flag
+        const uint32_t v = lane < bw ? M[(int64_t)q * ldm + c0 + lane] : 0u;
+        for (int i = 0; i < dg.nd; ++i)
+            dg.L[i * dg.lplane + (int64_t)q * dg.stride + 64 * a + lane] = (int8_t)digit_word(v, p, dg.hi, i);
+        const int nz = __any(v != 0u) ? 1 : 0;
+        act[q] = nz;  // every lane stores the same flags
+        any[q] = a == 0 ? nz : (any[q] | nz);
     }
 }
 
@@ -218,15 +174,15 @@ int run_dense_route(gbgpu_mat *m, const uint32_t *tinv) {
     const uint32_t r64 = (uint32_t)((((unsigned __int128)1) << 64) % m->p);
     Digits dg;
     digits_setup(dg, m->p, digits_for(m->p));
-    uint32_t *M = nullptr, *P = nullptr, *Rp = nullptr, *C = nullptr;
+    uint32_t *M = nullptr, *P = nullptr, *Rp = nullptr;
     int8_t *dl = nullptr, *dr = nullptr;
-    int32_t *active = nullptr, *pk = nullptr;  // pk: a panel record whose k is 64
+    int32_t *active = nullptr, *act = nullptr, *pk = nullptr;  // pk: a panel record whose k is 64
     int rc;
     auto cleanup = [&]() {
-        dfree(M); dfree(P); dfree(Rp); dfree(C); dfree(dl); dfree(dr); dfree(active); dfree(pk);
+        dfree(M); dfree(P); dfree(Rp); dfree(dl); dfree(dr); dfree(active); dfree(act); dfree(pk);
     };
     if ((rc = dalloc(ctx, &M, (size_t)nrl * ldm)) || (rc = dalloc(ctx, &P, (size_t)PW * ldp)) ||
-        (rc = dalloc(ctx, &Rp, (size_t)PW * ldr)) || (rc = dalloc(ctx, &C, (size_t)GB * KB)) ||
+        (rc = dalloc(ctx, &Rp, (size_t)PW * ldr)) || (rc = dalloc(ctx, &act, (size_t)nrl)) ||
         (rc = dalloc(ctx, &dl, (size_t)dg.nd * nrl * PW)) ||
         (rc = dalloc(ctx, &dr, (size_t)dg.nd * ldr * PW)) || (rc = dalloc(ctx, &active, (size_t)nrl)) ||
         (rc = dalloc(ctx, &pk, (dense_panel_bytes() + 3) / 4))) {
@@ -237,6 +193,7 @@ int run_dense_route(gbgpu_mat *m, const uint32_t *tinv) {
     dg.R = dr;
     dg.lplane = (int64_t)nrl * PW;
     dg.rplane = ldr * PW;
+    dg.stride = PW;
     const int32_t k64 = GB;
     GBGPU_HIP(ctx, hipMemcpyAsync(pk, &k64, sizeof(int32_t), hipMemcpyHostToDevice, s));
     GBGPU_HIP(ctx, hipMemsetAsync(M, 0, (size_t)nrl * ldm * sizeof(uint32_t), s));
@@ -251,27 +208,30 @@ int run_dense_route(gbgpu_mat *m, const uint32_t *tinv) {
         GBGPU_HIP(ctx, hipMemsetAsync(P, 0, (size_t)PW * ldp * sizeof(uint32_t), s));
         k_ds_group_rows<<<PW / 4, 256, 0, s>>>(g0, ncl, PW, m->first, m->rp, m->ci, m->rcf, m->cp,
                                                m->cf, P, ldp);
-        // each group's rows made unit on its own pivot columns ...
+        // each group's rows made unit on its own pivot columns
         for (int a = 0; a < na; ++a)
             k_ds_rp<<<nb, 256, 0, s>>>(g0, nc, tinv + (int64_t)(g0 / GB + a) * 4096,
                                        P + (int64_t)a * GB * ldp, ldp, Rp + (int64_t)a * GB * ldr, ldr,
                                        md, r64);
-        // ... then zero on the later groups' pivot columns (back substitution)
-        for (int a = na - 2; a >= 0; --a) {
-            const int K = GB * (na - 1 - a);
-            uint32_t *X = Rp + (int64_t)a * GB * ldr;
-            k_ds_coef<<<(GB * K + 255) / 256, 256, 0, s>>>(X, ldr, (a + 1) * GB, K, C);
-            k_ds_backsub<<<nb, 256, 0, s>>>(X, ldr, C, K, Rp + (int64_t)(a + 1) * GB * ldr, w, md, r64);
-        }
         k_ds_rpdigits<<<grid_rows(w, 4), 256, 0, s>>>(Rp, ldr, na * GB, w, KCH, m->p, dg);
-        k_ds_gather<<<grid_rows(nrl, 4), 256, 0, s>>>(nrl, g0, bw, KCH, M, ldm, active, m->p, dg);
-        // columns right of the panel's pivot columns (the last panel may be
-        // partial: its right-hand columns are updated too)
-        const int32_t cb = g0 + bw, ce = nc;
-        if (cb < ce) {
-            dim3 gr((unsigned)((ce - cb + 63) / 64), (unsigned)((nrl + 63) / 64));
-            launch_update_i8(dg.nd, KCH, gr, s, M, ldm, nrl, cb, ce, g0, pan, active, md, dg);
+        // the panel's groups in order: the multipliers of group a are M's
+        // current values on its columns; its rows update the panel's later
+        // columns at once (K = 64) ...
+        for (int a = 0; a < na; ++a) {
+            const int32_t ca = g0 + a * GB;
+            const int32_t wa = (bw - a * GB) < GB ? (bw - a * GB) : GB;
+            k_ds_gather<<<grid_rows(nrl, 4), 256, 0, s>>>(nrl, ca, wa, a, M, ldm, act, active, m->p, dg);
+            const int32_t nb0 = ca + GB, nb1 = g0 + bw;
+            if (nb0 < nb1) {
+                Digits d1 = dg;
+                d1.k0 = a;
+                launch_update_i8(dg.nd, 1, s, M, ldm, nrl, nb0, nb1, g0, pan, act, md, d1);
+            }
         }
+        // ... and the columns right of the panel all four together (K = 256;
+        // the last panel may be partial: its right-hand columns are updated too)
+        const int32_t cb = g0 + bw, ce = nc;
+        launch_update_i8(dg.nd, KCH, s, M, ldm, nrl, cb, ce, g0, pan, active, md, dg);
         GBGPU_HIP(ctx, hipGetLastError());
     }
     k_ds_extract<<<grid_rows(nrl, 4), 256, 0, s>>>(nrl, ncl, nc, M, ldm, m->dprime, m->dnz);

@@ -136,8 +136,11 @@ struct Digits {
     int64_t rplane = 0;    // bytes per R plane
     int nd = 0;            // 0: no digit planes (plain form)
     uint32_t hi = 0;       // 127 S
-    uint64_t off = 0;      // multiple of p above any combined product
-    uint32_t w[7] = {};    // 256^s mod p
+    uint64_t off = 0;      // multiples of p above the combined products
+    uint64_t off2 = 0;
+    uint32_t w32 = 0;      // 2^32 mod p
+    int32_t stride = 64;   // bytes per row (L) / column (R) of a plane
+    int32_t k0 = 0;        // first 64-byte chunk an update reads
 };
 
 __host__ inline int digits_for(uint32_t p) {
@@ -149,11 +152,12 @@ __host__ inline void digits_setup(Digits &g, uint32_t p, int nd) {
     uint64_t S = 0, pw = 1;
     for (int i = 0; i < nd; ++i, pw *= 256) S += pw;
     g.hi = (uint32_t)(127 * S);
-    uint64_t w = 1 % p;
-    for (int s_ = 0; s_ < 7; ++s_, w = (w * 256) % p) g.w[s_] = (uint32_t)w;
-    // |sum_s C_s w_s| < (2 nd - 1) nd 256 128^2 p < 2^58 (up to 256 pivot
-    // slots per product)
-    g.off = ((1ull << 59) / p + 1) * p;
+    // the digit products C_s (|C_s| < 2^24, up to 256 pivot slots) combine
+    // as sum_s C_s 2^(8 s): below 2^57 for nd <= 3; for nd = 4 the terms
+    // s >= 4 (below 2^41) are reduced first and scaled by 2^32 mod p
+    g.w32 = (uint32_t)((1ull << 32) % p);
+    g.off = ((1ull << 57) / p + 1) * p;
+    g.off2 = ((1ull << 41) / p + 1) * p;
 }
 
 __device__ __forceinline__ uint32_t digit_word(uint32_t x, uint32_t p, uint32_t hi, int i) {
@@ -168,11 +172,11 @@ __device__ __forceinline__ uint32_t digit_word(uint32_t x, uint32_t p, uint32_t 
 }
 
 // the int8-digit MFMA update A[r][cb, ce) -= L[r] Rp (dense.hip) over kc
-// chunks of 64 pivot slots (kc = 1 or 4): L and Rp^T as digit planes of dg
-// with 64 kc bytes per row / column (Rp^T indexed from column c0), rows
-// with active[r]
-void launch_update_i8(int nd, int kc, dim3 grid, hipStream_t s, uint32_t *a, int64_t ld,
-                      int32_t m, int32_t cb, int32_t ce, int32_t c0, const Panel *pan,
+// chunks of 64 pivot slots (kc = 1 or 4) from chunk dg.k0: L and Rp^T as
+// digit planes of dg with dg.stride bytes per row / column (Rp^T indexed
+// from column c0), rows with active[r]
+void launch_update_i8(int nd, int kc, hipStream_t s, uint32_t *a, int64_t ld, int32_t m,
+                      int32_t cb, int32_t ce, int32_t c0, const Panel *pan,
                       const int32_t *active, Mod md, const Digits &dg);
 
 // dsparse.hip: the sparse pass as blocked elimination on the matrix cores
