@@ -557,40 +557,63 @@ __global__ __launch_bounds__(256) void k_update_plain(uint32_t *a, int64_t ld, i
 // ------------------------------------------------- update (MFMA int8)
 
 // Block: 4 independent waves, 64 rows x 64 columns; each wave a 32 x 32
-// tile of 2 x 2 products v_mfma_i32_16x16x64_i8 (A 16 x 64, B 64 x 16,
-// C 16 x 16; K = the panel's 64 pivot slots).  With L = sum_i 256^i Li and
-// Rp = sum_j 256^j Rj in signed digits, L Rp = sum_s 256^s Cs with
-// Cs = sum_{i+j=s} Li Rj: nd^2 products into 2 nd - 1 int32 accumulators,
-// each sum below nd 64 128^2 = nd 2^20 and so exact.  The combination
-// sum_s Cs (256^s mod p) is formed in 64-bit integers and reduced once.
-// Operand layout per lane l: A row l % 16, bytes 16 (l / 16) .. + 15 of the
-// 64 slots; B column l % 16, the same slots; result register e of lane l
-// is C[4 (l / 16) + e][l % 16] (measured on gfx950 with an in-bounds probe
-// of the builtin).  No LDS and no barrier: the operands are 16-byte loads
-// of the digit planes, which stay in L2 across the grid.
+// Products v_mfma_i32_16x16x64_i8 (A 16 x 64, B 64 x 16, C 16 x 16; K =
+// 64 pivot slots).  With L = sum_i 256^i Li and Rp = sum_j 256^j Rj in
+// signed digits, L Rp = sum_s 256^s Cs with Cs = sum_{i+j=s} Li Rj: nd^2
+// products into 2 nd - 1 int32 accumulators, each below
+// min(s + 1, 2 nd - 1 - s) 256 128^2 <= 2^24 over up to 256 slots and so
+// exact.  Operand layout per lane l: A row l % 16, bytes 16 (l / 16) .. + 15
+// of the 64 slots; B column l % 16, the same slots; result register e of
+// lane l is C[4 (l / 16) + e][l % 16] (measured on gfx950 with an in-bounds
+// probe of the builtin).
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-// combination of the digit products of one element:
-// sum_s C_s 256^s mod p.  For nd <= 3 the shifted sum fits in 57 bits; for
-// nd = 4 the terms s >= 4 are reduced first and enter as (.. mod p)
-// (2^32 mod p).  |C_s| < nd 256 128^2 <= 2^24.
+// x - sum_s Cs 256^s as a double that is exact and congruent to it modulo
+// p.  Magnitudes (256 slots): nd = 2 below 2^39; nd = 3 below 2^48, with C4
+// entering as C4 (2^32 mod p); nd = 4: C0..C3 below 2^49, and
+// hi = C4 + 2^8 C5 + 2^16 C6 (below 2^39) enters as h = hi mod p split in
+// 16-bit halves, h0 (2^32 mod p) + h1 (2^48 mod p), each below 2^47.  Every
+// partial sum is an integer below 2^53, so each step is exact.
 template <int ND>
-__device__ __forceinline__ uint32_t digits_combine(const v4i *acc, int e, const Mod &md,
-                                                   const Digits &dg) {
-    int64_t lo = 0;
-#pragma unroll
-    for (int q = 0; q < (2 * ND - 1 < 4 ? 2 * ND - 1 : 4); ++q) lo += (int64_t)acc[q][e] << (8 * q);
-    if constexpr (ND == 3) lo += (int64_t)acc[4][e] << 32;
-    if constexpr (ND == 4) {
-        int64_t hi = 0;
-#pragma unroll
-        for (int q = 4; q < 2 * ND - 1; ++q) hi += (int64_t)acc[q][e] << (8 * (q - 4));
-        const uint32_t hr = md.reduce((uint64_t)(hi + (int64_t)dg.off2));
-        lo += (int64_t)md.mul(hr, dg.w32);
+__device__ __forceinline__ double digits_residual(uint32_t x, const v4i *acc, int e,
+                                                  const Digits &dg) {
+    double d = (double)x - (double)acc[0][e];
+    if constexpr (ND >= 2) {
+        d = fma(-256.0, (double)acc[1][e], d);
+        d = fma(-65536.0, (double)acc[2][e], d);
     }
-    return md.reduce((uint64_t)(lo + (int64_t)dg.off));
+    if constexpr (ND == 3) {
+        d = fma(-16777216.0, (double)acc[3][e], d);
+        d = fma(-dg.w32, (double)acc[4][e], d);
+    }
+    if constexpr (ND == 4) {
+        d = fma(-16777216.0, (double)acc[3][e], d);
+        double hi = (double)acc[4][e];
+        hi = fma(256.0, (double)acc[5][e], hi);
+        hi = fma(65536.0, (double)acc[6][e], hi);
+        double h = fma(-rint(hi * dg.pinv), dg.pd, hi);
+        h = h < 0.0 ? h + dg.pd : h;
+        const double h1 = floor(h * (1.0 / 65536.0));
+        const double h0 = fma(-65536.0, h1, h);
+        d = fma(-dg.w32, h0, d);
+        d = fma(-dg.w48, h1, d);
+    }
+    return d;
 }
 
+// d mod p for an integer-valued double |d| < 2^53: q = rint(d / p) is
+// within one of the nearest quotient (the product d (1 / p) is off by less
+// than 1 / (2 p)), so d - q p lies in [-(p + 1) / 2, (p + 1) / 2] and one
+// correction lands in [0, p)
+__device__ __forceinline__ uint32_t reduce_f64(double d, const Digits &dg) {
+    double r = fma(-rint(d * dg.pinv), dg.pd, d);
+    r = r < 0.0 ? r + dg.pd : r;
+    return (uint32_t)r;
+}
+
+// A block tile of 64 x 64: four waves of 32 x 32 (RS x CS products of
+// 16 x 16), over KC chunks of 64 slots from chunk dg.k0.  The operands are
+// 16-byte loads of the digit planes (L2-resident); used for K = 64.
 template <int ND, int KC, int RS = 2, int CS = 2>
 __device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t m, int32_t cb,
                                                int32_t ce, int32_t c0, const Panel *pan,
@@ -630,27 +653,30 @@ __device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t 
         for (int cs = 0; cs < CS; ++cs)
 #pragma unroll
             for (int q = 0; q < 2 * ND - 1; ++q) acc[rs][cs][q] = v4i{0, 0, 0, 0};
-    // KC chunks of 64 pivot slots from chunk k0; the digit planes hold
-    // dg.stride bytes per row (L) and per column (R)
-#pragma unroll 1
+    // rows past m and columns past ce read a valid row / column and are
+    // never stored
+    const int8_t *lp[RS], *rp[CS];
+#pragma unroll
+    for (int h = 0; h < RS; ++h) {
+        const int32_t row = r0 + 16 * h + lr;
+        lp[h] = dg.L + (int64_t)(row < m ? row : m - 1) * 64 + 16 * lk;
+    }
+#pragma unroll
+    for (int h = 0; h < CS; ++h) {
+        const int32_t col = q0 + 16 * h + lr;
+        rp[h] = dg.R + (int64_t)((col < ce ? col : ce - 1) - c0) * 64 + 16 * lk;
+    }
+#pragma unroll
     for (int kc = 0; kc < KC; ++kc) {
-        const int off = 64 * (dg.k0 + kc) + 16 * lk;
         v4i A[ND][RS], B[ND][CS];
 #pragma unroll
         for (int i = 0; i < ND; ++i) {
 #pragma unroll
-            for (int h = 0; h < RS; ++h) {
-                const int32_t row = r0 + 16 * h + lr;
-                A[i][h] = row < m ? *(const v4i *)(dg.L + i * dg.lplane + (int64_t)row * dg.stride + off)
-                                  : v4i{0, 0, 0, 0};
-            }
+            for (int h = 0; h < RS; ++h)
+                A[i][h] = *(const v4i *)(lp[h] + i * dg.lplane + (dg.k0 + kc) * dg.lchunk);
 #pragma unroll
-            for (int h = 0; h < CS; ++h) {
-                const int32_t col = q0 + 16 * h + lr;
-                B[i][h] = col < ce ? *(const v4i *)(dg.R + i * dg.rplane +
-                                                    (int64_t)(col - c0) * dg.stride + off)
-                                   : v4i{0, 0, 0, 0};
-            }
+            for (int h = 0; h < CS; ++h)
+                B[i][h] = *(const v4i *)(rp[h] + i * dg.rplane + (dg.k0 + kc) * dg.rchunk);
         }
 #pragma unroll
         for (int i = 0; i < ND; ++i)
@@ -673,10 +699,11 @@ __device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t 
             for (int cs = 0; cs < CS; ++cs) {
                 const int32_t col = q0 + 16 * cs + lr;
                 if (col >= ce) continue;
-                const uint32_t u = digits_combine<ND>(acc[rs][cs], e, md, dg);
-                a[(int64_t)row * ld + col] = sub_mod(xv[rs][e][cs], u, md.p);
+                a[(int64_t)row * ld + col] =
+                    reduce_f64(digits_residual<ND>(xv[rs][e][cs], acc[rs][cs], e, dg), dg);
             }
         }
+    (void)md;
 }
 
 template <int ND, int KC = 1, int RS = 2, int CS = 2>
@@ -688,39 +715,194 @@ __global__ __launch_bounds__(256) void k_update_mfma(uint32_t *a, int64_t ld, in
                                    blockIdx.y);
 }
 
-// block tile rows x columns (32 RS x 32 CS)
-static void update_i8_tile(int nd, int kc, int *rows, int *cols) {
-    *rows = 64;
-    (void)nd;
-    (void)kc;
-    *cols = 64;
+// K = 256 (four chunks from chunk 0) over wide column ranges.  The block's
+// four waves take four strips of 16 RS rows and walk WT_NT column tiles of
+// 32 together; each wave keeps its rows' L digits in registers for the
+// whole walk.  A tile's Rp digits (nd x 4 chunks x 32 columns x 64 bytes)
+// are staged in LDS once for the block, one tile ahead (two buffers, one
+// barrier per tile), and the A values of the next tile are loaded while
+// the current one is multiplied.
+constexpr int WT_NT = 16;   // column tiles per block
+constexpr int WT_CS = 2;    // 16-column products per tile
+
+template <int ND, int RS>
+__global__ __launch_bounds__(256) void k_update_wide(uint32_t *a, int64_t ld, int32_t m,
+                                                     int32_t cb, int32_t ce, int32_t c0,
+                                                     const Panel *pan, const int32_t *active,
+                                                     Digits dg) {
+    constexpr int CS = WT_CS, TW = 16 * CS;
+    constexpr int TB = ND * 4 * TW * 64;   // bytes of Rp digits per tile
+    constexpr int PPT = TB / (16 * 256);   // 16-byte pieces per thread
+    __shared__ __align__(16) int8_t rl[2][TB];
+    const int kk = __builtin_amdgcn_readfirstlane(pan->k);
+    const int32_t qb = cb + (int32_t)blockIdx.x * (WT_NT * TW);
+    if (kk == 0 || qb >= ce) return;  // uniform over the block
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int32_t r0 = ((int32_t)blockIdx.y * 4 + wv) * (16 * RS);
+    const bool wact = __any(lane < 16 * RS && r0 + lane < m && active[r0 + lane] != 0);
+    const int lr = lane & 15, lk = lane >> 4;
+    // piece u 256 + tid of a tile: digit i, chunk kc, column c / 4, bytes
+    // 16 (c % 4) .. + 15; the LDS image is [i][kc][column][64]
+    auto rload = [&](int32_t q0, v4i (&st)[PPT]) {
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            const int idx = u * 256 + tid;
+            const int ik = idx / (TW * 4), c = idx % (TW * 4);
+            const int i = ik >> 2, kc = ik & 3;
+            int32_t col = q0 + (c >> 2);
+            col = col < ce ? col : ce - 1;
+            st[u] = *(const v4i *)(dg.R + i * dg.rplane + kc * dg.rchunk + (int64_t)(col - c0) * 64 +
+                                   16 * (c & 3));
+        }
+    };
+    auto rstore = [&](int buf, const v4i (&st)[PPT]) {
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) *(v4i *)(rl[buf] + (u * 256 + tid) * 16) = st[u];
+    };
+    v4i Lr[4][ND][RS];
+    if (wact) {
+#pragma unroll
+        for (int h = 0; h < RS; ++h) {
+            const int32_t row = r0 + 16 * h + lr;
+            const int8_t *lp = dg.L + (int64_t)(row < m ? row : m - 1) * 64 + 16 * lk;
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc)
+#pragma unroll
+                for (int i = 0; i < ND; ++i)
+                    Lr[kc][i][h] = *(const v4i *)(lp + i * dg.lplane + kc * dg.lchunk);
+        }
+    }
+    bool xa[RS][4];
+    uint32_t *xp[RS][4];
+#pragma unroll
+    for (int rs = 0; rs < RS; ++rs)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int32_t row = r0 + 16 * rs + 4 * lk + e;
+            xa[rs][e] = row < m && active[row] != 0;
+            xp[rs][e] = a + (int64_t)(xa[rs][e] ? row : 0) * ld + lr;
+        }
+    // A values: columns past ce read the row's last column instead
+    uint32_t xv[RS][4][CS], xn[RS][4][CS];
+    const int32_t clast = ce - 1 - lr;
+    auto loadx = [&](int32_t q0, uint32_t (&x)[RS][4][CS]) {
+#pragma unroll
+        for (int rs = 0; rs < RS; ++rs)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (xa[rs][e]) {
+#pragma unroll
+                    for (int cs = 0; cs < CS; ++cs) {
+                        const int32_t col = q0 + 16 * cs;
+                        x[rs][e][cs] = xp[rs][e][col < clast ? col : clast];
+                    }
+                }
+    };
+    int nt = (ce - qb + TW - 1) / TW;
+    nt = nt < WT_NT ? nt : WT_NT;
+    {
+        v4i st[PPT];
+        rload(qb, st);
+        rstore(0, st);
+    }
+    if (wact) loadx(qb, xv);
+    __syncthreads();
+#pragma unroll 1
+    for (int t = 0; t < nt; ++t) {
+        const int32_t q0 = qb + t * TW;
+        const bool more = t + 1 < nt;
+        v4i st[PPT];
+        if (more) {
+            rload(q0 + TW, st);
+            if (wact) loadx(q0 + TW, xn);
+        }
+        if (wact) {
+            const int8_t *lb = rl[t & 1] + lr * 64 + 16 * lk;
+            v4i acc[RS][CS][2 * ND - 1];
+#pragma unroll
+            for (int rs = 0; rs < RS; ++rs)
+#pragma unroll
+                for (int cs = 0; cs < CS; ++cs)
+#pragma unroll
+                    for (int q = 0; q < 2 * ND - 1; ++q) acc[rs][cs][q] = v4i{0, 0, 0, 0};
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) {
+                v4i B[ND][CS];
+#pragma unroll
+                for (int i = 0; i < ND; ++i)
+#pragma unroll
+                    for (int h = 0; h < CS; ++h)
+                        B[i][h] = *(const v4i *)(lb + ((i * 4 + kc) * TW + 16 * h) * 64);
+#pragma unroll
+                for (int i = 0; i < ND; ++i)
+#pragma unroll
+                    for (int j = 0; j < ND; ++j)
+#pragma unroll
+                        for (int rs = 0; rs < RS; ++rs)
+#pragma unroll
+                            for (int cs = 0; cs < CS; ++cs)
+                                acc[rs][cs][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                    Lr[kc][i][rs], B[j][cs], acc[rs][cs][i + j], 0, 0, 0);
+            }
+            const bool full = q0 + TW <= ce;
+#pragma unroll
+            for (int rs = 0; rs < RS; ++rs)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (!xa[rs][e]) continue;
+                    uint32_t v[CS];
+#pragma unroll
+                    for (int cs = 0; cs < CS; ++cs)
+                        v[cs] = reduce_f64(digits_residual<ND>(xv[rs][e][cs], acc[rs][cs], e, dg), dg);
+                    if (full) {
+#pragma unroll
+                        for (int cs = 0; cs < CS; ++cs) xp[rs][e][q0 + 16 * cs] = v[cs];
+                    } else {
+#pragma unroll
+                        for (int cs = 0; cs < CS; ++cs)
+                            if (q0 + 16 * cs + lr < ce) xp[rs][e][q0 + 16 * cs] = v[cs];
+                    }
+                }
+        }
+        if (more) {
+            rstore((t + 1) & 1, st);
+#pragma unroll
+            for (int rs = 0; rs < RS; ++rs)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+#pragma unroll
+                    for (int cs = 0; cs < CS; ++cs) xv[rs][e][cs] = xn[rs][e][cs];
+        }
+        // the next tile's Rp digits are visible, and every wave is done
+        // with the buffer the tile after it overwrites
+        __syncthreads();
+    }
 }
 
 void launch_update_i8(int nd, int kc, hipStream_t s, uint32_t *a, int64_t ld, int32_t m,
                       int32_t cb, int32_t ce, int32_t c0, const Panel *pan,
                       const int32_t *active, Mod md, const Digits &dg) {
     if (cb >= ce || m <= 0) return;
-    int tr, tc;
-    update_i8_tile(nd, kc, &tr, &tc);
-    dim3 grid((unsigned)((ce - cb + tc - 1) / tc), (unsigned)((m + tr - 1) / tr));
-#define GBGPU_UPD(ND, KC, R, C) \
-    k_update_mfma<ND, KC, R, C><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg)
     if (kc == 4) {
+        // 64 rows per block for nd <= 2 (32 per wave), 32 rows for nd >= 3
+        const int rows = nd <= 2 ? 128 : 64;
+        const int cols = WT_NT * 16 * WT_CS;
+        dim3 grid((unsigned)((ce - cb + cols - 1) / cols), (unsigned)((m + rows - 1) / rows));
         switch (nd) {
-            case 1: GBGPU_UPD(1, 4, 2, 2); break;
-            case 2: GBGPU_UPD(2, 4, 2, 2); break;
-            case 3: GBGPU_UPD(3, 4, 2, 2); break;
-            default: GBGPU_UPD(4, 4, 2, 2); break;
+            case 1: k_update_wide<1, 2><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, dg); break;
+            case 2: k_update_wide<2, 2><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, dg); break;
+            case 3: k_update_wide<3, 1><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, dg); break;
+            default: k_update_wide<4, 1><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, dg); break;
         }
-    } else {
-        switch (nd) {
-            case 1: GBGPU_UPD(1, 1, 2, 2); break;
-            case 2: GBGPU_UPD(2, 1, 2, 2); break;
-            case 3: GBGPU_UPD(3, 1, 2, 2); break;
-            default: GBGPU_UPD(4, 1, 2, 2); break;
-        }
+        return;
     }
-#undef GBGPU_UPD
+    dim3 grid((unsigned)((ce - cb + 63) / 64), (unsigned)((m + 63) / 64));
+    switch (nd) {
+        case 1: k_update_mfma<1><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
+        case 2: k_update_mfma<2><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
+        case 3: k_update_mfma<3><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
+        default: k_update_mfma<4><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
+    }
 }
 
 // ------------------------------------------------------------ output

@@ -96,8 +96,8 @@ __global__ __launch_bounds__(256) void k_ds_rp(int32_t g0, int32_t n, const uint
     }
 }
 
-// digit planes of Rp^T: column c holds dg.stride bytes per digit, byte t =
-// row t (zero from row `rows` on)
+// digit planes of Rp^T: slot t of column c (zero from row `rows` on) at
+// byte t % 64 of column c in chunk t / 64
 __global__ void k_ds_rpdigits(const uint32_t *Rp, int64_t ldr, int rows, int32_t w, int kc,
                               uint32_t p, Digits dg) {
     const int lane = threadIdx.x & 63;
@@ -106,13 +106,14 @@ __global__ void k_ds_rpdigits(const uint32_t *Rp, int64_t ldr, int rows, int32_t
         for (int t = lane; t < 64 * kc; t += 64) {
             const uint32_t v = t < rows ? Rp[(int64_t)t * ldr + c] : 0u;
             for (int i = 0; i < dg.nd; ++i)
-                dg.R[i * dg.rplane + (int64_t)c * dg.stride + t] = (int8_t)digit_word(v, p, dg.hi, i);
+                dg.R[i * dg.rplane + (t >> 6) * dg.rchunk + (int64_t)c * 64 + (t & 63)] =
+                    (int8_t)digit_word(v, p, dg.hi, i);
         }
     }
 }
 
-// chunk a of L: M[q][c0 .. c0 + bw) (bw <= 64) as digits at bytes 64 a ..
-// 64 a + 63 of row q; act[q] = any nonzero, any[q] = OR over the chunks
+// chunk a of L: M[q][c0 .. c0 + bw) (bw <= 64) as the digits of row q in
+// chunk a; act[q] = any nonzero, any[q] = OR over the chunks
 __global__ void k_ds_gather(int32_t nrl, int32_t c0, int32_t bw, int a, const uint32_t *M,
                             int64_t ldm, int32_t *act, int32_t *any, uint32_t p, Digits dg) {
     const int lane = threadIdx.x & 63;
@@ -120,7 +121,8 @@ __global__ void k_ds_gather(int32_t nrl, int32_t c0, int32_t bw, int a, const ui
     for (int32_t q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < nrl; q += step) {
         const uint32_t v = lane < bw ? M[(int64_t)q * ldm + c0 + lane] : 0u;
         for (int i = 0; i < dg.nd; ++i)
-            dg.L[i * dg.lplane + (int64_t)q * dg.stride + 64 * a + lane] = (int8_t)digit_word(v, p, dg.hi, i);
+            dg.L[i * dg.lplane + a * dg.lchunk + (int64_t)q * 64 + lane] =
+                (int8_t)digit_word(v, p, dg.hi, i);
         const int nz = __any(v != 0u) ? 1 : 0;
         act[q] = nz;  // every lane stores the same flags
         any[q] = a == 0 ? nz : (any[q] | nz);
@@ -193,7 +195,8 @@ int run_dense_route(gbgpu_mat *m, const uint32_t *tinv) {
     dg.R = dr;
     dg.lplane = (int64_t)nrl * PW;
     dg.rplane = ldr * PW;
-    dg.stride = PW;
+    dg.lchunk = (int64_t)nrl * 64;
+    dg.rchunk = ldr * 64;
     const int32_t k64 = GB;
     GBGPU_HIP(ctx, hipMemcpyAsync(pk, &k64, sizeof(int32_t), hipMemcpyHostToDevice, s));
     GBGPU_HIP(ctx, hipMemsetAsync(M, 0, (size_t)nrl * ldm * sizeof(uint32_t), s));

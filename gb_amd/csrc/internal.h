@@ -126,21 +126,24 @@ struct Panel;
 // + 256^(nd-1)), a window of 256^nd consecutive integers; so for
 // p <= 256^nd every residue x has a representative in it: x itself when
 // x <= 127 S, else x - p.  nd = 1 for p <= 255, 2 for p <= 65536, 3 for
-// p <= 2^24, 4 below 2^31.  The digit planes hold, per digit, L as
-// [row][64] and Rp transposed as [column][64], so that one lane's 16
-// consecutive bytes are an MFMA operand.
+// p <= 2^24, 4 below 2^31.  A plane holds one digit of L (rows) or of Rp
+// transposed (columns) in chunks of 64 pivot slots, chunk-major:
+// [chunk][row or column][64 bytes], so that one lane's 16 consecutive
+// bytes are an MFMA operand and 16 rows or columns of a chunk are 1 KiB
+// of consecutive memory.
 struct Digits {
-    int8_t *L = nullptr;   // nd planes of m x 64
-    int8_t *R = nullptr;   // nd planes of ldr x 64
+    int8_t *L = nullptr;   // nd planes of nchunks x m x 64
+    int8_t *R = nullptr;   // nd planes of nchunks x ldr x 64
     int64_t lplane = 0;    // bytes per L plane
     int64_t rplane = 0;    // bytes per R plane
+    int64_t lchunk = 0;    // bytes per chunk of an L plane (m x 64)
+    int64_t rchunk = 0;    // bytes per chunk of an R plane (ldr x 64)
     int nd = 0;            // 0: no digit planes (plain form)
     uint32_t hi = 0;       // 127 S
-    uint64_t off = 0;      // multiples of p above the combined products
-    uint64_t off2 = 0;
-    uint32_t w32 = 0;      // 2^32 mod p
-    int32_t stride = 64;   // bytes per row (L) / column (R) of a plane
-    int32_t k0 = 0;        // first 64-byte chunk an update reads
+    int32_t k0 = 0;        // first chunk an update reads
+    // exact reduction of the digit products in double precision: p, 1 / p,
+    // 2^32 mod p and 2^48 mod p
+    double pd = 0, pinv = 0, w32 = 0, w48 = 0;
 };
 
 __host__ inline int digits_for(uint32_t p) {
@@ -152,12 +155,10 @@ __host__ inline void digits_setup(Digits &g, uint32_t p, int nd) {
     uint64_t S = 0, pw = 1;
     for (int i = 0; i < nd; ++i, pw *= 256) S += pw;
     g.hi = (uint32_t)(127 * S);
-    // the digit products C_s (|C_s| < 2^24, up to 256 pivot slots) combine
-    // as sum_s C_s 2^(8 s): below 2^57 for nd <= 3; for nd = 4 the terms
-    // s >= 4 (below 2^41) are reduced first and scaled by 2^32 mod p
-    g.w32 = (uint32_t)((1ull << 32) % p);
-    g.off = ((1ull << 57) / p + 1) * p;
-    g.off2 = ((1ull << 41) / p + 1) * p;
+    g.pd = (double)p;
+    g.pinv = 1.0 / (double)p;
+    g.w32 = (double)((1ull << 32) % p);
+    g.w48 = (double)((1ull << 48) % p);
 }
 
 __device__ __forceinline__ uint32_t digit_word(uint32_t x, uint32_t p, uint32_t hi, int i) {
@@ -173,8 +174,9 @@ __device__ __forceinline__ uint32_t digit_word(uint32_t x, uint32_t p, uint32_t 
 
 // the int8-digit MFMA update A[r][cb, ce) -= L[r] Rp (dense.hip) over kc
 // chunks of 64 pivot slots (kc = 1 or 4) from chunk dg.k0: L and Rp^T as
-// digit planes of dg with dg.stride bytes per row / column (Rp^T indexed
-// from column c0), rows with active[r]
+// the digit planes of dg (Rp^T indexed from column c0), rows with
+// active[r].  kc = 4 takes the kernel that walks column tiles with the
+// block's Rp digits staged in LDS.
 void launch_update_i8(int nd, int kc, hipStream_t s, uint32_t *a, int64_t ld, int32_t m,
                       int32_t cb, int32_t ce, int32_t c0, const Panel *pan,
                       const int32_t *active, Mod md, const Digits &dg);
