@@ -10,18 +10,19 @@
 // (four groups of 64) after the other:
 //
 //   P     = the panel's pivot rows, dense from the panel on
-//   Rp_a  = T_a^-1 P_a per group a (T_a the group's unit upper-triangular
-//           block, the inverses of the blocked pass): identity on the
-//           group's pivot columns
-//   for a = 0..3:  L_a = M[:, group a] (current values)
-//                  M[:, later groups of the panel] -= L_a Rp_a   (K = 64)
-//   M[:, right of the panel] -= [L_0 .. L_3] [Rp_0; ..; Rp_3]    (K = 256)
+//   for a = 0..3:  x_a = M[:, group a] T_a^-1   (T_a the group's unit
+//                  upper-triangular block, inverted for the blocked pass):
+//                  the multipliers that clear the group's columns
+//                  M[:, later groups of the panel] -= x_a P_a   (K = 64)
+//   M[:, right of the panel] -= [x_0 .. x_3] [P_0; ..; P_3]   (K = 256)
 //
-// The updates are the int8-digit MFMA kernel of the dense echelon form; the
-// wide one covers four groups per pass over M.
-// Columns without a pivot get identity rows in T_g and zero rows in P_g,
-// so a row's entry there is left alone, as the reference skips them
-// (la_ff_32.c:517-519); only the right-hand part is kept.
+// All products run on the int8 matrix cores in signed base-256 digits: the
+// multipliers (nrl x 64 x 64 per group), and the updates with the dense
+// echelon form's kernels (the wide one covers four groups per pass over M).
+// Columns without a pivot get identity rows in T_g and zero rows in P_g:
+// their multiplier meets a zero row, so a row's entry there is left alone,
+// as the reference skips them (la_ff_32.c:517-519); only the right-hand part
+// is kept.
 #include "internal.h"
 
 namespace gbgpu {
@@ -61,71 +62,141 @@ __global__ void k_ds_group_rows(int32_t g0, int32_t ncl, int rows, const int32_t
     for (int64_t k = b + lane; k < e; k += 64) out[ci[k]] = c[k];
 }
 
-// Rp[t][c] = sum_s Tinv[t][s] P[s][c] for the columns c of [g0, n) (c
-// relative to g0): a block covers 64 columns, each wave 16 rows t; sums in
-// 128 bits, reduced once
-__global__ __launch_bounds__(256) void k_ds_rp(int32_t g0, int32_t n, const uint32_t *tinv,
-                                               const uint32_t *P, int64_t ldp, uint32_t *Rp,
-                                               int64_t ldr, Mod md, uint32_t r64) {
-    __shared__ uint32_t T[GB][GB + 1];
-    __shared__ uint32_t V[GB][GB + 1];
+// the nd signed digits of x's representative (x or x - p), packed four to a
+// word: byte u of out[i] word w = digit i of value 4 w + u
+__device__ __forceinline__ void digits4(const uint32_t (&x)[4], uint32_t p, uint32_t hi, int nd,
+                                        uint32_t *out) {
+    for (int i = 0; i < nd; ++i) out[i] = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        int64_t v = x[u] <= hi ? (int64_t)x[u] : (int64_t)x[u] - (int64_t)p;
+        for (int i = 0; i < nd; ++i) {
+            const int8_t d = (int8_t)(uint8_t)(v & 0xFF);
+            v = (v - d) >> 8;
+            out[i] |= (uint32_t)(uint8_t)d << (8 * u);
+        }
+    }
+}
+
+// digit planes of the panel's pivot rows P (rows >= `rows` as zero): slot t
+// of column c at byte t % 64 of column c in chunk t / 64.  A block turns 64
+// columns x 256 slots: each thread reads four slots of one column
+// (coalesced across the wave), and the digits go through LDS (17 words per
+// column and chunk, so neither the writes nor the reads collide) to leave as
+// 4 KiB of consecutive bytes per digit and chunk.
+__global__ __launch_bounds__(256) void k_ds_pdigits(const uint32_t *P, int64_t ldp, int rows,
+                                                    int32_t w, uint32_t p, Digits dg) {
+    __shared__ uint32_t t4[4 * 4 * 64 * 17];  // [digit][chunk][column][17 words]
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int32_t jb = blockIdx.x * 64;
-    for (int i = tid; i < GB * GB; i += 256) T[i >> 6][i & 63] = tinv[i];
-    for (int i = tid; i < GB * GB; i += 256) {
-        const int s_ = i >> 6, c = i & 63;
-        V[s_][c] = (g0 + jb + c < n) ? P[(int64_t)s_ * ldp + jb + c] : 0u;
+    const int32_t cb = (int32_t)blockIdx.x * 64;
+    const int32_t c = cb + lane;
+    for (int q = wv; q < 64; q += 4) {  // slots 4 q .. 4 q + 3
+        uint32_t x[4], d[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int t = 4 * q + u;
+            x[u] = (t < rows && c < w) ? P[(int64_t)t * ldp + c] : 0u;
+        }
+        digits4(x, p, dg.hi, dg.nd, d);
+        const int kc = q >> 4, wd = q & 15;
+        for (int i = 0; i < dg.nd; ++i) t4[((i * 4 + kc) * 64 + lane) * 17 + wd] = d[i];
     }
     __syncthreads();
-    const int32_t j = jb + lane;
-    if (g0 + j >= n) return;
-#pragma unroll 4
-    for (int q = 0; q < 16; ++q) {
-        const int t = 16 * wv + q;
-        uint64_t lo = 0;
-        uint32_t hi = 0;
-        for (int s_ = 0; s_ < GB; ++s_) {
-            const uint64_t pr = (uint64_t)T[t][s_] * (uint64_t)V[s_][lane];
-            const uint64_t o = lo;
-            lo += pr;
-            hi += lo < o;
+    const int col = tid >> 2, j = tid & 3;
+    for (int i = 0; i < dg.nd; ++i)
+        for (int kc = 0; kc < 4; ++kc) {
+            const uint32_t *src = t4 + ((i * 4 + kc) * 64 + col) * 17 + 4 * j;
+            const v4i v = v4i{(int)src[0], (int)src[1], (int)src[2], (int)src[3]};
+            *(v4i *)(dg.R + i * dg.rplane + kc * dg.rchunk + (int64_t)(cb + col) * 64 + 16 * j) = v;
         }
-        uint32_t r = md.reduce(lo);
-        if (hi) r = (uint32_t)((r + (uint64_t)md.mul(hi, r64)) % md.p);
-        Rp[(int64_t)t * ldr + j] = r;
-    }
 }
 
-// digit planes of Rp^T: slot t of column c (zero from row `rows` on) at
-// byte t % 64 of column c in chunk t / 64
-__global__ void k_ds_rpdigits(const uint32_t *Rp, int64_t ldr, int rows, int32_t w, int kc,
-                              uint32_t p, Digits dg) {
-    const int lane = threadIdx.x & 63;
-    const int32_t step = gridDim.x * (blockDim.x >> 6);
-    for (int32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < w; c += step) {
-        for (int t = lane; t < 64 * kc; t += 64) {
-            const uint32_t v = t < rows ? Rp[(int64_t)t * ldr + c] : 0u;
-            for (int i = 0; i < dg.nd; ++i)
-                dg.R[i * dg.rplane + (t >> 6) * dg.rchunk + (int64_t)c * 64 + (t & 63)] =
-                    (int8_t)digit_word(v, p, dg.hi, i);
-        }
-    }
-}
-
-// chunk a of L: M[q][c0 .. c0 + bw) (bw <= 64) as the digits of row q in
-// chunk a; act[q] = any nonzero, any[q] = OR over the chunks
-__global__ void k_ds_gather(int32_t nrl, int32_t c0, int32_t bw, int a, const uint32_t *M,
-                            int64_t ldm, int32_t *act, int32_t *any, uint32_t p, Digits dg) {
-    const int lane = threadIdx.x & 63;
-    const int32_t step = gridDim.x * (blockDim.x >> 6);
-    for (int32_t q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < nrl; q += step) {
-        const uint32_t v = lane < bw ? M[(int64_t)q * ldm + c0 + lane] : 0u;
+// T_g^-1 as B operands of x = v T_g^-1: digit i of T_g^-1[k][s] at byte k of
+// column s; group g's nd planes of 64 x 64 bytes at td + g nd 4096
+__global__ void k_ds_tdigits(int32_t ngroups, const uint32_t *tinv, int8_t *td, uint32_t p,
+                             Digits dg) {
+    const int64_t n = (int64_t)ngroups * GB * GB;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = q >> 12;
+        const int s = (int)((q >> 6) & 63), k = (int)(q & 63);
+        const uint32_t v = tinv[(g << 12) + k * GB + s];
         for (int i = 0; i < dg.nd; ++i)
-            dg.L[i * dg.lplane + a * dg.lchunk + (int64_t)q * 64 + lane] =
-                (int8_t)digit_word(v, p, dg.hi, i);
-        const int nz = __any(v != 0u) ? 1 : 0;
-        act[q] = nz;  // every lane stores the same flags
-        any[q] = a == 0 ? nz : (any[q] | nz);
+            td[(g * dg.nd + i) * 4096 + s * 64 + k] = (int8_t)digit_word(v, p, dg.hi, i);
+    }
+}
+
+// Multipliers of group a: x = v T^-1 with v = M[q][ca .. ca + wa) (zero from
+// wa on), on the matrix cores (a wave: 16 rows x 64 columns, K = 64), written
+// as the digits of chunk a of L; act[q] = row has a nonzero multiplier,
+// any[q] = OR over the panel's groups.  The digits of v are formed in
+// registers from M.
+template <int ND>
+__global__ __launch_bounds__(256) void k_ds_mult(int32_t nrl, int32_t ca, int32_t wa, int a,
+                                                 const uint32_t *M, int64_t ldm, const int8_t *td,
+                                                 int32_t *act, int32_t *any, uint32_t p, Digits dg) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int32_t r0 = ((int32_t)blockIdx.x * 4 + wv) * 16;
+    if (r0 >= nrl) return;  // wave-uniform
+    // A operand: row r0 + lr, slots 16 lk .. 16 lk + 15 (columns past ncl,
+    // from wa on, count as zero; they lie inside the row: ca + 64 <= ldm)
+    const int32_t row = r0 + lr;
+    const uint32_t *src = M + (int64_t)(row < nrl ? row : nrl - 1) * ldm + ca + 16 * lk;
+    v4i A[ND];
+    bool nz = false;
+#pragma unroll
+    for (int w4 = 0; w4 < 4; ++w4) {
+        uint32_t x[4], d[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = 16 * lk + 4 * w4 + u;
+            x[u] = (row < nrl && k < wa) ? src[4 * w4 + u] : 0u;
+            nz = nz || x[u] != 0u;
+        }
+        digits4(x, p, dg.hi, ND, d);
+#pragma unroll
+        for (int i = 0; i < ND; ++i) A[i][w4] = (int)d[i];
+    }
+    const uint64_t ball = __ballot(nz);
+    v4i acc[4][2 * ND - 1];
+#pragma unroll
+    for (int cs = 0; cs < 4; ++cs)
+#pragma unroll
+        for (int q = 0; q < 2 * ND - 1; ++q) acc[cs][q] = v4i{0, 0, 0, 0};
+#pragma unroll
+    for (int cs = 0; cs < 4; ++cs) {
+        v4i B[ND];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) B[j] = *(const v4i *)(td + j * 4096 + (16 * cs + lr) * 64 + 16 * lk);
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+#pragma unroll
+            for (int j = 0; j < ND; ++j)
+                acc[cs][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[i], B[j], acc[cs][i + j], 0, 0, 0);
+    }
+    // result element e of lane l: row r0 + 4 (l / 16) + e, column 16 cs + l % 16
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int rl = 4 * lk + e;
+        const int32_t ro = r0 + rl;
+        if (ro >= nrl) continue;
+        const int f = ((ball >> rl) & 0x0001000100010001ull) != 0 ? 1 : 0;
+#pragma unroll
+        for (int cs = 0; cs < 4; ++cs) {
+            const uint32_t r = reduce_f64(digits_residual<ND>(0u, acc[cs], e, dg), dg);
+            const uint32_t x = r == 0u ? 0u : p - r;  // the residual is -x
+            int64_t v = x <= dg.hi ? (int64_t)x : (int64_t)x - (int64_t)p;
+#pragma unroll
+            for (int i = 0; i < ND; ++i) {
+                const int8_t d = (int8_t)(uint8_t)(v & 0xFF);
+                v = (v - d) >> 8;
+                dg.L[i * dg.lplane + a * dg.lchunk + (int64_t)ro * 64 + 16 * cs + lr] = d;
+            }
+        }
+        // the sixteen lanes of a row store the same flags
+        act[ro] = f;
+        any[ro] = a == 0 ? f : (any[ro] | f);
     }
 }
 
@@ -159,7 +230,7 @@ bool dense_route_pays(const gbgpu_mat *m) {
     // pivot rows' mean length against the width: dense work ~ nc / 2 per
     // pivot and row, sparse work ~ the pivot's length; the matrix cores
     // are worth it from ~1/40 density on
-    if (m->ncl < GB || m->nrl < 64) return false;
+    if (m->ncl < GB || m->nrl < 1) return false;
     const double mean_len = (double)m->nbucket / (double)m->ncl;
     return mean_len * 40.0 > (double)m->nc;
 }
@@ -170,23 +241,25 @@ int run_dense_route(gbgpu_mat *m, const uint32_t *tinv) {
     constexpr int KCH = 4;               // groups per panel
     constexpr int PW = GB * KCH;         // pivot columns per panel
     const int32_t nrl = m->nrl, ncl = m->ncl, nc = m->nc;
+    const int32_t ngroups = (ncl + GB - 1) / GB;
     const int64_t ldm = ((int64_t)nc + 63) & ~63ll;
     const int64_t ldp = ldm, ldr = ldm;
     const Mod md = Mod::make(m->p);
-    const uint32_t r64 = (uint32_t)((((unsigned __int128)1) << 64) % m->p);
     Digits dg;
     digits_setup(dg, m->p, digits_for(m->p));
-    uint32_t *M = nullptr, *P = nullptr, *Rp = nullptr;
-    int8_t *dl = nullptr, *dr = nullptr;
+    const int nd = dg.nd;
+    uint32_t *M = nullptr, *P = nullptr;
+    int8_t *dl = nullptr, *dr = nullptr, *td = nullptr;
     int32_t *active = nullptr, *act = nullptr, *pk = nullptr;  // pk: a panel record whose k is 64
     int rc;
     auto cleanup = [&]() {
-        dfree(M); dfree(P); dfree(Rp); dfree(dl); dfree(dr); dfree(active); dfree(act); dfree(pk);
+        dfree(M); dfree(P); dfree(dl); dfree(dr); dfree(td); dfree(active); dfree(act); dfree(pk);
     };
     if ((rc = dalloc(ctx, &M, (size_t)nrl * ldm)) || (rc = dalloc(ctx, &P, (size_t)PW * ldp)) ||
-        (rc = dalloc(ctx, &Rp, (size_t)PW * ldr)) || (rc = dalloc(ctx, &act, (size_t)nrl)) ||
-        (rc = dalloc(ctx, &dl, (size_t)dg.nd * nrl * PW)) ||
-        (rc = dalloc(ctx, &dr, (size_t)dg.nd * ldr * PW)) || (rc = dalloc(ctx, &active, (size_t)nrl)) ||
+        (rc = dalloc(ctx, &act, (size_t)nrl)) || (rc = dalloc(ctx, &dl, (size_t)nd * nrl * PW)) ||
+        (rc = dalloc(ctx, &dr, (size_t)nd * ldr * PW)) ||
+        (rc = dalloc(ctx, &td, (size_t)nd * ngroups * GB * GB)) ||
+        (rc = dalloc(ctx, &active, (size_t)nrl)) ||
         (rc = dalloc(ctx, &pk, (dense_panel_bytes() + 3) / 4))) {
         cleanup();
         return rc;
@@ -202,39 +275,41 @@ int run_dense_route(gbgpu_mat *m, const uint32_t *tinv) {
     GBGPU_HIP(ctx, hipMemsetAsync(M, 0, (size_t)nrl * ldm * sizeof(uint32_t), s));
     k_ds_scatter<<<grid_rows(nrl, 4), 256, 0, s>>>(nrl, m->upiv, m->rp, m->ci, m->rcf, m->cp, m->cf,
                                                     M, ldm);
+    k_ds_tdigits<<<grid_rows((int64_t)ngroups * GB * GB, 256), 256, 0, s>>>(ngroups, tinv, td, m->p, dg);
     const Panel *pan = (const Panel *)pk;
+    const int mult_blocks = (nrl + 63) / 64;
     for (int32_t g0 = 0; g0 < ncl; g0 += PW) {
         const int32_t bw = (ncl - g0) < PW ? (ncl - g0) : PW;  // pivot columns of the panel
         const int na = (bw + GB - 1) / GB;                     // its groups
         const int32_t w = nc - g0;                             // columns from the panel on
-        const int nb = (w + 63) / 64;
         GBGPU_HIP(ctx, hipMemsetAsync(P, 0, (size_t)PW * ldp * sizeof(uint32_t), s));
         k_ds_group_rows<<<PW / 4, 256, 0, s>>>(g0, ncl, PW, m->first, m->rp, m->ci, m->rcf, m->cp,
                                                m->cf, P, ldp);
-        // each group's rows made unit on its own pivot columns
-        for (int a = 0; a < na; ++a)
-            k_ds_rp<<<nb, 256, 0, s>>>(g0, nc, tinv + (int64_t)(g0 / GB + a) * 4096,
-                                       P + (int64_t)a * GB * ldp, ldp, Rp + (int64_t)a * GB * ldr, ldr,
-                                       md, r64);
-        k_ds_rpdigits<<<grid_rows(w, 4), 256, 0, s>>>(Rp, ldr, na * GB, w, KCH, m->p, dg);
-        // the panel's groups in order: the multipliers of group a are M's
-        // current values on its columns; its rows update the panel's later
-        // columns at once (K = 64) ...
+        k_ds_pdigits<<<(w + 63) / 64, 256, 0, s>>>(P, ldp, na * GB, w, m->p, dg);
+        // the panel's groups in order: group a's multipliers clear its
+        // columns; its rows update the panel's later columns at once
+        // (K = 64) ...
         for (int a = 0; a < na; ++a) {
             const int32_t ca = g0 + a * GB;
             const int32_t wa = (bw - a * GB) < GB ? (bw - a * GB) : GB;
-            k_ds_gather<<<grid_rows(nrl, 4), 256, 0, s>>>(nrl, ca, wa, a, M, ldm, act, active, m->p, dg);
+            const int8_t *tg = td + (int64_t)(g0 / GB + a) * nd * GB * GB;
+            switch (nd) {
+                case 1: k_ds_mult<1><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, m->p, dg); break;
+                case 2: k_ds_mult<2><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, m->p, dg); break;
+                case 3: k_ds_mult<3><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, m->p, dg); break;
+                default: k_ds_mult<4><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, m->p, dg); break;
+            }
             const int32_t nb0 = ca + GB, nb1 = g0 + bw;
             if (nb0 < nb1) {
                 Digits d1 = dg;
                 d1.k0 = a;
-                launch_update_i8(dg.nd, 1, s, M, ldm, nrl, nb0, nb1, g0, pan, act, md, d1);
+                launch_update_i8(nd, 1, s, M, ldm, nrl, nb0, nb1, g0, pan, act, md, d1);
             }
         }
         // ... and the columns right of the panel all four together (K = 256;
-        // the last panel may be partial: its right-hand columns are updated too)
+        // the digits of a partial panel's missing groups meet zero rows)
         const int32_t cb = g0 + bw, ce = nc;
-        launch_update_i8(dg.nd, KCH, s, M, ldm, nrl, cb, ce, g0, pan, active, md, dg);
+        launch_update_i8(nd, KCH, s, M, ldm, nrl, cb, ce, g0, pan, active, md, dg);
         GBGPU_HIP(ctx, hipGetLastError());
     }
     k_ds_extract<<<grid_rows(nrl, 4), 256, 0, s>>>(nrl, ncl, nc, M, ldm, m->dprime, m->dnz);

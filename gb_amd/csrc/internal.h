@@ -172,6 +172,51 @@ __device__ __forceinline__ uint32_t digit_word(uint32_t x, uint32_t p, uint32_t 
     return (uint8_t)d;
 }
 
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// x - sum_s Cs 256^s as a double that is exact and congruent to it modulo
+// p.  Magnitudes (256 slots): nd = 2 below 2^39; nd = 3 below 2^48, with C4
+// entering as C4 (2^32 mod p); nd = 4: C0..C3 below 2^49, and
+// hi = C4 + 2^8 C5 + 2^16 C6 (below 2^39) enters as h = hi mod p split in
+// 16-bit halves, h0 (2^32 mod p) + h1 (2^48 mod p), each below 2^47.  Every
+// partial sum is an integer below 2^53, so each step is exact.
+template <int ND>
+__device__ __forceinline__ double digits_residual(uint32_t x, const v4i *acc, int e,
+                                                  const Digits &dg) {
+    double d = (double)x - (double)acc[0][e];
+    if constexpr (ND >= 2) {
+        d = fma(-256.0, (double)acc[1][e], d);
+        d = fma(-65536.0, (double)acc[2][e], d);
+    }
+    if constexpr (ND == 3) {
+        d = fma(-16777216.0, (double)acc[3][e], d);
+        d = fma(-dg.w32, (double)acc[4][e], d);
+    }
+    if constexpr (ND == 4) {
+        d = fma(-16777216.0, (double)acc[3][e], d);
+        double hi = (double)acc[4][e];
+        hi = fma(256.0, (double)acc[5][e], hi);
+        hi = fma(65536.0, (double)acc[6][e], hi);
+        double h = fma(-rint(hi * dg.pinv), dg.pd, hi);
+        h = h < 0.0 ? h + dg.pd : h;
+        const double h1 = floor(h * (1.0 / 65536.0));
+        const double h0 = fma(-65536.0, h1, h);
+        d = fma(-dg.w32, h0, d);
+        d = fma(-dg.w48, h1, d);
+    }
+    return d;
+}
+
+// d mod p for an integer-valued double |d| < 2^53: q = rint(d / p) is
+// within one of the nearest quotient (the product d (1 / p) is off by less
+// than 1 / (2 p)), so d - q p lies in [-(p + 1) / 2, (p + 1) / 2] and one
+// correction lands in [0, p)
+__device__ __forceinline__ uint32_t reduce_f64(double d, const Digits &dg) {
+    double r = fma(-rint(d * dg.pinv), dg.pd, d);
+    r = r < 0.0 ? r + dg.pd : r;
+    return (uint32_t)r;
+}
+
 // the int8-digit MFMA update A[r][cb, ce) -= L[r] Rp (dense.hip) over kc
 // chunks of 64 pivot slots (kc = 1 or 4) from chunk dg.k0: L and Rp^T as
 // the digit planes of dg (Rp^T indexed from column c0), rows with
