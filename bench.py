@@ -98,7 +98,10 @@ def f4_runs(nvars: int, ctx, with_cpu: bool, forms=("mfma", "plain"), cpu_vars: 
         st = b.stats
         res[name] = {"total_s": time.perf_counter() - t0, "la_s": st["la_ms"] / 1e3,
                      "la_device_s": st["device_ms"] / 1e3,
-                     "host_s": (st["total_ms"] - st["la_ms"]) / 1e3, "rounds": st["rounds"],
+                     "host_s": (st["total_ms"] - st["la_ms"]) / 1e3,
+                     "host_phases_s": {k: st[k + "_ms"] / 1e3
+                                       for k in ("select", "symbolic", "build", "update")},
+                     "rounds": st["rounds"],
                      "basis_elements": len(b.lens), "max_matrix": [st["max_rows"], st["max_cols"]],
                      "max_nnz": st["max_nnz"], "rows_reduced": st["rows_total"],
                      "rows_per_s_la": st["rows_total"] / (st["la_ms"] / 1e3)}
@@ -142,6 +145,7 @@ def main(argv=None) -> int:
                     help="variables of the dense quadratic system run end to end by the F4 host "
                          "(14 = the 14-variable configuration; plain and MFMA forms)")
     ap.add_argument("--only-f4", action="store_true", help="only the end-to-end F4 runs")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-matrix batch extra")
     ap.add_argument("--f4-forms", default="mfma,plain", help="forms of the F4 runs")
     ap.add_argument("--f4-info", action="store_true", help="print one line per F4 round")
     ap.add_argument("--f4-cpu-vars", type=int, default=12,
@@ -267,10 +271,12 @@ def main(argv=None) -> int:
         # (rows lognormal in [500, 20000], columns 1.5 x rows, density
         # 0.5-2 %), each modulo its own prime below 2^31; la_option 1 on all
         # of them as one graph launch, against the same matrices one by one
-        mats = synth.batch_31bit(64, seed=7)
+        mats = [] if args.no_batch else synth.batch_31bit(64, seed=7)
         nrl_b = sum(mm.nr - len(np.unique(mm.leads())) for mm in mats)
-        bt = ctx.batch(mats)
+        bt = ctx.batch(mats) if mats else None
         for form, name in ((native.FORM_MFMA, "mfma"), (native.FORM_PLAIN, "plain")):
+            if bt is None:
+                break
             bt.la(form)  # device layout, tuning, graph capture
             ts, gs = [], []
             for _ in range(3):
@@ -285,7 +291,8 @@ def main(argv=None) -> int:
             extra[f"batch64_{name}_graph_ms"] = sorted(gs)[1]
             extra[f"batch64_{name}_rows_per_s"] = nrl_b / tb
             extra[f"batch64_{name}_tuned"] = bt.info()
-        bt.free()
+        if bt is not None:
+            bt.free()
         singles = [ctx.upload(mm) for mm in mats]
         for dm in singles:
             dm.la_sparse_dense(native.FORM_MFMA)
@@ -295,8 +302,9 @@ def main(argv=None) -> int:
             dm.la_sparse_dense(native.FORM_MFMA)
         _sync()
         t1 = time.perf_counter() - t0
-        extra["batch64_one_by_one_mfma_ms"] = 1000.0 * t1
-        extra["batch64_to_reduce_rows"] = nrl_b
+        if mats:
+            extra["batch64_one_by_one_mfma_ms"] = 1000.0 * t1
+            extra["batch64_to_reduce_rows"] = nrl_b
         for dm in singles:
             dm.free()
         out["extra"] = extra

@@ -585,8 +585,10 @@ __device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t 
     if (!__any(act) || q0 >= ce) return;
     const int lr = lane & 15, lk = lane >> 4;
     // this lane's values of A (rows r0 + 16 rs + 4 (l / 16) + e, columns
-    // q0 + 16 cs + l % 16), loaded before the products so that their
-    // latency hides behind them
+    // q0 + 16 cs + l % 16): for nd <= 2 loaded before the products so that
+    // their latency hides behind them; for nd >= 3 after them, where the
+    // registers would cost a wave per SIMD
+    constexpr bool PRE = ND <= 2;
     uint32_t xv[RS][4][CS];
     bool xa[RS][4];
 #pragma unroll
@@ -595,10 +597,12 @@ __device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t 
         for (int e = 0; e < 4; ++e) {
             const int32_t row = r0 + 16 * rs + 4 * lk + e;
             xa[rs][e] = row < m && active[row] != 0;
+            if constexpr (PRE) {
 #pragma unroll
-            for (int cs = 0; cs < CS; ++cs) {
-                const int32_t col = q0 + 16 * cs + lr;
-                xv[rs][e][cs] = (xa[rs][e] && col < ce) ? a[(int64_t)row * ld + col] : 0u;
+                for (int cs = 0; cs < CS; ++cs) {
+                    const int32_t col = q0 + 16 * cs + lr;
+                    xv[rs][e][cs] = (xa[rs][e] && col < ce) ? a[(int64_t)row * ld + col] : 0u;
+                }
             }
         }
     v4i acc[RS][CS][2 * ND - 1];
@@ -608,30 +612,25 @@ __device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t 
         for (int cs = 0; cs < CS; ++cs)
 #pragma unroll
             for (int q = 0; q < 2 * ND - 1; ++q) acc[rs][cs][q] = v4i{0, 0, 0, 0};
-    // rows past m and columns past ce read a valid row / column and are
-    // never stored
-    const int8_t *lp[RS], *rp[CS];
-#pragma unroll
-    for (int h = 0; h < RS; ++h) {
-        const int32_t row = r0 + 16 * h + lr;
-        lp[h] = dg.L + (int64_t)(row < m ? row : m - 1) * 64 + 16 * lk;
-    }
-#pragma unroll
-    for (int h = 0; h < CS; ++h) {
-        const int32_t col = q0 + 16 * h + lr;
-        rp[h] = dg.R + (int64_t)((col < ce ? col : ce - 1) - c0) * 64 + 16 * lk;
-    }
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc) {
+        const int8_t *lb = dg.L + (dg.k0 + kc) * dg.lchunk + 16 * lk;
+        const int8_t *rb = dg.R + (dg.k0 + kc) * dg.rchunk + 16 * lk;
         v4i A[ND][RS], B[ND][CS];
 #pragma unroll
         for (int i = 0; i < ND; ++i) {
 #pragma unroll
-            for (int h = 0; h < RS; ++h)
-                A[i][h] = *(const v4i *)(lp[h] + i * dg.lplane + (dg.k0 + kc) * dg.lchunk);
+            for (int h = 0; h < RS; ++h) {
+                const int32_t row = r0 + 16 * h + lr;
+                A[i][h] = row < m ? *(const v4i *)(lb + i * dg.lplane + (int64_t)row * 64)
+                                  : v4i{0, 0, 0, 0};
+            }
 #pragma unroll
-            for (int h = 0; h < CS; ++h)
-                B[i][h] = *(const v4i *)(rp[h] + i * dg.rplane + (dg.k0 + kc) * dg.rchunk);
+            for (int h = 0; h < CS; ++h) {
+                const int32_t col = q0 + 16 * h + lr;
+                B[i][h] = col < ce ? *(const v4i *)(rb + i * dg.rplane + (int64_t)(col - c0) * 64)
+                                   : v4i{0, 0, 0, 0};
+            }
         }
 #pragma unroll
         for (int i = 0; i < ND; ++i)
@@ -654,11 +653,11 @@ __device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t 
             for (int cs = 0; cs < CS; ++cs) {
                 const int32_t col = q0 + 16 * cs + lr;
                 if (col >= ce) continue;
-                a[(int64_t)row * ld + col] =
-                    reduce_f64(digits_residual<ND>(xv[rs][e][cs], acc[rs][cs], e, dg), dg);
+                uint32_t *x = a + (int64_t)row * ld + col;
+                const uint32_t xo = PRE ? xv[rs][e][cs] : *x;
+                *x = sub_mod(xo, digits_combine<ND>(acc[rs][cs], e, md, dg), md.p);
             }
         }
-    (void)md;
 }
 
 template <int ND, int KC = 1, int RS = 2, int CS = 2>
@@ -684,7 +683,7 @@ template <int ND, int RS>
 __global__ __launch_bounds__(256) void k_update_wide(uint32_t *a, int64_t ld, int32_t m,
                                                      int32_t cb, int32_t ce, int32_t c0,
                                                      const Panel *pan, const int32_t *active,
-                                                     Digits dg) {
+                                                     Mod md, Digits dg) {
     constexpr int CS = WT_CS, TW = 16 * CS;
     constexpr int TB = ND * 4 * TW * 64;   // bytes of Rp digits per tile
     constexpr int PPT = TB / (16 * 256);   // 16-byte pieces per thread
@@ -808,7 +807,7 @@ __global__ __launch_bounds__(256) void k_update_wide(uint32_t *a, int64_t ld, in
                     uint32_t v[CS];
 #pragma unroll
                     for (int cs = 0; cs < CS; ++cs)
-                        v[cs] = reduce_f64(digits_residual<ND>(xv[rs][e][cs], acc[rs][cs], e, dg), dg);
+                        v[cs] = sub_mod(xv[rs][e][cs], digits_combine<ND>(acc[rs][cs], e, md, dg), md.p);
                     if (full) {
 #pragma unroll
                         for (int cs = 0; cs < CS; ++cs) xp[rs][e][q0 + 16 * cs] = v[cs];
@@ -844,10 +843,10 @@ void launch_update_i8(int nd, int kc, hipStream_t s, uint32_t *a, int64_t ld, in
         const int cols = WT_NT * 16 * WT_CS;
         dim3 grid((unsigned)((ce - cb + cols - 1) / cols), (unsigned)((m + rows - 1) / rows));
         switch (nd) {
-            case 1: k_update_wide<1, 2><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, dg); break;
-            case 2: k_update_wide<2, 2><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, dg); break;
-            case 3: k_update_wide<3, 1><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, dg); break;
-            default: k_update_wide<4, 1><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, dg); break;
+            case 1: k_update_wide<1, 2><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
+            case 2: k_update_wide<2, 2><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
+            case 3: k_update_wide<3, 1><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
+            default: k_update_wide<4, 1><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
         }
         return;
     }

@@ -134,7 +134,8 @@ __global__ void k_ds_tdigits(int32_t ngroups, const uint32_t *tinv, int8_t *td, 
 template <int ND>
 __global__ __launch_bounds__(256) void k_ds_mult(int32_t nrl, int32_t ca, int32_t wa, int a,
                                                  const uint32_t *M, int64_t ldm, const int8_t *td,
-                                                 int32_t *act, int32_t *any, uint32_t p, Digits dg) {
+                                                 int32_t *act, int32_t *any, Mod md, Digits dg) {
+    const uint32_t p = md.p;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const int32_t r0 = ((int32_t)blockIdx.x * 4 + wv) * 16;
@@ -184,8 +185,7 @@ __global__ __launch_bounds__(256) void k_ds_mult(int32_t nrl, int32_t ca, int32_
         const int f = ((ball >> rl) & 0x0001000100010001ull) != 0 ? 1 : 0;
 #pragma unroll
         for (int cs = 0; cs < 4; ++cs) {
-            const uint32_t r = reduce_f64(digits_residual<ND>(0u, acc[cs], e, dg), dg);
-            const uint32_t x = r == 0u ? 0u : p - r;  // the residual is -x
+            const uint32_t x = digits_combine<ND>(acc[cs], e, md, dg);
             int64_t v = x <= dg.hi ? (int64_t)x : (int64_t)x - (int64_t)p;
 #pragma unroll
             for (int i = 0; i < ND; ++i) {
@@ -294,10 +294,10 @@ int run_dense_route(gbgpu_mat *m, const uint32_t *tinv) {
             const int32_t wa = (bw - a * GB) < GB ? (bw - a * GB) : GB;
             const int8_t *tg = td + (int64_t)(g0 / GB + a) * nd * GB * GB;
             switch (nd) {
-                case 1: k_ds_mult<1><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, m->p, dg); break;
-                case 2: k_ds_mult<2><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, m->p, dg); break;
-                case 3: k_ds_mult<3><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, m->p, dg); break;
-                default: k_ds_mult<4><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, m->p, dg); break;
+                case 1: k_ds_mult<1><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg); break;
+                case 2: k_ds_mult<2><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg); break;
+                case 3: k_ds_mult<3><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg); break;
+                default: k_ds_mult<4><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg); break;
             }
             const int32_t nb0 = ca + GB, nb1 = g0 + bw;
             if (nb0 < nb1) {

@@ -260,12 +260,6 @@ static int32_t mt_div(mt_t *t, int32_t b, int32_t a) {
     return mt_get(t, t->tmp, t->hv[b] - t->hv[a]);
 }
 
-static int32_t mt_lcm(mt_t *t, int32_t a, int32_t b) {
-    const uint8_t *ea = EV(t, a), *eb = EV(t, b);
-    for (int i = 0; i < t->nvp; ++i) t->tmp[i] = ea[i] > eb[i] ? ea[i] : eb[i];
-    return mt_get(t, t->tmp, mt_hash(t, t->tmp));
-}
-
 static inline int mt_divides(const mt_t *t, int32_t a, int32_t b) {
     if (t->dm[a] & ~t->dm[b]) return 0;
     if (t->deg[a] > t->deg[b]) return 0;
@@ -573,11 +567,35 @@ static int cmp_rkey(const void *a, const void *b) {
     return x->idx < y->idx ? -1 : (x->idx > y->idx);
 }
 
+/* a round's matrix in gb's row format, kept from round to round so that
+ * the large arrays are allocated (and their pages touched) once */
+typedef struct {
+    int64_t *rp;
+    int32_t *ci, *rcf, *eid;
+    int64_t *cp;
+    uint32_t *cf;
+    rkey_t *key;
+    int64_t crp, cci, crcf, ceid, ccp, ccf, ckey;
+} csrbuf_t;
+
+/* room for n elements of size elt; the old contents are not kept */
+static int buf_fit(void **p, int64_t *cap, int64_t n, size_t elt) {
+    if (n <= *cap) return 0;
+    int64_t nc = *cap ? *cap : 1024;
+    while (nc < n) nc += nc / 2;
+    free(*p);
+    *p = malloc((size_t)nc * elt);
+    *cap = *p ? nc : 0;
+    return *p ? 0 : -1;
+}
+
 typedef struct {
     mt_t T;
     basis_t B;
     pairs_t P;
     marks_t M;
+    rows_t R;     /* the round's rows, reused */
+    csrbuf_t W;   /* the round's matrix, reused */
     uint32_t p;
     int32_t round;
     gbgpu_f4_la_t la;
@@ -586,21 +604,23 @@ typedef struct {
     int info;
 } f4_t;
 
-/* turn rows (element, multiplier) into gb's CSR over the given columns */
-static int build_csr(f4_t *F, rows_t *R, int32_t *cols, int32_t nc, int32_t ncl,
-                     gbgpu_csr_t *m, int64_t **rp_, int32_t **ci_, int32_t **rcf_,
-                     int64_t **cp_, uint32_t **cf_) {
-    mt_t *T = &F->T;
+/* turn rows (element, multiplier) into gb's CSR over the given columns
+ * (arrays in F->W) */
+static int build_csr(f4_t *F, rows_t *R, int32_t nc, int32_t ncl, gbgpu_csr_t *m) {
+    csrbuf_t *W = &F->W;
     int64_t nnz = 0;
     for (int64_t r = 0; r < R->n; ++r) nnz += R->r[r].len;
-    /* rows by (lead column, length) */
-    rkey_t *key = (rkey_t *)malloc((size_t)(R->n ? R->n : 1) * sizeof(rkey_t));
-    /* coefficient arrays: one per distinct element */
-    int32_t *eid = (int32_t *)malloc((size_t)(F->B.n ? F->B.n : 1) * sizeof(int32_t));
-    int64_t *rp = (int64_t *)malloc((size_t)(R->n + 1) * sizeof(int64_t));
-    int32_t *ci = (int32_t *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int32_t));
-    int32_t *rcf = (int32_t *)malloc((size_t)(R->n ? R->n : 1) * sizeof(int32_t));
-    if (!key || !eid || !rp || !ci || !rcf) goto fail;
+    /* rows by (lead column, length); coefficient arrays: one per distinct
+     * element */
+    if (buf_fit((void **)&W->key, &W->ckey, R->n ? R->n : 1, sizeof(rkey_t)) ||
+        buf_fit((void **)&W->eid, &W->ceid, F->B.n ? F->B.n : 1, sizeof(int32_t)) ||
+        buf_fit((void **)&W->rp, &W->crp, R->n + 1, sizeof(int64_t)) ||
+        buf_fit((void **)&W->ci, &W->cci, nnz ? nnz : 1, sizeof(int32_t)) ||
+        buf_fit((void **)&W->rcf, &W->crcf, R->n ? R->n : 1, sizeof(int32_t)))
+        return -1;
+    rkey_t *key = W->key;
+    int32_t *eid = W->eid, *ci = W->ci, *rcf = W->rcf;
+    int64_t *rp = W->rp;
     for (int32_t i = 0; i < F->B.n; ++i) eid[i] = -1;
     int32_t ncf = 0;
     int64_t npool = 0;
@@ -615,9 +635,11 @@ static int build_csr(f4_t *F, rows_t *R, int32_t *cols, int32_t nc, int32_t ncl,
         }
     }
     qsort(key, (size_t)R->n, sizeof(rkey_t), cmp_rkey);
-    int64_t *cp = (int64_t *)malloc((size_t)(ncf + 1) * sizeof(int64_t));
-    uint32_t *cf = (uint32_t *)malloc((size_t)(npool ? npool : 1) * sizeof(uint32_t));
-    if (!cp || !cf) { free(cp); free(cf); goto fail; }
+    if (buf_fit((void **)&W->cp, &W->ccp, ncf + 1, sizeof(int64_t)) ||
+        buf_fit((void **)&W->cf, &W->ccf, npool ? npool : 1, sizeof(uint32_t)))
+        return -1;
+    int64_t *cp = W->cp;
+    uint32_t *cf = W->cf;
     cp[0] = 0;
     for (int32_t i = 0; i < F->B.n; ++i) {
         if (eid[i] < 0) continue;
@@ -644,20 +666,13 @@ static int build_csr(f4_t *F, rows_t *R, int32_t *cols, int32_t nc, int32_t ncl,
         const int32_t *mon = R->mon + x->off;
         for (int32_t t = 0; t < x->len; ++t) o[t] = colmap[mon[t]];
     }
-    free(key);
-    free(eid);
     m->nr = (int32_t)R->n;
     m->nc = nc;
     m->ncl = ncl;
     m->ncf = ncf;
     m->p = F->p;
     m->rp = rp; m->ci = ci; m->rcf = rcf; m->cp = cp; m->cf = cf;
-    *rp_ = rp; *ci_ = ci; *rcf_ = rcf; *cp_ = cp; *cf_ = cf;
-    (void)cols;
     return 0;
-fail:
-    free(key); free(eid); free(rp); free(ci); free(rcf);
-    return -1;
 }
 
 static int find_reducer(f4_t *F, int32_t mon) {
@@ -893,14 +908,13 @@ static int f4_round(f4_t *F) {
     while (nsel < P->n && P->p[nsel].deg == dmin) ++nsel;
     /* generators of the selected pairs, one row per distinct (e, mul) */
     em_t *gen = (em_t *)malloc((size_t)(2 * nsel) * sizeof(em_t));
-    rows_t R = {0};
+    rows_t *R = &F->R;
+    R->n = 0;
+    R->nmon = 0;
     int32_t *queue = NULL;
     int64_t nq = 0, capq = 0;
     int32_t *cols = NULL;
     gbgpu_rows_t out = {0};
-    int64_t *rp = NULL, *cp = NULL;
-    int32_t *ci = NULL, *rcf = NULL;
-    uint32_t *cf = NULL;
     if (!gen) goto done;
     for (int64_t q = 0; q < nsel; ++q) {
         const pair_t x = P->p[q];
@@ -915,13 +929,13 @@ static int f4_round(f4_t *F) {
     qsort(gen, (size_t)(2 * nsel), sizeof(em_t), cmp_em);
     for (int64_t q = 0; q < 2 * nsel; ++q) {
         if (q > 0 && gen[q].e == gen[q - 1].e && gen[q].mul == gen[q - 1].mul) continue;
-        if (row_stub(&R, B, gen[q].e, gen[q].mul)) goto done;
+        if (row_stub(R, B, gen[q].e, gen[q].mul)) goto done;
     }
-    if (rows_products(F, &R, 0, R.n) || rows_mark(F, &R, 0, R.n, &queue, &nq, &capq)) goto done;
-    const int64_t nspair_rows = R.n;
+    if (rows_products(F, R, 0, R->n) || rows_mark(F, R, 0, R->n, &queue, &nq, &capq)) goto done;
+    const int64_t nspair_rows = R->n;
     F->out->select_ms += now_ms() - tp;
     tp = now_ms();
-    if (symbolic(F, &R, &queue, &nq, &capq)) goto done;
+    if (symbolic(F, R, &queue, &nq, &capq)) goto done;
     F->out->symbolic_ms += now_ms() - tp;
     tp = now_ms();
     /* columns: pivot columns, then the rest, each decreasing */
@@ -940,8 +954,8 @@ static int f4_round(f4_t *F) {
     qsort(cols + ncl, (size_t)(nc - ncl), sizeof(int32_t), cmp_mon_desc);
     for (int32_t c = 0; c < nc; ++c) F->M.col[cols[c]] = c;
     gbgpu_csr_t m;
-    if (build_csr(F, &R, cols, nc, ncl, &m, &rp, &ci, &rcf, &cp, &cf)) goto done;
-    int64_t nnz = rp[m.nr];
+    if (build_csr(F, R, nc, ncl, &m)) goto done;
+    int64_t nnz = m.rp[m.nr];
     F->out->build_ms += now_ms() - tp;
     if (m.nr > F->out->max_rows) F->out->max_rows = m.nr;
     if (nc > F->out->max_cols) F->out->max_cols = nc;
@@ -969,8 +983,7 @@ static int f4_round(f4_t *F) {
     rc = GBGPU_OK;
 done:
     gbgpu_rows_free(&out);
-    free(gen); free(R.r); free(R.mon); free(queue); free(cols);
-    free(rp); free(ci); free(rcf); free(cp); free(cf);
+    free(gen); free(queue); free(cols);
     return rc;
 }
 
@@ -980,12 +993,11 @@ static int f4_reduce(f4_t *F) {
     basis_t *B = &F->B;
     int rc = GBGPU_ENOMEM;
     ++F->round;
-    rows_t R = {0};
+    rows_t *R = &F->R;
+    R->n = 0;
+    R->nmon = 0;
     int32_t *queue = NULL, *cols = NULL;
     int64_t nq = 0, capq = 0;
-    int64_t *rp = NULL, *cp = NULL;
-    int32_t *ci = NULL, *rcf = NULL;
-    uint32_t *cf = NULL;
     gbgpu_rows_t out = {0};
     int32_t one;
     {
@@ -996,11 +1008,11 @@ static int f4_reduce(f4_t *F) {
         if (one < 0) goto done;
     }
     for (int32_t q = 0; q < B->nlm; ++q)
-        if (row_stub(&R, B, B->lm[q], one)) goto done;
-    if (rows_products(F, &R, 0, R.n) || rows_mark(F, &R, 0, R.n, &queue, &nq, &capq)) goto done;
+        if (row_stub(R, B, B->lm[q], one)) goto done;
+    if (rows_products(F, R, 0, R->n) || rows_mark(F, R, 0, R->n, &queue, &nq, &capq)) goto done;
     /* reducers for the tail monomials divisible by a lead, so that the
      * reduced rows have no such monomial left */
-    if (symbolic(F, &R, &queue, &nq, &capq)) goto done;
+    if (symbolic(F, R, &queue, &nq, &capq)) goto done;
     cols = (int32_t *)malloc((size_t)(nq ? nq : 1) * sizeof(int32_t));
     if (!cols) goto done;
     memcpy(cols, queue, (size_t)nq * sizeof(int32_t));
@@ -1008,7 +1020,7 @@ static int f4_reduce(f4_t *F) {
     qsort(cols, (size_t)nq, sizeof(int32_t), cmp_mon_desc);
     for (int64_t c = 0; c < nq; ++c) F->M.col[cols[c]] = (int32_t)c;
     gbgpu_csr_t m;
-    if (build_csr(F, &R, cols, (int32_t)nq, 0, &m, &rp, &ci, &rcf, &cp, &cf)) goto done;
+    if (build_csr(F, R, (int32_t)nq, 0, &m)) goto done;
     if ((rc = run_la(F, &m, 1, &out)) != 0) goto done;
     rc = GBGPU_ENOMEM;
     /* the reduced basis replaces the old one: the rows whose lead no
@@ -1029,8 +1041,7 @@ static int f4_reduce(f4_t *F) {
     rc = GBGPU_OK;
 done:
     gbgpu_rows_free(&out);
-    free(R.r); free(R.mon); free(queue); free(cols);
-    free(rp); free(ci); free(rcf); free(cp); free(cf);
+    free(queue); free(cols);
     return rc;
 }
 
@@ -1174,16 +1185,27 @@ int gbgpu_f4(const gbgpu_f4_in_t *in, gbgpu_f4_la_t la, void *user, gbgpu_f4_out
         out->bexp = (int32_t *)malloc((size_t)(nterms ? nterms : 1) * (size_t)in->nr_vars * sizeof(int32_t));
         out->bcf = (uint32_t *)malloc((size_t)(nterms ? nterms : 1) * sizeof(uint32_t));
         if (!out->blen || !out->bexp || !out->bcf) { free(lm); gbgpu_f4_out_free(out); goto done; }
+        int64_t *toff = (int64_t *)malloc(nl * sizeof(int64_t));
+        if (!toff) { free(lm); gbgpu_f4_out_free(out); goto done; }
         int64_t t = 0;
         for (int32_t q = 0; q < F.B.nlm; ++q) {
+            toff[q] = t;
+            out->blen[q] = F.B.e[lm[q]].len;
+            t += out->blen[q];
+        }
+        const int nv = in->nr_vars;
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int32_t q = 0; q < F.B.nlm; ++q) {
             const poly_t *f = &F.B.e[lm[q]];
-            out->blen[q] = f->len;
-            for (int32_t a = 0; a < f->len; ++a, ++t) {
-                out->bcf[t] = f->c[a];
+            uint32_t *oc = out->bcf + toff[q];
+            int32_t *oe = out->bexp + toff[q] * nv;
+            for (int32_t a = 0; a < f->len; ++a) {
+                oc[a] = f->c[a];
                 const uint8_t *e = EV(&F.T, f->m[a]);
-                for (int i = 0; i < in->nr_vars; ++i) out->bexp[t * in->nr_vars + i] = e[i];
+                for (int i = 0; i < nv; ++i) oe[(int64_t)a * nv + i] = e[i];
             }
         }
+        free(toff);
         free(lm);
     }
     rc = GBGPU_OK;
@@ -1195,6 +1217,9 @@ done:
     free(F.B.e); free(F.B.red); free(F.B.lm);
     free(F.P.p);
     free(F.M.stamp); free(F.M.col); free(F.M.piv);
+    free(F.R.r); free(F.R.mon);
+    free(F.W.rp); free(F.W.ci); free(F.W.rcf); free(F.W.eid); free(F.W.cp); free(F.W.cf);
+    free(F.W.key);
     mt_free(&F.T);
     out->total_ms = now_ms() - t_start;
     return rc;

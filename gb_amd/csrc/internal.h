@@ -141,9 +141,9 @@ struct Digits {
     int nd = 0;            // 0: no digit planes (plain form)
     uint32_t hi = 0;       // 127 S
     int32_t k0 = 0;        // first chunk an update reads
-    // exact reduction of the digit products in double precision: p, 1 / p,
-    // 2^32 mod p and 2^48 mod p
-    double pd = 0, pinv = 0, w32 = 0, w48 = 0;
+    uint64_t off = 0;      // multiple of p above 2^57: makes a combined sum positive
+    uint64_t off2 = 0;     // multiple of p above 2^41 (the high part for nd = 4)
+    uint32_t w32 = 0;      // 2^32 mod p
 };
 
 __host__ inline int digits_for(uint32_t p) {
@@ -155,10 +155,9 @@ __host__ inline void digits_setup(Digits &g, uint32_t p, int nd) {
     uint64_t S = 0, pw = 1;
     for (int i = 0; i < nd; ++i, pw *= 256) S += pw;
     g.hi = (uint32_t)(127 * S);
-    g.pd = (double)p;
-    g.pinv = 1.0 / (double)p;
-    g.w32 = (double)((1ull << 32) % p);
-    g.w48 = (double)((1ull << 48) % p);
+    g.w32 = (uint32_t)((1ull << 32) % p);
+    g.off = ((1ull << 57) / p + 1) * p;
+    g.off2 = ((1ull << 41) / p + 1) * p;
 }
 
 __device__ __forceinline__ uint32_t digit_word(uint32_t x, uint32_t p, uint32_t hi, int i) {
@@ -174,47 +173,28 @@ __device__ __forceinline__ uint32_t digit_word(uint32_t x, uint32_t p, uint32_t 
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-// x - sum_s Cs 256^s as a double that is exact and congruent to it modulo
-// p.  Magnitudes (256 slots): nd = 2 below 2^39; nd = 3 below 2^48, with C4
-// entering as C4 (2^32 mod p); nd = 4: C0..C3 below 2^49, and
-// hi = C4 + 2^8 C5 + 2^16 C6 (below 2^39) enters as h = hi mod p split in
-// 16-bit halves, h0 (2^32 mod p) + h1 (2^48 mod p), each below 2^47.  Every
-// partial sum is an integer below 2^53, so each step is exact.
+// sum_s Cs 256^s mod p for result element e of the accumulators.  Over up
+// to 256 pivot slots |Cs| <= min(s + 1, 2 nd - 1 - s) 256 128^2 < 2^24, so
+// for nd <= 3 the shifted sum (C4 enters as C4 2^32) is below 2^55 in size
+// and dg.off (a multiple of p above 2^57) makes it positive; for nd = 4 the
+// terms s >= 4 form hi = C4 + 2^8 C5 + 2^16 C6 (below 2^39), reduced with
+// dg.off2 and brought in as (hi mod p) (2^32 mod p).  One reduction at the
+// end.
 template <int ND>
-__device__ __forceinline__ double digits_residual(uint32_t x, const v4i *acc, int e,
-                                                  const Digits &dg) {
-    double d = (double)x - (double)acc[0][e];
-    if constexpr (ND >= 2) {
-        d = fma(-256.0, (double)acc[1][e], d);
-        d = fma(-65536.0, (double)acc[2][e], d);
-    }
-    if constexpr (ND == 3) {
-        d = fma(-16777216.0, (double)acc[3][e], d);
-        d = fma(-dg.w32, (double)acc[4][e], d);
-    }
+__device__ __forceinline__ uint32_t digits_combine(const v4i *acc, int e, const Mod &md,
+                                                   const Digits &dg) {
+    int64_t lo = 0;
+#pragma unroll
+    for (int q = 0; q < (2 * ND - 1 < 4 ? 2 * ND - 1 : 4); ++q) lo += (int64_t)acc[q][e] << (8 * q);
+    if constexpr (ND == 3) lo += (int64_t)acc[4][e] << 32;
     if constexpr (ND == 4) {
-        d = fma(-16777216.0, (double)acc[3][e], d);
-        double hi = (double)acc[4][e];
-        hi = fma(256.0, (double)acc[5][e], hi);
-        hi = fma(65536.0, (double)acc[6][e], hi);
-        double h = fma(-rint(hi * dg.pinv), dg.pd, hi);
-        h = h < 0.0 ? h + dg.pd : h;
-        const double h1 = floor(h * (1.0 / 65536.0));
-        const double h0 = fma(-65536.0, h1, h);
-        d = fma(-dg.w32, h0, d);
-        d = fma(-dg.w48, h1, d);
+        int64_t hi = 0;
+#pragma unroll
+        for (int q = 4; q < 2 * ND - 1; ++q) hi += (int64_t)acc[q][e] << (8 * (q - 4));
+        const uint32_t hr = md.reduce((uint64_t)(hi + (int64_t)dg.off2));
+        lo += (int64_t)md.mul(hr, dg.w32);
     }
-    return d;
-}
-
-// d mod p for an integer-valued double |d| < 2^53: q = rint(d / p) is
-// within one of the nearest quotient (the product d (1 / p) is off by less
-// than 1 / (2 p)), so d - q p lies in [-(p + 1) / 2, (p + 1) / 2] and one
-// correction lands in [0, p)
-__device__ __forceinline__ uint32_t reduce_f64(double d, const Digits &dg) {
-    double r = fma(-rint(d * dg.pinv), dg.pd, d);
-    r = r < 0.0 ? r + dg.pd : r;
-    return (uint32_t)r;
+    return md.reduce((uint64_t)(lo + (int64_t)dg.off));
 }
 
 // the int8-digit MFMA update A[r][cb, ce) -= L[r] Rp (dense.hip) over kc

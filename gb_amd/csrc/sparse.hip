@@ -851,7 +851,9 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
     if (grid > m->nrl) grid = m->nrl > 0 ? m->nrl : 1;
     uint32_t *xs = nullptr;
     int32_t *queue = nullptr;
-    if ((rc = dalloc(ctx, &xs, (size_t)grid * (size_t)(m->ncl > 0 ? m->ncl : 1)))) { dfree(tinv); return rc; }
+    // per-wave multipliers of the row-parallel kernels (the dense route has none)
+    const size_t nxs = dense ? 1 : (size_t)grid * (size_t)(m->ncl > 0 ? m->ncl : 1);
+    if ((rc = dalloc(ctx, &xs, nxs))) { dfree(tinv); return rc; }
     if ((rc = dalloc(ctx, &queue, 6))) { dfree(xs); dfree(tinv); return rc; }
     GBGPU_HIP(ctx, hipMemsetAsync(queue, 0, 6 * sizeof(int32_t), s));
     SpArgs a;
