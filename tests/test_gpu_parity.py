@@ -45,12 +45,15 @@ def test_sparse_pass_many_windows(gpu_lib, p, form):
     assert np.array_equal(got, exp)
 
 
-@pytest.mark.parametrize("p", [251, 65521, 131071, 2147483647])
-@pytest.mark.parametrize("shape", [(300, 420, 200), (1500, 2600, 900)])
+@pytest.mark.parametrize("p", [251, 65521, 131071, 16777213, 2147483647])
+@pytest.mark.parametrize("shape", [(300, 420, 200), (1500, 2600, 900), (620, 900, 600),
+                                   (1000, 1200, 700)])
 def test_sparse_pass_dense_route(gpu_lib, p, shape):
     """Pivot rows carrying ~10 % of the columns: the MFMA form reduces on
     the matrix cores (blocked elimination on a dense copy); same D' rows,
-    same order, as the oracle, and the same la_option 1 result."""
+    same order, as the oracle, and the same la_option 1 result.  Shapes:
+    one partial panel; several panels; 20 rows to reduce; a last panel of
+    three groups, the third partial."""
     nr, nc, npiv = shape
     m = _f4(nr, nc, npiv, p, seed=7, piv_len=(nc // 30, nc // 5), piv_mu=np.log(nc / 10),
             red_nnz=nc // 10)
