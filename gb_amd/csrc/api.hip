@@ -207,7 +207,8 @@ static int check_csr_header(gbgpu_ctx *ctx, const gbgpu_csr_t *m) {
     return GBGPU_OK;
 }
 
-// host-side validation of the few things the kernels index with
+// host-side validation of the row structure the kernels index with (the
+// entries themselves are checked on the device right after the copy)
 static int check_csr_body(gbgpu_ctx *ctx, const gbgpu_csr_t *m) {
     if (m->nr > 0 && m->rp[0] != 0) return set_error(ctx, GBGPU_EINVAL, "rp[0] must be 0");
     for (int32_t i = 0; i < m->nr; ++i) {
@@ -217,13 +218,21 @@ static int check_csr_body(gbgpu_ctx *ctx, const gbgpu_csr_t *m) {
         if (m->cp[id + 1] - m->cp[id] < m->rp[i + 1] - m->rp[i])
             return set_error(ctx, GBGPU_EINVAL, "coefficient array shorter than its row");
     }
-    const int64_t nnz = m->nr > 0 ? m->rp[m->nr] : 0;
-    for (int64_t k = 0; k < nnz; ++k)
-        if (m->ci[k] < 0 || m->ci[k] >= m->nc) return set_error(ctx, GBGPU_EINVAL, "column index out of range");
-    const int64_t npool = m->ncf > 0 ? m->cp[m->ncf] : 0;
-    for (int64_t k = 0; k < npool; ++k)
-        if (m->cf[k] >= m->p) return set_error(ctx, GBGPU_EINVAL, "coefficient not reduced mod p");
+    for (int32_t i = 0; i < m->ncf; ++i)
+        if (m->cp[i + 1] < m->cp[i]) return set_error(ctx, GBGPU_EINVAL, "cp must not decrease");
     return GBGPU_OK;
+}
+
+// flag |= 1 for a column index outside [0, nc), |= 2 for a coefficient >= p
+__global__ void k_check_entries(int64_t nnz, const int32_t *ci, int32_t nc, int64_t npool,
+                                const uint32_t *cf, uint32_t p, int32_t *flag) {
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    int bad = 0;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += step)
+        if (ci[k] < 0 || ci[k] >= nc) bad |= 1;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < npool; k += step)
+        if (cf[k] >= p) bad |= 2;
+    if (bad) atomicOr(flag, bad);
 }
 
 int gbgpu_mat_upload(gbgpu_ctx *ctx, const gbgpu_csr_t *h, gbgpu_mat **out) {
@@ -256,10 +265,29 @@ int gbgpu_mat_upload(gbgpu_ctx *ctx, const gbgpu_csr_t *h, gbgpu_mat **out) {
         e = hipMemcpyAsync(m->cp, h->cp, ((size_t)m->ncf + 1) * 8, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipMemcpyAsync(m->cf, h->cf, (size_t)m->npool * 4, hipMemcpyHostToDevice, s);
     }
+    // every column index and coefficient checked before any kernel uses them
+    int32_t *flag = nullptr;
+    int32_t hflag = 0;
+    if (e == hipSuccess && (rc = dalloc(ctx, &flag, 1))) {
+        delete m;
+        return rc;
+    }
+    if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(int32_t), s);
+    if (e == hipSuccess && (m->nnz > 0 || m->npool > 0)) {
+        k_check_entries<<<1024, 256, 0, s>>>(m->nnz, m->ci, m->nc, m->npool, m->cf, m->p, flag);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&hflag, flag, sizeof(int32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(flag);
     if (e != hipSuccess) {
         delete m;
         return check_hip(ctx, e, "upload");
+    }
+    if (hflag) {
+        delete m;
+        return set_error(ctx, GBGPU_EINVAL, (hflag & 1) ? "column index out of range"
+                                                        : "coefficient not reduced mod p");
     }
     *out = m;
     return GBGPU_OK;

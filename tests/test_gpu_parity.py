@@ -192,7 +192,7 @@ def test_contract_violation_is_an_error(gpu_lib):
                    ci=np.array([0, 9], np.int32), rcf=np.zeros(1, np.int32),
                    cp=np.array([0, 2], np.int64), cf=np.array([1, 5], np.uint32))
     with pytest.raises(GpuError):
-        gpu_lib.upload(bad)  # column out of range is rejected before any kernel
+        gpu_lib.upload(bad)  # column out of range: rejected by the upload, before any compute kernel
 
 
 def test_shared_coefficient_arrays(gpu_lib):
