@@ -569,8 +569,8 @@ __global__ __launch_bounds__(256) void k_update_plain(uint32_t *a, int64_t ld, i
 // A block tile of 64 x 64: four waves of 32 x 32 (RS x CS products of
 // 16 x 16), over KC chunks of 64 slots from chunk dg.k0.  The operands are
 // 16-byte loads of the digit planes (L2-resident); used for K = 64.
-template <int ND, int KC, int RS = 2, int CS = 2>
-__device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t m, int32_t cb,
+template <int ND, int KC, int RS = 2, int CS = 2, typename T = uint32_t>
+__device__ __forceinline__ void update_i8_body(T *a, int64_t ld, int32_t m, int32_t cb,
                                                int32_t ce, int32_t c0, const Panel *pan,
                                                const int32_t *active, Mod md, const Digits &dg,
                                                int32_t bx, int32_t by) {
@@ -653,19 +653,19 @@ __device__ __forceinline__ void update_i8_body(uint32_t *a, int64_t ld, int32_t 
             for (int cs = 0; cs < CS; ++cs) {
                 const int32_t col = q0 + 16 * cs + lr;
                 if (col >= ce) continue;
-                uint32_t *x = a + (int64_t)row * ld + col;
-                const uint32_t xo = PRE ? xv[rs][e][cs] : *x;
-                *x = sub_mod(xo, digits_combine<ND>(acc[rs][cs], e, md, dg), md.p);
+                T *x = a + (int64_t)row * ld + col;
+                const uint32_t xo = PRE ? xv[rs][e][cs] : (uint32_t)*x;
+                *x = (T)sub_mod(xo, digits_combine<ND>(acc[rs][cs], e, md, dg), md.p);
             }
         }
 }
 
-template <int ND, int KC = 1, int RS = 2, int CS = 2>
-__global__ __launch_bounds__(256) void k_update_mfma(uint32_t *a, int64_t ld, int32_t m,
+template <int ND, int KC = 1, int RS = 2, int CS = 2, typename T = uint32_t>
+__global__ __launch_bounds__(256) void k_update_mfma(T *a, int64_t ld, int32_t m,
                                                      int32_t cb, int32_t ce, int32_t c0,
                                                      const Panel *pan, const int32_t *active,
                                                      Mod md, Digits dg) {
-    update_i8_body<ND, KC, RS, CS>(a, ld, m, cb, ce, c0, pan, active, md, dg, blockIdx.x,
+    update_i8_body<ND, KC, RS, CS, T>(a, ld, m, cb, ce, c0, pan, active, md, dg, blockIdx.x,
                                    blockIdx.y);
 }
 
@@ -679,8 +679,8 @@ __global__ __launch_bounds__(256) void k_update_mfma(uint32_t *a, int64_t ld, in
 constexpr int WT_NT = 16;   // column tiles per block
 constexpr int WT_CS = 2;    // 16-column products per tile
 
-template <int ND, int RS>
-__global__ __launch_bounds__(256) void k_update_wide(uint32_t *a, int64_t ld, int32_t m,
+template <int ND, int RS, typename T = uint32_t>
+__global__ __launch_bounds__(256) void k_update_wide(T *a, int64_t ld, int32_t m,
                                                      int32_t cb, int32_t ce, int32_t c0,
                                                      const Panel *pan, const int32_t *active,
                                                      Mod md, Digits dg) {
@@ -727,7 +727,7 @@ __global__ __launch_bounds__(256) void k_update_wide(uint32_t *a, int64_t ld, in
         }
     }
     bool xa[RS][4];
-    uint32_t *xp[RS][4];
+    T *xp[RS][4];
 #pragma unroll
     for (int rs = 0; rs < RS; ++rs)
 #pragma unroll
@@ -748,7 +748,7 @@ __global__ __launch_bounds__(256) void k_update_wide(uint32_t *a, int64_t ld, in
 #pragma unroll
                     for (int cs = 0; cs < CS; ++cs) {
                         const int32_t col = q0 + 16 * cs;
-                        x[rs][e][cs] = xp[rs][e][col < clast ? col : clast];
+                        x[rs][e][cs] = (uint32_t)xp[rs][e][col < clast ? col : clast];
                     }
                 }
     };
@@ -810,11 +810,11 @@ __global__ __launch_bounds__(256) void k_update_wide(uint32_t *a, int64_t ld, in
                         v[cs] = sub_mod(xv[rs][e][cs], digits_combine<ND>(acc[rs][cs], e, md, dg), md.p);
                     if (full) {
 #pragma unroll
-                        for (int cs = 0; cs < CS; ++cs) xp[rs][e][q0 + 16 * cs] = v[cs];
+                        for (int cs = 0; cs < CS; ++cs) xp[rs][e][q0 + 16 * cs] = (T)v[cs];
                     } else {
 #pragma unroll
                         for (int cs = 0; cs < CS; ++cs)
-                            if (q0 + 16 * cs + lr < ce) xp[rs][e][q0 + 16 * cs] = v[cs];
+                            if (q0 + 16 * cs + lr < ce) xp[rs][e][q0 + 16 * cs] = (T)v[cs];
                     }
                 }
         }
@@ -833,31 +833,40 @@ __global__ __launch_bounds__(256) void k_update_wide(uint32_t *a, int64_t ld, in
     }
 }
 
-void launch_update_i8(int nd, int kc, hipStream_t s, uint32_t *a, int64_t ld, int32_t m,
-                      int32_t cb, int32_t ce, int32_t c0, const Panel *pan,
-                      const int32_t *active, Mod md, const Digits &dg) {
+template <typename T>
+void launch_update_i8(int nd, int kc, hipStream_t s, T *a, int64_t ld, int32_t m, int32_t cb,
+                      int32_t ce, int32_t c0, const Panel *pan, const int32_t *active, Mod md,
+                      const Digits &dg) {
     if (cb >= ce || m <= 0) return;
+    // 16-bit storage holds residues of p <= 65536: one or two digits
+    constexpr bool W16 = sizeof(T) == 2;
     if (kc == 4) {
         // 64 rows per block for nd <= 2 (32 per wave), 32 rows for nd >= 3
         const int rows = nd <= 2 ? 128 : 64;
         const int cols = WT_NT * 16 * WT_CS;
         dim3 grid((unsigned)((ce - cb + cols - 1) / cols), (unsigned)((m + rows - 1) / rows));
-        switch (nd) {
-            case 1: k_update_wide<1, 2><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
-            case 2: k_update_wide<2, 2><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
-            case 3: k_update_wide<3, 1><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
-            default: k_update_wide<4, 1><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
+        if (nd == 1) k_update_wide<1, 2, T><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg);
+        else if (nd == 2 || W16) k_update_wide<2, 2, T><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg);
+        else if constexpr (!W16) {
+            if (nd == 3) k_update_wide<3, 1><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg);
+            else k_update_wide<4, 1><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg);
         }
         return;
     }
     dim3 grid((unsigned)((ce - cb + 63) / 64), (unsigned)((m + 63) / 64));
-    switch (nd) {
-        case 1: k_update_mfma<1><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
-        case 2: k_update_mfma<2><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
-        case 3: k_update_mfma<3><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
-        default: k_update_mfma<4><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg); break;
+    if (nd == 1) k_update_mfma<1, 1, 2, 2, T><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg);
+    else if (nd == 2 || W16) k_update_mfma<2, 1, 2, 2, T><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg);
+    else if constexpr (!W16) {
+        if (nd == 3) k_update_mfma<3><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg);
+        else k_update_mfma<4><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg);
     }
 }
+template void launch_update_i8<uint32_t>(int, int, hipStream_t, uint32_t *, int64_t, int32_t,
+                                         int32_t, int32_t, int32_t, const Panel *, const int32_t *,
+                                         Mod, const Digits &);
+template void launch_update_i8<uint16_t>(int, int, hipStream_t, uint16_t *, int64_t, int32_t,
+                                         int32_t, int32_t, int32_t, const Panel *, const int32_t *,
+                                         Mod, const Digits &);
 
 // ------------------------------------------------------------ output
 

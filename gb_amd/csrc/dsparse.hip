@@ -30,17 +30,18 @@ namespace gbgpu {
 constexpr int GB = 64;  // pivot columns per group
 
 // M[q][c] = entry of to-reduce row upiv[q] at column c (M zeroed before)
+template <typename T>
 __global__ void k_ds_scatter(int32_t nrl, const int32_t *upiv, const int64_t *rp,
                              const int32_t *ci, const int32_t *rcf, const int64_t *cp,
-                             const uint32_t *cf, uint32_t *M, int64_t ldm) {
+                             const uint32_t *cf, T *M, int64_t ldm) {
     const int lane = threadIdx.x & 63;
     const int32_t step = gridDim.x * (blockDim.x >> 6);
     for (int32_t q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < nrl; q += step) {
         const int32_t row = upiv[q];
         const int64_t b = rp[row], e = rp[row + 1];
         const uint32_t *c = cf + cp[rcf[row]] - b;
-        uint32_t *out = M + (int64_t)q * ldm;
-        for (int64_t k = b + lane; k < e; k += 64) out[ci[k]] = c[k];
+        T *out = M + (int64_t)q * ldm;
+        for (int64_t k = b + lane; k < e; k += 64) out[ci[k]] = (T)c[k];
     }
 }
 
@@ -131,9 +132,9 @@ __global__ void k_ds_tdigits(int32_t ngroups, const uint32_t *tinv, int8_t *td, 
 // as the digits of chunk a of L; act[q] = row has a nonzero multiplier,
 // any[q] = OR over the panel's groups.  The digits of v are formed in
 // registers from M.
-template <int ND>
+template <int ND, typename T>
 __global__ __launch_bounds__(256) void k_ds_mult(int32_t nrl, int32_t ca, int32_t wa, int a,
-                                                 const uint32_t *M, int64_t ldm, const int8_t *td,
+                                                 const T *M, int64_t ldm, const int8_t *td,
                                                  int32_t *act, int32_t *any, Mod md, Digits dg) {
     const uint32_t p = md.p;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(256) void k_ds_mult(int32_t nrl, int32_t ca, int32_
     // A operand: row r0 + lr, slots 16 lk .. 16 lk + 15 (columns past ncl,
     // from wa on, count as zero; they lie inside the row: ca + 64 <= ldm)
     const int32_t row = r0 + lr;
-    const uint32_t *src = M + (int64_t)(row < nrl ? row : nrl - 1) * ldm + ca + 16 * lk;
+    const T *src = M + (int64_t)(row < nrl ? row : nrl - 1) * ldm + ca + 16 * lk;
     v4i A[ND];
     bool nz = false;
 #pragma unroll
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(256) void k_ds_mult(int32_t nrl, int32_t ca, int32_
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int k = 16 * lk + 4 * w4 + u;
-            x[u] = (row < nrl && k < wa) ? src[4 * w4 + u] : 0u;
+            x[u] = (row < nrl && k < wa) ? (uint32_t)src[4 * w4 + u] : 0u;
             nz = nz || x[u] != 0u;
         }
         digits4(x, p, dg.hi, ND, d);
@@ -201,17 +202,18 @@ __global__ __launch_bounds__(256) void k_ds_mult(int32_t nrl, int32_t ca, int32_
 }
 
 // D'[q] = M[q][ncl .. nc), dnz[q] = row nonzero
-__global__ void k_ds_extract(int32_t nrl, int32_t ncl, int32_t nc, const uint32_t *M,
+template <typename T>
+__global__ void k_ds_extract(int32_t nrl, int32_t ncl, int32_t nc, const T *M,
                              int64_t ldm, uint32_t *dprime, int32_t *dnz) {
     const int lane = threadIdx.x & 63;
     const int32_t ncr = nc - ncl;
     const int32_t step = gridDim.x * (blockDim.x >> 6);
     for (int32_t q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < nrl; q += step) {
-        const uint32_t *src = M + (int64_t)q * ldm + ncl;
+        const T *src = M + (int64_t)q * ldm + ncl;
         uint32_t *dst = dprime + (int64_t)q * ncr;
         bool nz = false;
         for (int32_t c = lane; c < ncr; c += 64) {
-            const uint32_t v = src[c];
+            const uint32_t v = src[c];  // widened from T
             dst[c] = v;
             nz = nz || v != 0u;
         }
@@ -235,7 +237,10 @@ bool dense_route_pays(const gbgpu_mat *m) {
     return mean_len * 40.0 > (double)m->nc;
 }
 
-int run_dense_route(gbgpu_mat *m, const uint32_t *tinv) {
+// M holds residues in T: uint16_t for p <= 65536, which halves the bytes
+// every update moves, else uint32_t
+template <typename T>
+static int dense_route(gbgpu_mat *m, const uint32_t *tinv) {
     gbgpu_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
     constexpr int KCH = 4;               // groups per panel
@@ -248,7 +253,8 @@ int run_dense_route(gbgpu_mat *m, const uint32_t *tinv) {
     Digits dg;
     digits_setup(dg, m->p, digits_for(m->p));
     const int nd = dg.nd;
-    uint32_t *M = nullptr, *P = nullptr;
+    T *M = nullptr;
+    uint32_t *P = nullptr;
     int8_t *dl = nullptr, *dr = nullptr, *td = nullptr;
     int32_t *active = nullptr, *act = nullptr, *pk = nullptr;  // pk: a panel record whose k is 64
     int rc;
@@ -272,7 +278,7 @@ int run_dense_route(gbgpu_mat *m, const uint32_t *tinv) {
     dg.rchunk = ldr * 64;
     const int32_t k64 = GB;
     GBGPU_HIP(ctx, hipMemcpyAsync(pk, &k64, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    GBGPU_HIP(ctx, hipMemsetAsync(M, 0, (size_t)nrl * ldm * sizeof(uint32_t), s));
+    GBGPU_HIP(ctx, hipMemsetAsync(M, 0, (size_t)nrl * ldm * sizeof(T), s));
     k_ds_scatter<<<grid_rows(nrl, 4), 256, 0, s>>>(nrl, m->upiv, m->rp, m->ci, m->rcf, m->cp, m->cf,
                                                     M, ldm);
     k_ds_tdigits<<<grid_rows((int64_t)ngroups * GB * GB, 256), 256, 0, s>>>(ngroups, tinv, td, m->p, dg);
@@ -293,11 +299,11 @@ int run_dense_route(gbgpu_mat *m, const uint32_t *tinv) {
             const int32_t ca = g0 + a * GB;
             const int32_t wa = (bw - a * GB) < GB ? (bw - a * GB) : GB;
             const int8_t *tg = td + (int64_t)(g0 / GB + a) * nd * GB * GB;
-            switch (nd) {
-                case 1: k_ds_mult<1><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg); break;
-                case 2: k_ds_mult<2><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg); break;
-                case 3: k_ds_mult<3><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg); break;
-                default: k_ds_mult<4><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg); break;
+            if (nd == 1) k_ds_mult<1, T><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg);
+            else if (nd == 2 || sizeof(T) == 2) k_ds_mult<2, T><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg);
+            else if constexpr (sizeof(T) == 4) {
+                if (nd == 3) k_ds_mult<3, T><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg);
+                else k_ds_mult<4, T><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg);
             }
             const int32_t nb0 = ca + GB, nb1 = g0 + bw;
             if (nb0 < nb1) {
@@ -316,6 +322,10 @@ int run_dense_route(gbgpu_mat *m, const uint32_t *tinv) {
     GBGPU_HIP(ctx, hipGetLastError());
     cleanup();
     return GBGPU_OK;
+}
+
+int run_dense_route(gbgpu_mat *m, const uint32_t *tinv) {
+    return m->p <= 65536u ? dense_route<uint16_t>(m, tinv) : dense_route<uint32_t>(m, tinv);
 }
 
 }  // namespace gbgpu

@@ -201,10 +201,12 @@ __device__ __forceinline__ uint32_t digits_combine(const v4i *acc, int e, const 
 // chunks of 64 pivot slots (kc = 1 or 4) from chunk dg.k0: L and Rp^T as
 // the digit planes of dg (Rp^T indexed from column c0), rows with
 // active[r].  kc = 4 takes the kernel that walks column tiles with the
-// block's Rp digits staged in LDS.
-void launch_update_i8(int nd, int kc, hipStream_t s, uint32_t *a, int64_t ld, int32_t m,
-                      int32_t cb, int32_t ce, int32_t c0, const Panel *pan,
-                      const int32_t *active, Mod md, const Digits &dg);
+// block's Rp digits staged in LDS.  A is uint32_t, or uint16_t for
+// p <= 65536 (nd <= 2).
+template <typename T>
+void launch_update_i8(int nd, int kc, hipStream_t s, T *a, int64_t ld, int32_t m, int32_t cb,
+                      int32_t ce, int32_t c0, const Panel *pan, const int32_t *active, Mod md,
+                      const Digits &dg);
 
 // dsparse.hip: the sparse pass as blocked elimination on the matrix cores
 bool dense_route_pays(const gbgpu_mat *m);
