@@ -655,7 +655,7 @@ __device__ __forceinline__ void update_i8_body(T *a, int64_t ld, int32_t m, int3
                 if (col >= ce) continue;
                 T *x = a + (int64_t)row * ld + col;
                 const uint32_t xo = PRE ? xv[rs][e][cs] : (uint32_t)*x;
-                *x = (T)sub_mod(xo, digits_combine<ND>(acc[rs][cs], e, md, dg), md.p);
+                *x = (T)digits_sub<ND>(xo, acc[rs][cs], e, md, dg);
             }
         }
 }
@@ -807,7 +807,7 @@ __global__ __launch_bounds__(256) void k_update_wide(T *a, int64_t ld, int32_t m
                     uint32_t v[CS];
 #pragma unroll
                     for (int cs = 0; cs < CS; ++cs)
-                        v[cs] = sub_mod(xv[rs][e][cs], digits_combine<ND>(acc[rs][cs], e, md, dg), md.p);
+                        v[cs] = digits_sub<ND>(xv[rs][e][cs], acc[rs][cs], e, md, dg);
                     if (full) {
 #pragma unroll
                         for (int cs = 0; cs < CS; ++cs) xp[rs][e][q0 + 16 * cs] = (T)v[cs];
@@ -838,7 +838,7 @@ void launch_update_i8(int nd, int kc, hipStream_t s, T *a, int64_t ld, int32_t m
                       int32_t ce, int32_t c0, const Panel *pan, const int32_t *active, Mod md,
                       const Digits &dg) {
     if (cb >= ce || m <= 0) return;
-    // 16-bit storage holds residues of p <= 65536: one or two digits
+    // 16-bit storage holds residues of p <= 65521: one or two digits
     constexpr bool W16 = sizeof(T) == 2;
     if (kc == 4) {
         // 64 rows per block for nd <= 2 (32 per wave), 32 rows for nd >= 3

@@ -186,7 +186,8 @@ __global__ __launch_bounds__(256) void k_ds_mult(int32_t nrl, int32_t ca, int32_
         const int f = ((ball >> rl) & 0x0001000100010001ull) != 0 ? 1 : 0;
 #pragma unroll
         for (int cs = 0; cs < 4; ++cs) {
-            const uint32_t x = digits_combine<ND>(acc[cs], e, md, dg);
+            const uint32_t r = digits_sub<ND>(0u, acc[cs], e, md, dg);  // -x mod p
+            const uint32_t x = r == 0u ? 0u : p - r;
             int64_t v = x <= dg.hi ? (int64_t)x : (int64_t)x - (int64_t)p;
 #pragma unroll
             for (int i = 0; i < ND; ++i) {
@@ -237,8 +238,8 @@ bool dense_route_pays(const gbgpu_mat *m) {
     return mean_len * 40.0 > (double)m->nc;
 }
 
-// M holds residues in T: uint16_t for p <= 65536, which halves the bytes
-// every update moves, else uint32_t
+// M holds residues in T: uint16_t for p <= 65521 (one or two digits), which
+// halves the bytes every update moves, else uint32_t
 template <typename T>
 static int dense_route(gbgpu_mat *m, const uint32_t *tinv) {
     gbgpu_ctx *ctx = m->ctx;
@@ -325,7 +326,8 @@ static int dense_route(gbgpu_mat *m, const uint32_t *tinv) {
 }
 
 int run_dense_route(gbgpu_mat *m, const uint32_t *tinv) {
-    return m->p <= 65536u ? dense_route<uint16_t>(m, tinv) : dense_route<uint32_t>(m, tinv);
+    // one or two digits: p <= 65521, residues fit 16 bits
+    return digits_for(m->p) <= 2 ? dense_route<uint16_t>(m, tinv) : dense_route<uint32_t>(m, tinv);
 }
 
 }  // namespace gbgpu

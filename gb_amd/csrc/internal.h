@@ -125,7 +125,7 @@ struct Panel;
 // [-128, 127] the integers 127 S .. -128 S are reachable (S = 1 + 256 + ...
 // + 256^(nd-1)), a window of 256^nd consecutive integers; so for
 // p <= 256^nd every residue x has a representative in it: x itself when
-// x <= 127 S, else x - p.  nd = 1 for p <= 255, 2 for p <= 65536, 3 for
+// x <= 127 S, else x - p.  nd = 1 for p <= 255, 2 for p <= 65521, 3 for
 // p <= 2^24, 4 below 2^31.  A plane holds one digit of L (rows) or of Rp
 // transposed (columns) in chunks of 64 pivot slots, chunk-major:
 // [chunk][row or column][64 bytes], so that one lane's 16 consecutive
@@ -144,10 +144,13 @@ struct Digits {
     uint64_t off = 0;      // multiple of p above 2^57: makes a combined sum positive
     uint64_t off2 = 0;     // multiple of p above 2^41 (the high part for nd = 4)
     uint32_t w32 = 0;      // 2^32 mod p
+    float pinvf = 0;       // 1 / p in single precision (digits_balanced2, nd = 2)
 };
 
+// nd = 2 stops at 65521, the largest prime below 2^16, where
+// digits_balanced2 is exact; the composite moduli above it take nd = 3
 __host__ inline int digits_for(uint32_t p) {
-    return p <= 255u ? 1 : p <= 65536u ? 2 : p <= (1u << 24) ? 3 : 4;
+    return p <= 255u ? 1 : p <= 65521u ? 2 : p <= (1u << 24) ? 3 : 4;
 }
 
 __host__ inline void digits_setup(Digits &g, uint32_t p, int nd) {
@@ -158,6 +161,7 @@ __host__ inline void digits_setup(Digits &g, uint32_t p, int nd) {
     g.w32 = (uint32_t)((1ull << 32) % p);
     g.off = ((1ull << 57) / p + 1) * p;
     g.off2 = ((1ull << 41) / p + 1) * p;
+    g.pinvf = 1.0f / (float)p;
 }
 
 __device__ __forceinline__ uint32_t digit_word(uint32_t x, uint32_t p, uint32_t hi, int i) {
@@ -197,12 +201,50 @@ __device__ __forceinline__ uint32_t digits_combine(const v4i *acc, int e, const 
     return md.reduce((uint64_t)(lo + (int64_t)dg.off));
 }
 
+// C0 + 2^8 C1 + 2^16 C2 (nd = 2, up to 256 pivot slots: |C0|, |C2| <= 2^22,
+// |C1| <= 2^23) modulo p for 256 <= p <= 65521, as a residue in
+// [-(p/2 + 2), p/2 + 2], with single-precision quotients and 24-bit
+// multiplies (all full rate) instead of a 64-bit reduction.  Horner in base
+// 256: a = C2, then a = 256 a' + C1, then a = 256 a' + C0, where a' is a
+// reduced by q = rint(a / p).  Every a is below 2^24 in size, so it is exact
+// in single precision: |a'| <= p/2 + 2 <= 32762 gives |256 a'| <= 8387072.
+// The computed quotient is within 2^-23 |a / p| <= 2 / p of a / p, so
+// |a - q p| <= p/2 + 2.  The 24-bit products q p are exact (|q| < 2^17).
+__device__ __forceinline__ int32_t digits_balanced2(const v4i *acc, int e, int32_t p, float pinv) {
+    int32_t a = acc[2][e];
+    int32_t q = (int32_t)__builtin_rintf((float)a * pinv);
+    a -= __mul24(q, p);
+    a = a * 256 + acc[1][e];
+    q = (int32_t)__builtin_rintf((float)a * pinv);
+    a -= __mul24(q, p);
+    a = a * 256 + acc[0][e];
+    q = (int32_t)__builtin_rintf((float)a * pinv);
+    return a - __mul24(q, p);
+}
+
+// (x - combined digit products) mod p for x in [0, p): the single-precision
+// form for nd = 2 (256 <= p <= 65521), the 64-bit combination otherwise
+template <int ND>
+__device__ __forceinline__ uint32_t digits_sub(uint32_t x, const v4i *acc, int e, const Mod &md,
+                                               const Digits &dg) {
+    if constexpr (ND == 2) {
+        const int32_t ip = (int32_t)md.p;
+        int32_t r = (int32_t)x - digits_balanced2(acc, e, ip, dg.pinvf);
+        r += r < 0 ? ip : 0;
+        r -= r >= ip ? ip : 0;
+        return (uint32_t)r;
+    } else {
+        const uint32_t u = digits_combine<ND>(acc, e, md, dg);
+        return x >= u ? x - u : x + md.p - u;
+    }
+}
+
 // the int8-digit MFMA update A[r][cb, ce) -= L[r] Rp (dense.hip) over kc
 // chunks of 64 pivot slots (kc = 1 or 4) from chunk dg.k0: L and Rp^T as
 // the digit planes of dg (Rp^T indexed from column c0), rows with
 // active[r].  kc = 4 takes the kernel that walks column tiles with the
 // block's Rp digits staged in LDS.  A is uint32_t, or uint16_t for
-// p <= 65536 (nd <= 2).
+// p <= 65521 (nd <= 2).
 template <typename T>
 void launch_update_i8(int nd, int kc, hipStream_t s, T *a, int64_t ld, int32_t m, int32_t cb,
                       int32_t ce, int32_t c0, const Panel *pan, const int32_t *active, Mod md,
