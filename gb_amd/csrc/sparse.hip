@@ -805,6 +805,11 @@ int batch_sparse_launch(gbgpu_ctx *ctx, const BlockedArgs *mats, const int32_t *
         }                                                                                        \
     } while (0)
     switch (wshift) {
+        case 10:  // blocked form only (the row-parallel kernels use 2048 .. 8192)
+            if (form != GBGPU_FORM_MFMA) return set_error(ctx, GBGPU_EINVAL, "window too narrow for the plain form");
+            if (big) k_sparse_blocked_batch<10, true><<<grid, 64, 0, s>>>(mats, row_off, nmat, queue, xs, xs_stride);
+            else k_sparse_blocked_batch<10, false><<<grid, 64, 0, s>>>(mats, row_off, nmat, queue, xs, xs_stride);
+            break;
         case 11: GBGPU_BATCH_SP(11); break;
         case 12: GBGPU_BATCH_SP(12); break;
         default: GBGPU_BATCH_SP(13); break;
