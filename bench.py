@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), each scaled by what the same
 # counter reads for a plain 256 MiB device copy; the commands and raw counts
 # are in profiles/k_sparse_blocked_f4_20000x30000_p65521.md
-PMC_TRAFFIC_BYTES = 3.11e9 + 0.770e9
+PMC_TRAFFIC_BYTES = 13.40e9 + 0.770e9
 
 
 def _sync():
