@@ -342,7 +342,7 @@ __device__ __forceinline__ void prune_compact_body(int32_t *cand, ListState *st,
     if (nzero < PB || done) return;
     const int32_t head = __builtin_amdgcn_readfirstlane(st->head);
     const int32_t count = __builtin_amdgcn_readfirstlane(st->count);
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x, wv = tid >> 6;
     int32_t out = head;
     // stable in-place compaction: writes never pass the reads of later chunks
     for (int32_t base = head; base < count; base += 256) {
@@ -510,7 +510,7 @@ __device__ __forceinline__ void update_plain_body(uint32_t *a, int64_t ld, int32
         const int rr = i / PB, t = i % PB;
         Ls[rr][t] = (r0 + rr < m) ? L[(int64_t)(r0 + rr) * PB + t] : 0u;
     }
-    if (threadIdx.x < RT) act[threadIdx.x] = (r0 + threadIdx.x < m) ? active[r0 + threadIdx.x] : 0;
+    if ((int)threadIdx.x < RT) act[threadIdx.x] = (r0 + (int32_t)threadIdx.x < m) ? active[r0 + threadIdx.x] : 0;
     __syncthreads();
     int anyact = 0;
 #pragma unroll
@@ -1083,7 +1083,7 @@ __device__ __forceinline__ int tab_find(const int32_t *pre, int32_t nmat, int32_
 __global__ __launch_bounds__(256) void k_batch_dense_init(const DenseDesc *D) {
     __shared__ int32_t wsum[4];
     const DenseDesc &d = D[blockIdx.x];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x, wv = tid >> 6;
     const int32_t m = d.m, n = d.n;
     for (int32_t i = tid; i < m; i += 256) d.rowpiv[i] = -1;
     for (int32_t i = tid; i < n; i += 256) d.colpiv[i] = -1;

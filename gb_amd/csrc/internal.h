@@ -126,8 +126,10 @@ struct Panel;
 // + 256^(nd-1)), a window of 256^nd consecutive integers; so for
 // p <= 256^nd every residue x has a representative in it: x itself when
 // x <= 127 S, else x - p.  nd = 1 for p <= 255, 2 for p <= 65521, 3 for
-// p <= 2^24, 4 below 2^31.  A plane holds one digit of L (rows) or of Rp
-// transposed (columns) in chunks of 64 pivot slots, chunk-major:
+// p <= 2^24, 4 below 2^31.  A plane holds one digit of the left operand L
+// (rows) or of the right operand transposed (columns: Rp in the dense echelon
+// form, the raw pivot rows P in the sparse pass's dense route) in chunks of
+// 64 pivot slots, chunk-major:
 // [chunk][row or column][64 bytes], so that one lane's 16 consecutive
 // bytes are an MFMA operand and 16 rows or columns of a chunk are 1 KiB
 // of consecutive memory.
@@ -239,11 +241,11 @@ __device__ __forceinline__ uint32_t digits_sub(uint32_t x, const v4i *acc, int e
     }
 }
 
-// the int8-digit MFMA update A[r][cb, ce) -= L[r] Rp (dense.hip) over kc
-// chunks of 64 pivot slots (kc = 1 or 4) from chunk dg.k0: L and Rp^T as
-// the digit planes of dg (Rp^T indexed from column c0), rows with
+// the int8-digit MFMA update A[r][cb, ce) -= L[r] R (dense.hip) over kc
+// chunks of 64 pivot slots (kc = 1 or 4) from chunk dg.k0: L and R^T as
+// the digit planes of dg (R^T indexed from column c0), rows with
 // active[r].  kc = 4 takes the kernel that walks column tiles with the
-// block's Rp digits staged in LDS.  A is uint32_t, or uint16_t for
+// block's R digits staged in LDS.  A is uint32_t, or uint16_t for
 // p <= 65521 (nd <= 2).
 template <typename T>
 void launch_update_i8(int nd, int kc, hipStream_t s, T *a, int64_t ld, int32_t m, int32_t cb,
