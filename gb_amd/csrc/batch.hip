@@ -19,9 +19,9 @@
 //     synchronisation inside.
 //
 // The launch parameters of the sparse pass (window width 2^10 .. 2^13
-// columns, 2^10 for the blocked form only, and waves per CU) are chosen by
-// timing the candidates on the batch itself the first time a batch of that
-// shape runs in a form; the choice is kept per process.
+// columns and waves per CU) are chosen by timing the candidates on the batch
+// itself the first time a batch of that shape runs in a form; the choice is
+// kept per process.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -175,8 +175,7 @@ int tune(gbgpu_batch *b, int form, int *best_ws, int *best_wpc) {
     for (auto *m : b->mats) max_nc = std::max(max_nc, m->nc);
     float best = -1;
     int rc;
-    // 1024-column windows exist for the blocked form only
-    for (int ws = form == GBGPU_FORM_MFMA ? 10 : 11; ws <= 13; ++ws) {
+    for (int ws = 10; ws <= 13; ++ws) {
         if ((((int64_t)max_nc + (1 << ws) - 1) >> ws) > 63) continue;
         if ((rc = layout_sparse(b, form, ws))) return rc;
         const int mw = max_waves(ws, form);

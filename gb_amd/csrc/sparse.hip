@@ -184,7 +184,8 @@ int prep_buckets(gbgpu_mat *m, int force_wshift) {
     dfree(m->seg);
     dfree(m->bucket);
     dfree(m->pbits);
-    // smallest window (2048 .. 8192 columns) that keeps nwin <= 63
+    // unless a width is forced: the smallest window (2048 .. 8192 columns)
+    // that keeps nwin <= 63
     int wshift = 11;
     while (((int64_t)m->nc + (1 << wshift) - 1) >> wshift > 63 && wshift < 13) ++wshift;
     if (force_wshift) wshift = force_wshift;
@@ -805,11 +806,7 @@ int batch_sparse_launch(gbgpu_ctx *ctx, const BlockedArgs *mats, const int32_t *
         }                                                                                        \
     } while (0)
     switch (wshift) {
-        case 10:  // blocked form only (the row-parallel kernels use 2048 .. 8192)
-            if (form != GBGPU_FORM_MFMA) return set_error(ctx, GBGPU_EINVAL, "window too narrow for the plain form");
-            if (big) k_sparse_blocked_batch<10, true><<<grid, 64, 0, s>>>(mats, row_off, nmat, queue, xs, xs_stride);
-            else k_sparse_blocked_batch<10, false><<<grid, 64, 0, s>>>(mats, row_off, nmat, queue, xs, xs_stride);
-            break;
+        case 10: GBGPU_BATCH_SP(10); break;
         case 11: GBGPU_BATCH_SP(11); break;
         case 12: GBGPU_BATCH_SP(12); break;
         default: GBGPU_BATCH_SP(13); break;
@@ -829,12 +826,12 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
     hipStream_t s = ctx->stream;
     int rc;
     if ((rc = prep_split(m, m->ncl))) return rc;
-    // the blocked kernel keeps one uint64 per window column in LDS: windows of
-    // 1024 columns (when at most 63 of them cover the matrix) fit twice the
-    // waves per CU of 2048-column windows and measured 12 % faster on a
-    // 20000 x 30000 matrix; the row-parallel kernels keep 2048 .. 8192
+    // the kernels keep one uint64 per window column in LDS: windows of 1024
+    // columns (when at most 63 of them cover the matrix) fit more waves per
+    // CU than 2048-column ones (blocked: 16 against 9, 12 % faster on a
+    // 20000 x 30000 matrix)
     int wshift = 0;
-    if (form == GBGPU_FORM_MFMA && work == nullptr && ((int64_t)m->nc + 1023) / 1024 <= 63) wshift = 10;
+    if (((int64_t)m->nc + 1023) / 1024 <= 63) wshift = 10;
     if ((rc = prep_buckets(m, wshift))) return rc;
     const int32_t ncr = m->nc - m->ncl;
     dfree(m->dprime); dfree(m->dnz); dfree(m->dlist);
@@ -887,6 +884,7 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
         }
     } else if (m->nrl > 0) {
         switch (m->wshift) {
+            case 10: launch_sparse<10>(a, big, work != nullptr, grid, s); break;
             case 11: launch_sparse<11>(a, big, work != nullptr, grid, s); break;
             case 12: launch_sparse<12>(a, big, work != nullptr, grid, s); break;
             default: launch_sparse<13>(a, big, work != nullptr, grid, s); break;

@@ -37,7 +37,7 @@ def test_batch_mixed_primes(gpu_lib, form):
             assert b.rows(i).equals(exp[i])
     info = b.info()
     assert info["replays"] == 3 and info["matrices"] == len(mats)
-    assert (10 if form == FORM_MFMA else 11) <= info["window_shift"] <= 13
+    assert 10 <= info["window_shift"] <= 13
 
 
 def _all_zero_matrix(p, seed):
