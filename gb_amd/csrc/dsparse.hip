@@ -46,10 +46,13 @@ __global__ void k_ds_scatter(int32_t nrl, const int32_t *upiv, const int64_t *rp
 }
 
 // P[s][c - g0] = entry of the pivot of column g0 + s at column c >= g0, for
-// s < rows (zero rows for columns without a pivot); P zeroed before
+// s < rows (zero rows for columns without a pivot); P zeroed before.  Row s
+// is shared by the gridDim.y waves with that blockIdx.x lane group, each
+// taking every gridDim.y-th run of 64 entries.
+template <typename T>
 __global__ void k_ds_group_rows(int32_t g0, int32_t ncl, int rows, const int32_t *first,
                                 const int64_t *rp, const int32_t *ci, const int32_t *rcf,
-                                const int64_t *cp, const uint32_t *cf, uint32_t *P, int64_t ldp) {
+                                const int64_t *cp, const uint32_t *cf, T *P, int64_t ldp) {
     const int lane = threadIdx.x & 63;
     const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (s >= rows) return;
@@ -59,8 +62,9 @@ __global__ void k_ds_group_rows(int32_t g0, int32_t ncl, int rows, const int32_t
     if (row == INT32_MAX) return;
     const int64_t b = rp[row], e = rp[row + 1];
     const uint32_t *c = cf + cp[rcf[row]] - b;
-    uint32_t *out = P + (int64_t)s * ldp - g0;
-    for (int64_t k = b + lane; k < e; k += 64) out[ci[k]] = c[k];
+    T *out = P + (int64_t)s * ldp - g0;
+    const int64_t step = 64 * (int64_t)gridDim.y;
+    for (int64_t k = b + 64 * (int64_t)blockIdx.y + lane; k < e; k += step) out[ci[k]] = (T)c[k];
 }
 
 // the nd signed digits of x's representative (x or x - p), packed four to a
@@ -85,9 +89,10 @@ __device__ __forceinline__ void digits4(const uint32_t (&x)[4], uint32_t p, uint
 // (coalesced across the wave), and the digits go through LDS (17 words per
 // column and chunk, so neither the writes nor the reads collide) to leave as
 // 4 KiB of consecutive bytes per digit and chunk.
-__global__ __launch_bounds__(256) void k_ds_pdigits(const uint32_t *P, int64_t ldp, int rows,
+template <typename T, int NDL>
+__global__ __launch_bounds__(256) void k_ds_pdigits(const T *P, int64_t ldp, int rows,
                                                     int32_t w, uint32_t p, Digits dg) {
-    __shared__ uint32_t t4[4 * 4 * 64 * 17];  // [digit][chunk][column][17 words]
+    __shared__ uint32_t t4[NDL * 4 * 64 * 17];  // [digit][chunk][column][17 words], nd <= NDL
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int32_t cb = (int32_t)blockIdx.x * 64;
     const int32_t c = cb + lane;
@@ -96,7 +101,7 @@ __global__ __launch_bounds__(256) void k_ds_pdigits(const uint32_t *P, int64_t l
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int t = 4 * q + u;
-            x[u] = (t < rows && c < w) ? P[(int64_t)t * ldp + c] : 0u;
+            x[u] = (t < rows && c < w) ? (uint32_t)P[(int64_t)t * ldp + c] : 0u;
         }
         digits4(x, p, dg.hi, dg.nd, d);
         const int kc = q >> 4, wd = q & 15;
@@ -238,8 +243,8 @@ bool dense_route_pays(const gbgpu_mat *m) {
     return mean_len * 40.0 > (double)m->nc;
 }
 
-// M holds residues in T: uint16_t for p <= 65521 (one or two digits), which
-// halves the bytes every update moves, else uint32_t
+// M and the panel rows P hold residues in T: uint16_t for p <= 65521 (one or
+// two digits), which halves the bytes every update moves, else uint32_t
 template <typename T>
 static int dense_route(gbgpu_mat *m, const uint32_t *tinv) {
     gbgpu_ctx *ctx = m->ctx;
@@ -254,8 +259,7 @@ static int dense_route(gbgpu_mat *m, const uint32_t *tinv) {
     Digits dg;
     digits_setup(dg, m->p, digits_for(m->p));
     const int nd = dg.nd;
-    T *M = nullptr;
-    uint32_t *P = nullptr;
+    T *M = nullptr, *P = nullptr;
     int8_t *dl = nullptr, *dr = nullptr, *td = nullptr;
     int32_t *active = nullptr, *act = nullptr, *pk = nullptr;  // pk: a panel record whose k is 64
     int rc;
@@ -289,10 +293,11 @@ static int dense_route(gbgpu_mat *m, const uint32_t *tinv) {
         const int32_t bw = (ncl - g0) < PW ? (ncl - g0) : PW;  // pivot columns of the panel
         const int na = (bw + GB - 1) / GB;                     // its groups
         const int32_t w = nc - g0;                             // columns from the panel on
-        GBGPU_HIP(ctx, hipMemsetAsync(P, 0, (size_t)PW * ldp * sizeof(uint32_t), s));
-        k_ds_group_rows<<<PW / 4, 256, 0, s>>>(g0, ncl, PW, m->first, m->rp, m->ci, m->rcf, m->cp,
+        GBGPU_HIP(ctx, hipMemsetAsync(P, 0, (size_t)PW * ldp * sizeof(T), s));
+        k_ds_group_rows<<<dim3(PW / 4, 8), 256, 0, s>>>(g0, ncl, PW, m->first, m->rp, m->ci, m->rcf, m->cp,
                                                m->cf, P, ldp);
-        k_ds_pdigits<<<(w + 63) / 64, 256, 0, s>>>(P, ldp, na * GB, w, m->p, dg);
+        if (nd <= 2) k_ds_pdigits<T, 2><<<(w + 63) / 64, 256, 0, s>>>(P, ldp, na * GB, w, m->p, dg);
+        else k_ds_pdigits<T, 4><<<(w + 63) / 64, 256, 0, s>>>(P, ldp, na * GB, w, m->p, dg);
         // the panel's groups in order: group a's multipliers clear its
         // columns; its rows update the panel's later columns at once
         // (K = 64) ...

@@ -40,6 +40,22 @@ def test_batch_mixed_primes(gpu_lib, form):
     assert 10 <= info["window_shift"] <= 13
 
 
+@pytest.mark.parametrize("primes", [[65521, 32003, 257, 40009, 251], [251, 241, 239, 2]],
+                         ids=["two_digits", "one_digit"])
+@pytest.mark.parametrize("form", FORMS)
+def test_batch_small_primes(gpu_lib, primes, form):
+    """The batch's digit count follows its largest prime: two digits (the
+    single-precision reduction of the products) and one digit."""
+    mats = _small_batch(primes, 9)
+    exp = [oracle.la_sparse_dense(m) for m in mats]
+    b = gpu_lib.batch(mats)
+    for _ in range(2):
+        npv = b.la(form)
+        for i in range(len(mats)):
+            assert npv[i] == exp[i].np
+            assert b.rows(i).equals(exp[i])
+
+
 def _all_zero_matrix(p, seed):
     rng = np.random.default_rng(seed)
     nc, ncl = 80, 30
