@@ -138,13 +138,13 @@ __global__ void k_ds_tdigits(int32_t ngroups, const uint32_t *tinv, int8_t *td, 
 // any[q] = OR over the panel's groups.  The digits of v are formed in
 // registers from M.
 template <int ND, typename T>
-__global__ __launch_bounds__(256) void k_ds_mult(int32_t nrl, int32_t ca, int32_t wa, int a,
+__global__ __launch_bounds__(64) void k_ds_mult(int32_t nrl, int32_t ca, int32_t wa, int a,
                                                  const T *M, int64_t ldm, const int8_t *td,
                                                  int32_t *act, int32_t *any, Mod md, Digits dg) {
     const uint32_t p = md.p;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
     const int lr = lane & 15, lk = lane >> 4;
-    const int32_t r0 = ((int32_t)blockIdx.x * 4 + wv) * 16;
+    const int32_t r0 = (int32_t)blockIdx.x * 16;  // one wave per block, 16 rows
     if (r0 >= nrl) return;  // wave-uniform
     // A operand: row r0 + lr, slots 16 lk .. 16 lk + 15 (columns past ncl,
     // from wa on, count as zero; they lie inside the row: ca + 64 <= ldm)
@@ -288,7 +288,7 @@ static int dense_route(gbgpu_mat *m, const uint32_t *tinv) {
                                                     M, ldm);
     k_ds_tdigits<<<grid_rows((int64_t)ngroups * GB * GB, 256), 256, 0, s>>>(ngroups, tinv, td, m->p, dg);
     const Panel *pan = (const Panel *)pk;
-    const int mult_blocks = (nrl + 63) / 64;
+    const int mult_blocks = (nrl + 15) / 16;
     for (int32_t g0 = 0; g0 < ncl; g0 += PW) {
         const int32_t bw = (ncl - g0) < PW ? (ncl - g0) : PW;  // pivot columns of the panel
         const int na = (bw + GB - 1) / GB;                     // its groups
@@ -305,11 +305,11 @@ static int dense_route(gbgpu_mat *m, const uint32_t *tinv) {
             const int32_t ca = g0 + a * GB;
             const int32_t wa = (bw - a * GB) < GB ? (bw - a * GB) : GB;
             const int8_t *tg = td + (int64_t)(g0 / GB + a) * nd * GB * GB;
-            if (nd == 1) k_ds_mult<1, T><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg);
-            else if (nd == 2 || sizeof(T) == 2) k_ds_mult<2, T><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg);
+            if (nd == 1) k_ds_mult<1, T><<<mult_blocks, 64, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg);
+            else if (nd == 2 || sizeof(T) == 2) k_ds_mult<2, T><<<mult_blocks, 64, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg);
             else if constexpr (sizeof(T) == 4) {
-                if (nd == 3) k_ds_mult<3, T><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg);
-                else k_ds_mult<4, T><<<mult_blocks, 256, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg);
+                if (nd == 3) k_ds_mult<3, T><<<mult_blocks, 64, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg);
+                else k_ds_mult<4, T><<<mult_blocks, 64, 0, s>>>(nrl, ca, wa, a, M, ldm, tg, act, active, md, dg);
             }
             const int32_t nb0 = ca + GB, nb1 = g0 + bw;
             if (nb0 < nb1) {
