@@ -71,52 +71,75 @@ __device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p) 
 
 // ------------------------------------------------------------ select
 //
-// One workgroup finds the panel's pivots.  Candidate rows (rows that are no
-// pivot yet, kept in the list cand[head, count) in any order) are taken up
-// to 64 at a time into an LDS tile; every tile row carries [its 64 panel
-// entries | its transform over the selected rows] and its own coefficient.
-// The tile is first reduced by the pivot rows already found (E, normalised
-// and reduced within the panel).  Then, repeatedly, the leftmost column in
-// which an unselected tile row is nonzero is the next pivot column: the
-// row with pivot value a there joins E, and the column is cleared from
-// every other row x of the tile and of E without division,
-// x <- a x - x[col] * pivot row; so no inverse sits on the per-pivot path.
-// At the end of a tile the E rows are normalised, their inverses computed
-// in parallel.  Tile rows that are never selected end up zero on the
-// panel.  Updates touch only what can be nonzero: panel columns >= the
-// pivot column and transform slots <= k.  The selected rows' transform T
-// gives E = T * A[selected rows, panel], so Rp = T * A[selected rows, c0:n]
-// is in reduced form with an identity on the panel's pivot columns.
-// Selected rows are swapped to the front of the candidate list.
+// One workgroup finds the panel's pivots.  An LDS array of PB slots holds
+// rows of [64 panel entries | transform over the slots]: the pivot rows
+// found so far (E) and a tile of candidate rows (rows that are no pivot yet,
+// kept in the list cand[head, count) in any order) loaded into the free
+// slots in list order, each with a unit transform.  The tile is first
+// reduced by E (normalised and reduced within the panel).  Then, repeatedly,
+// the leftmost column in which an unselected tile row is nonzero is the next
+// pivot column, the lowest slot nonzero there gives the pivot row, with
+// pivot value a, and every other slot x is replaced by a x - x[col] * pivot
+// row, without division: no inverse sits on the per-pivot path, the pivot
+// row becomes a row of E where it is, and the column is cleared everywhere
+// else.  At the end of a tile the E rows are normalised, their inverses
+// computed in parallel.  Tile rows that are never selected end up zero on
+// the panel.  A row's transform only ever takes in pivot rows, so the
+// transform T of E over the selected rows gives E = T * A[selected rows,
+// panel], and Rp = T * A[selected rows, c0:n] is in reduced form with an
+// identity on the panel's pivot columns.  Selected rows are swapped to the
+// front of the candidate list.
 
-constexpr int XW = 2 * PB + 1;  // padded tile row: panel part, transform part
+constexpr int XW = 2 * PB + 1;  // padded slot row: panel part, transform part
 
-// wave 0: the smallest first-nonzero column over the lanes (tile rows) and
-// the lowest lane holding it; every lane stores the same two values
+// wave 0: the smallest first-nonzero column over the lanes (slots) and the
+// lowest lane holding it; every lane stores the same two values
 __device__ __forceinline__ void pivot_search(int fc, int32_t &sh_js, int32_t &sh_is) {
-    int mn = fc;
+    // fc <= PB has 7 bits: the minimum is found bit by bit from the top, each
+    // ballot keeping the lanes that can still hold it
+    uint64_t live = ~0ull;
+    int mn = 0;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) mn = min(mn, __shfl_xor(mn, d, 64));
-    const uint64_t bal = __ballot(fc == mn);
+    for (int b = 6; b >= 0; --b) {
+        const uint64_t z = live & (uint64_t)__ballot(((fc >> b) & 1) == 0);
+        if (z) live = z;
+        else mn |= 1 << b;
+    }
     sh_js = mn;
-    sh_is = __ffsll((unsigned long long)bal) - 1;
+    sh_is = __ffsll((unsigned long long)live) - 1;
+}
+
+// a x + b y mod p for residues of p < 2^16: both products are below 2^32
+// (24-bit multiplies), the quotient of their sum by p is estimated in single
+// precision with an absolute error below 0.05, so the remainder computed in
+// wrapped 32-bit arithmetic lies in [-p, 2p) before the two corrections
+__device__ __forceinline__ uint32_t mad2_mod16(uint32_t a, uint32_t x, uint32_t b, uint32_t y,
+                                               uint32_t p, float pinv) {
+    const uint32_t s = __umul24(a, x), t = __umul24(b, y);
+    const uint32_t q = (uint32_t)(((float)s + (float)t) * pinv);
+    int32_t r = (int32_t)(s + t - __umul24(q, p));
+    r += r < 0 ? (int32_t)p : 0;
+    r -= r >= (int32_t)p ? (int32_t)p : 0;
+    return (uint32_t)r;
 }
 constexpr int SEL_T = 1024;     // threads of the select workgroup
 constexpr int SEL_RG = SEL_T / (2 * PB);  // row groups: thread = (row group, column)
+constexpr int SEL_RS = PB / SEL_RG;       // slots per thread
 
 __device__ __forceinline__ void select_body(const uint32_t *a, int64_t ld, int32_t n, int32_t c0,
                                             int32_t *cand, int32_t *rowpiv, int32_t *colpiv,
                                             Panel *pan, ListState *st, Mod md) {
-    __shared__ uint32_t E[PB][XW];
-    __shared__ uint32_t C[PB][XW];
+    __shared__ uint32_t X[PB][XW];
     __shared__ uint32_t G[PB][PB + 1];
-    __shared__ uint32_t own[PB], einv[PB];
-    __shared__ int32_t ecol[PB], erow[PB], epos[PB], crow[PB], used[PB];
+    __shared__ uint32_t einv[PB];
+    __shared__ int32_t ecol[PB], eslot[PB], erow[PB], epos[PB], srow[PB], spos[PB];
     __shared__ int32_t sh_js, sh_is;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int col = tid & (2 * PB - 1), rg = tid >> 7;  // this thread's column and row group
     const uint32_t p = md.p;
     const bool small = p < (1u << 17);  // 64 products of residues fit in 64 bits
+    const bool p16 = p < (1u << 16);    // mad2_mod16 applies
+    const float pinv = 1.0f / (float)p;
     const int32_t done = __builtin_amdgcn_readfirstlane(st->done);
     if (done) {
         if (tid == 0) pan->k = 0;
@@ -128,53 +151,67 @@ __device__ __forceinline__ void select_body(const uint32_t *a, int64_t ld, int32
     int kk = 0;  // pivots found (same value in every thread)
     int nzero = 0;
     int32_t pos = head;
+    uint64_t pmask = 0;  // slots holding E rows (same value in every thread)
+    for (int s = rg; s < PB; s += SEL_RG) X[s][col] = 0u;
     while (pos < count && kk < bw) {
-        const int nc = min(PB, count - pos);
         const int k0 = kk;  // pivots found before this tile
-        if (tid < PB) {
-            crow[tid] = tid < nc ? cand[pos + tid] : 0;
-            used[tid] = tid < nc ? 0 : 1;
-            own[tid] = 1u;
+        const uint64_t freem = ~pmask;
+        const int nc = min(PB - kk, count - pos);
+        // the tile takes the nc lowest free slots
+        uint64_t tmask = freem;
+        if (nc < PB - kk) {
+            tmask = 0;
+            uint64_t f = freem;
+            for (int j = 0; j < nc; ++j) {
+                const uint64_t low = f & (~f + 1ull);
+                tmask |= low;
+                f ^= low;
+            }
+        }
+        if (tid < PB && ((tmask >> tid) & 1ull)) {
+            const int rk = __popcll(freem & ((1ull << tid) - 1ull));
+            srow[tid] = cand[pos + rk];
+            spos[tid] = pos + rk;
         }
         __syncthreads();
-        // 1. load the tile (transform part zero)
-        for (int i = rg; i < PB; i += SEL_RG) {
-            uint32_t v = 0;
-            if (i < nc && col < bw) v = a[(int64_t)crow[i] * ld + c0 + col];
-            C[i][col] = v;
+        // 1. load the tile: panel entries, unit transform
+        for (int s = rg; s < PB; s += SEL_RG) {
+            if (!((tmask >> s) & 1ull)) continue;
+            uint32_t v;
+            if (col < PB) v = col < bw ? a[(int64_t)srow[s] * ld + c0 + col] : 0u;
+            else v = (col - PB == s) ? 1u : 0u;
+            X[s][col] = v;
         }
         __syncthreads();
         // 2. reduce the tile by E (normalised)
         if (kk > 0) {
-            for (int idx = tid; idx < nc * PB; idx += SEL_T) {
-                const int i = idx >> 6, t = idx & 63;
-                G[i][t] = t < kk ? C[i][ecol[t] - c0] : 0u;
+            for (int idx = tid; idx < PB * PB; idx += SEL_T) {
+                const int s = idx >> 6, t = idx & 63;
+                G[s][t] = (t < kk && ((tmask >> s) & 1ull)) ? X[s][ecol[t] - c0] : 0u;
             }
             __syncthreads();
-            for (int i = rg; i < nc; i += SEL_RG) {
-                uint32_t red;
+            for (int s = rg; s < PB; s += SEL_RG) {
+                if (!((tmask >> s) & 1ull)) continue;
+                uint64_t acc = 0;
                 if (small) {
-                    uint64_t acc = 0;
-                    for (int t = 0; t < kk; ++t) acc += (uint64_t)G[i][t] * E[t][col];
-                    red = md.reduce(acc);
+                    for (int t = 0; t < kk; ++t) acc += (uint64_t)G[s][t] * X[eslot[t]][col];
                 } else {
-                    uint64_t acc = 0;
-                    for (int t = 0; t < kk; ++t) acc += (uint64_t)md.mul(G[i][t], E[t][col]);
-                    red = md.reduce(acc);
+                    for (int t = 0; t < kk; ++t) acc += (uint64_t)md.mul(G[s][t], X[eslot[t]][col]);
                 }
-                C[i][col] = sub_mod(C[i][col], red, p);
+                X[s][col] = sub_mod(X[s][col], md.reduce(acc), p);
             }
             __syncthreads();
         }
         // 3. pivots of the tile, leftmost column first.  Wave 0 keeps, per
-        // lane i, the first nonzero panel column of tile row i (PB: none or
-        // selected); columns left of the last pivot are zero in every
-        // unselected row, so only rows hit by an elimination rescan.
+        // lane s, the first nonzero panel column of tile slot s (PB: none, or
+        // not an unselected tile row); columns left of the last pivot are
+        // zero in every unselected row, so only rows hit by an elimination
+        // rescan.
         int fc = PB;
         if (wv == 0) {
-            if (lane < nc) {
+            if ((tmask >> lane) & 1ull) {
                 for (int c = 0; c < bw; ++c)
-                    if (C[lane][c] != 0u) { fc = c; break; }
+                    if (X[lane][c] != 0u) { fc = c; break; }
             }
             pivot_search(fc, sh_js, sh_is);
         }
@@ -187,61 +224,50 @@ __device__ __forceinline__ void select_body(const uint32_t *a, int64_t ld, int32
                 break;
             }
             const int is = __builtin_amdgcn_readfirstlane(sh_is);
-            const uint32_t av = C[is][js];
-            const uint32_t ownv = own[is];
-            // x <- a x - f pivot for the rows with f = x[js] != 0 (column js
-            // itself is cleared in the next phase, so every thread reads f
-            // before anything changes it): E rows over all panel columns
-            // (the scaling by a reaches their columns left of js) and tile
-            // rows over columns >= js (zero further left); transform slots
-            // <= kk, where the pivot row's own coefficient sits in slot kk.
-            {
-                const bool tcol = col >= PB;
-                const int sl = col - PB;
-                const bool slot_ok = tcol && sl <= kk;
-                const uint32_t pv = tcol ? (sl == kk ? ownv : (sl < kk ? C[is][col] : 0u))
-                                         : (col < bw ? C[is][col] : 0u);
-                const bool e_ok = (col < bw || slot_ok) && col != js;
-                const bool t_ok = ((col >= js && col < bw) || slot_ok) && col != js;
-                for (int r = rg; r < kk + nc; r += SEL_RG) {
-                    const bool isE = r < kk;
-                    const int rt = r - kk;
-                    uint32_t *x = isE ? &E[r][col] : &C[rt][col];
-                    const uint32_t f = isE ? E[r][js] : ((rt != is && !used[rt]) ? C[rt][js] : 0u);
-                    if (f == 0u || !(isE ? e_ok : t_ok)) continue;
+            const uint32_t av = X[is][js];
+            const uint32_t pv = X[is][col];
+            // x <- av x - x[js] * pivot row in every slot but the pivot's,
+            // whatever it holds (the scaling by av is harmless: E rows are
+            // normalised at the end of the tile, tile rows are either
+            // selected and normalised or end up zero, free slots are
+            // reloaded).  Column js comes out zero; it is written in the
+            // next phase, so every thread reads its x[js] before that.
+            uint32_t fv[SEL_RS], xv[SEL_RS];
+#pragma unroll
+            for (int i = 0; i < SEL_RS; ++i) {
+                const int r = rg + SEL_RG * i;
+                fv[i] = X[r][js];
+                xv[i] = X[r][col];
+            }
+            if (col != js) {
+#pragma unroll
+                for (int i = 0; i < SEL_RS; ++i) {
+                    const int r = rg + SEL_RG * i;
+                    if (r == is) continue;
+                    const uint32_t nf = fv[i] ? p - fv[i] : 0u;
                     // two products below 2^62 each: the sum fits in 64 bits
-                    *x = md.reduce((uint64_t)av * *x + (uint64_t)(p - f) * pv);
+                    X[r][col] = p16 ? mad2_mod16(av, xv[i], nf, pv, p, pinv)
+                                    : md.reduce((uint64_t)av * xv[i] + (uint64_t)nf * pv);
                 }
-                if (tid < nc && tid != is && !used[tid] && C[tid][js] != 0u)
-                    own[tid] = md.mul(av, own[tid]);
             }
             __syncthreads();
             // the pivot row joins E; column js is cleared in the other rows;
             // wave 0 finds the next pivot (rows hit by this elimination
             // rescan from js + 1, which this phase does not write)
-            if (tid < 2 * PB) {
-                const int sl = tid - PB;
-                E[kk][tid] = (tid >= PB && sl == kk) ? ownv : C[is][tid];
+            if (tid < PB && tid != is) X[tid][js] = 0u;
+            if (tid == 0) {
+                ecol[kk] = c0 + js;
+                eslot[kk] = is;
             }
-            if (tid < kk) E[tid][js] = 0u;
-            if (tid >= PB && tid < PB + nc) {
-                const int rt = tid - PB;
-                if (rt != is) C[rt][js] = 0u;
-            }
-            // same values from every thread
-            ecol[kk] = c0 + js;
-            erow[kk] = crow[is];
-            epos[kk] = pos + is;
-            used[is] = 1;
+            pmask |= 1ull << is;
+            tmask &= ~(1ull << is);
             ++kk;
             if (wv == 0) {
-                if (lane < nc) {
-                    if (lane == is) fc = PB;
-                    else if (fc == js) {
-                        fc = PB;
-                        for (int c = js + 1; c < bw; ++c)
-                            if (C[lane][c] != 0u) { fc = c; break; }
-                    }
+                if (lane == is) fc = PB;
+                else if (fc == js) {
+                    fc = PB;
+                    for (int c = js + 1; c < bw; ++c)
+                        if (X[lane][c] != 0u) { fc = c; break; }
                 }
                 pivot_search(fc, sh_js, sh_is);
             }
@@ -250,28 +276,36 @@ __device__ __forceinline__ void select_body(const uint32_t *a, int64_t ld, int32
         }
         // 4. normalise E by its pivot values (inverses in parallel)
         if (kk > k0) {
-            if (tid < kk) einv[tid] = inv_mod(E[tid][ecol[tid] - c0], p);
+            if (tid < kk) einv[tid] = inv_mod(X[eslot[tid]][ecol[tid] - c0], p);
             __syncthreads();
-            for (int t = rg; t < kk; t += SEL_RG) E[t][col] = md.mul(E[t][col], einv[t]);
+            for (int t = rg; t < kk; t += SEL_RG) {
+                const int s = eslot[t];
+                X[s][col] = md.mul(X[s][col], einv[t]);
+            }
             __syncthreads();
         }
         if (exhausted) {
             // rows of this tile that were not selected are zero on the panel
-            nzero += nc - (kk - k0);
+            nzero += __popcll(tmask);
         }
         pos += nc;
     }
     __syncthreads();
     if (tid < kk) {
-        rowpiv[erow[tid]] = ecol[tid];
-        colpiv[ecol[tid]] = erow[tid];
-        pan->erow[tid] = erow[tid];
+        const int s = eslot[tid];
+        const int32_t r = srow[s];
+        erow[tid] = r;
+        epos[tid] = spos[s];
+        rowpiv[r] = ecol[tid];
+        colpiv[ecol[tid]] = r;
+        pan->erow[tid] = r;
         pan->ecol[tid] = ecol[tid];
     }
     for (int i = tid; i < PB * PB; i += SEL_T) {
         const int t = i >> 6, sl = i & 63;
-        pan->T[i] = (t < kk && sl < kk) ? E[t][PB + sl] : 0u;
+        pan->T[i] = (t < kk && sl < kk) ? X[eslot[t]][PB + eslot[sl]] : 0u;
     }
+    __syncthreads();
     if (tid == 0) {
         // move the selected rows to cand[head, head + kk): an unselected row
         // from the front takes the place of each selected row further back
