@@ -486,10 +486,67 @@ __device__ __forceinline__ void rp_body(uint32_t *a, int64_t ld, int32_t n, int3
     }
 }
 
-__global__ __launch_bounds__(256) void k_panel_rp(uint32_t *a, int64_t ld, int32_t n, int32_t c0,
-                                                  const Panel *pan, uint32_t *rp, int64_t ldr,
-                                                  Mod md, uint32_t r64, Digits dg) {
-    rp_body(a, ld, n, c0, pan, rp, ldr, md, r64, blockIdx.x, dg);
+// The same product over the columns [jlo, jhi) (jlo >= c0) in narrower
+// blocks: a block covers RPC columns and every row t (a thread: one column,
+// four rows), so the blocks that read a column of the selected rows are the
+// ones that write it, and a launch over a few columns still has PB / 4
+// threads per column.
+constexpr int RPC = 16;
+
+__global__ __launch_bounds__(256) void k_panel_rp(uint32_t *a, int64_t ld, int32_t c0, int32_t jlo,
+                                                  int32_t jhi, const Panel *pan, uint32_t *rp,
+                                                  int64_t ldr, Mod md, uint32_t r64, Digits dg) {
+    const int kk = __builtin_amdgcn_readfirstlane(pan->k);
+    if (kk == 0) return;
+    __shared__ uint32_t T[PB][PB + 1];
+    __shared__ uint32_t V[PB][RPC + 1];
+    const int tid = threadIdx.x;
+    const int32_t jb = jlo + (int32_t)blockIdx.x * RPC;
+    for (int i = tid; i < PB * PB; i += 256) T[i >> 6][i & 63] = pan->T[i];
+    for (int i = tid; i < PB * RPC; i += 256) {
+        const int s_ = i / RPC, c = i % RPC;
+        V[s_][c] = (s_ < kk && jb + c < jhi) ? a[(int64_t)pan->erow[s_] * ld + jb + c] : 0u;
+    }
+    __syncthreads();
+    const int jj = tid % RPC, tg = tid / RPC;  // column, rows 4 tg .. 4 tg + 3
+    const int32_t j = jb + jj;
+    uint32_t res[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int t = 4 * tg + q;
+        uint64_t lo = 0;
+        uint32_t hi = 0;
+        for (int s_ = 0; s_ < kk; ++s_) {
+            const uint64_t pr = (uint64_t)T[t][s_] * (uint64_t)V[s_][jj];
+            const uint64_t o = lo;
+            lo += pr;
+            hi += lo < o;
+        }
+        uint32_t r = md.reduce(lo);
+        if (hi) r = (uint32_t)((r + (uint64_t)md.mul(hi, r64)) % md.p);
+        res[q] = r;
+    }
+    __syncthreads();  // all reads of the selected rows are done before any write
+    if (j >= jhi) return;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int t = 4 * tg + q;
+        if (t < kk) {
+            rp[(int64_t)t * ldr + (j - c0)] = res[q];
+            a[(int64_t)pan->erow[t] * ld + j] = res[q];
+        }
+    }
+    // digits of Rp^T: column j - c0, rows 4 tg .. 4 tg + 3 (zero beyond k),
+    // one 4-byte store per digit
+    for (int i = 0; i < dg.nd; ++i) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t r = (4 * tg + b) < kk ? res[b] : 0u;
+            v |= digit_word(r, md.p, dg.hi, i) << (8 * b);
+        }
+        *(uint32_t *)(dg.R + i * dg.rplane + (int64_t)(j - c0) * 64 + 4 * tg) = v;
+    }
 }
 
 // ------------------------------------------------------------ gather
@@ -1017,7 +1074,10 @@ int dense_rref(gbgpu_dense *d, int form) {
         const int bw = (n - c0) < PB ? (n - c0) : PB;
         const int32_t c1 = c0 + bw;                       // next panel
         const int32_t c2 = (n - c1) < PB ? n : c1 + PB;
-        k_panel_rp<<<grid_of(n - c0, 64), 256, 0, s>>>(d->a, d->ld, n, c0, cur, rp, ldr, md, r64, dg);
+        // Rp over the columns the next select needs first; the rest after
+        // that select is on its way
+        k_panel_rp<<<grid_of(c2 - c0, RPC), 256, 0, s>>>(d->a, d->ld, c0, c0, c2, cur, rp, ldr, md,
+                                                          r64, dg);
         k_panel_gather<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, m, c0, bw, rowpiv, cur, L, active,
                                                       d->p, dg);
         if (c1 < n) {
@@ -1032,6 +1092,9 @@ int dense_rref(gbgpu_dense *d, int form) {
                 cleanup();
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
+            if (c2 < n)
+                k_panel_rp<<<grid_of(n - c2, RPC), 256, 0, s>>>(d->a, d->ld, c0, c2, n, cur, rp, ldr,
+                                                                 md, r64, dg);
             update(cur, c0, c0, c1);
             update(cur, c0, c2, n);
             if (hipStreamWaitEvent(s, ctx->sync_ev[1], 0) != hipSuccess) {
