@@ -109,23 +109,24 @@ __device__ __forceinline__ void pivot_search(int fc, int32_t &sh_js, int32_t &sh
     sh_is = __ffsll((unsigned long long)live) - 1;
 }
 
-// a x + b y mod p for residues of p < 2^16: both products are below 2^32
-// (24-bit multiplies), the quotient of their sum by p is estimated in single
-// precision with an absolute error below 0.05, so the remainder computed in
-// wrapped 32-bit arithmetic lies in [-p, 2p) before the two corrections
+// a x + b y mod p for p < 2^16, residues a, x, y and b <= p: both products
+// are below 2^32 (24-bit multiplies), the quotient of their sum by p is
+// estimated in single precision with an absolute error below 0.05, so the
+// remainder r computed in wrapped 32-bit arithmetic lies in [-p, 2p); of r,
+// r + p and r - p as unsigned words the smallest is the residue
 __device__ __forceinline__ uint32_t mad2_mod16(uint32_t a, uint32_t x, uint32_t b, uint32_t y,
                                                uint32_t p, float pinv) {
     const uint32_t s = __umul24(a, x), t = __umul24(b, y);
     const uint32_t q = (uint32_t)(((float)s + (float)t) * pinv);
-    int32_t r = (int32_t)(s + t - __umul24(q, p));
-    r += r < 0 ? (int32_t)p : 0;
-    r -= r >= (int32_t)p ? (int32_t)p : 0;
-    return (uint32_t)r;
+    const uint32_t r = s + t - __umul24(q, p);
+    return min(min(r, r + p), r - p);
 }
 constexpr int SEL_T = 1024;     // threads of the select workgroup
 constexpr int SEL_RG = SEL_T / (2 * PB);  // row groups: thread = (row group, column)
 constexpr int SEL_RS = PB / SEL_RG;       // slots per thread
 
+// P16: 1 for p < 2^16, 0 for larger p, 2 to decide at run time
+template <int P16>
 __device__ __forceinline__ void select_body(const uint32_t *a, int64_t ld, int32_t n, int32_t c0,
                                             int32_t *cand, int32_t *rowpiv, int32_t *colpiv,
                                             Panel *pan, ListState *st, Mod md) {
@@ -135,10 +136,11 @@ __device__ __forceinline__ void select_body(const uint32_t *a, int64_t ld, int32
     __shared__ int32_t ecol[PB], eslot[PB], erow[PB], epos[PB], srow[PB], spos[PB];
     __shared__ int32_t sh_js, sh_is;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int col = tid & (2 * PB - 1), rg = tid >> 7;  // this thread's column and row group
+    // this thread's column and row group (the same in the whole wave)
+    const int col = tid & (2 * PB - 1), rg = __builtin_amdgcn_readfirstlane(tid >> 7);
     const uint32_t p = md.p;
     const bool small = p < (1u << 17);  // 64 products of residues fit in 64 bits
-    const bool p16 = p < (1u << 16);    // mad2_mod16 applies
+    const bool p16 = P16 == 2 ? p < (1u << 16) : P16 == 1;  // mad2_mod16 applies
     const float pinv = 1.0f / (float)p;
     const int32_t done = __builtin_amdgcn_readfirstlane(st->done);
     if (done) {
@@ -244,7 +246,7 @@ __device__ __forceinline__ void select_body(const uint32_t *a, int64_t ld, int32
                 for (int i = 0; i < SEL_RS; ++i) {
                     const int r = rg + SEL_RG * i;
                     if (r == is) continue;
-                    const uint32_t nf = fv[i] ? p - fv[i] : 0u;
+                    const uint32_t nf = p - fv[i];
                     // two products below 2^62 each: the sum fits in 64 bits
                     X[r][col] = p16 ? mad2_mod16(av, xv[i], nf, pv, p, pinv)
                                     : md.reduce((uint64_t)av * xv[i] + (uint64_t)nf * pv);
@@ -331,11 +333,12 @@ __device__ __forceinline__ void select_body(const uint32_t *a, int64_t ld, int32
     }
 }
 
+template <int P16>
 __global__ __launch_bounds__(SEL_T) void k_panel_select(const uint32_t *a, int64_t ld, int32_t n,
                                                         int32_t c0, int32_t *cand,
                                                         int32_t *rowpiv, int32_t *colpiv,
                                                         Panel *pan, ListState *st, Mod md) {
-    select_body(a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md);
+    select_body<P16>(a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md);
 }
 
 // ---------------------------------------------------------------- prune
@@ -1042,7 +1045,7 @@ int dense_rref(gbgpu_dense *d, int form) {
     k_dense_init<<<grid_of(m > n ? m : n, 256), 256, 0, s>>>(m, n, rowpiv, colpiv, cand, pan, st);
     const uint32_t r64 = (uint32_t)((((unsigned __int128)1) << 64) % d->p);
     // A[r][cb, ce) -= L[r] * Rp for the rows of the current panel's update
-    auto update = [&](const Panel *pp, int32_t c0, int32_t cb, int32_t ce) {
+    auto update = [&](hipStream_t s, const Panel *pp, int32_t c0, int32_t cb, int32_t ce) {
         if (cb >= ce) return;
         if (form == GBGPU_FORM_MFMA) {
             dim3 g((unsigned)((ce - cb + 63) / 64), (unsigned)((m + 63) / 64));
@@ -1060,13 +1063,22 @@ int dense_rref(gbgpu_dense *d, int form) {
     };
     if (hipEventRecord(ctx->ev[2], s) != hipSuccess) { cleanup(); return check_hip(ctx, hipGetLastError(), "event"); }
     // Look-ahead: once the next panel's columns are updated, its select (one
-    // workgroup) runs on the side stream while the rest of this panel's
-    // update fills the other CUs.  Panel records alternate between two
-    // slots; the candidate list is touched by the select and the prune only,
-    // which the stream order keeps apart.
+    // workgroup) runs while the rest of this panel's Rp and update fill the
+    // other CUs from the side stream.  The chain select - prune - Rp of the
+    // next panel's columns - gather - update of those columns - select stays
+    // on the main stream; the side stream's work is the shorter, so the main
+    // stream's wait for it finds it done.  Panel records alternate between
+    // two slots; the candidate list is touched by the select and the prune
+    // only, both on the main stream.
     hipStream_t x = ctx->side;
+    auto select = [&](hipStream_t sx, int32_t cs, Panel *pp) {
+        if (d->p < (1u << 16))
+            k_panel_select<1><<<1, SEL_T, 0, sx>>>(d->a, d->ld, n, cs, cand, rowpiv, colpiv, pp, st, md);
+        else
+            k_panel_select<0><<<1, SEL_T, 0, sx>>>(d->a, d->ld, n, cs, cand, rowpiv, colpiv, pp, st, md);
+    };
     if (m > 0 && n > 0)
-        k_panel_select<<<1, SEL_T, 0, s>>>(d->a, d->ld, n, 0, cand, rowpiv, colpiv, pan, st, md);
+        select(s, 0, pan);
     struct { int32_t done, k; } hs = {0, 0};
     int it = 0;
     for (int32_t c0 = 0; c0 < n && m > 0; c0 += PB, ++it) {
@@ -1081,22 +1093,24 @@ int dense_rref(gbgpu_dense *d, int form) {
         k_panel_gather<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, m, c0, bw, rowpiv, cur, L, active,
                                                       d->p, dg);
         if (c1 < n) {
-            update(cur, c0, c1, c2);
+            update(s, cur, c0, c1, c2);
             if (hipEventRecord(ctx->sync_ev[0], s) != hipSuccess ||
                 hipStreamWaitEvent(x, ctx->sync_ev[0], 0) != hipSuccess) {
                 cleanup();
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
-            k_panel_select<<<1, SEL_T, 0, x>>>(d->a, d->ld, n, c1, cand, rowpiv, colpiv, nxt, st, md);
+            if (c2 < n)
+                k_panel_rp<<<grid_of(n - c2, RPC), 256, 0, x>>>(d->a, d->ld, c0, c2, n, cur, rp, ldr,
+                                                                 md, r64, dg);
+            update(x, cur, c0, c0, c1);
+            update(x, cur, c0, c2, n);
             if (hipEventRecord(ctx->sync_ev[1], x) != hipSuccess) {
                 cleanup();
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
-            if (c2 < n)
-                k_panel_rp<<<grid_of(n - c2, RPC), 256, 0, s>>>(d->a, d->ld, c0, c2, n, cur, rp, ldr,
-                                                                 md, r64, dg);
-            update(cur, c0, c0, c1);
-            update(cur, c0, c2, n);
+            // the select reads the columns [c1, c2) of candidate rows, which
+            // the side stream's work does not touch
+            select(s, c1, nxt);
             if (hipStreamWaitEvent(s, ctx->sync_ev[1], 0) != hipSuccess) {
                 cleanup();
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
@@ -1104,7 +1118,7 @@ int dense_rref(gbgpu_dense *d, int form) {
             k_prune_flags<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, n, c1, cand, st, active);
             k_prune_compact<<<1, 256, 0, s>>>(cand, st, active);
         } else {
-            update(cur, c0, c0, n);
+            update(s, cur, c0, c0, n);
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { cleanup(); return check_hip(ctx, e, "dense panel launch"); }
@@ -1213,7 +1227,7 @@ __global__ __launch_bounds__(256) void k_batch_dense_init(const DenseDesc *D) {
 __global__ __launch_bounds__(SEL_T) void k_batch_select(const DenseDesc *D, int32_t c0) {
     const DenseDesc &d = D[blockIdx.x];
     if (c0 >= d.n) return;
-    select_body(d.a, d.ld, d.n, c0, d.cand, d.rowpiv, d.colpiv, d.pan, d.st, d.md);
+    select_body<2>(d.a, d.ld, d.n, c0, d.cand, d.rowpiv, d.colpiv, d.pan, d.st, d.md);
 }
 
 __global__ __launch_bounds__(256) void k_batch_rp(const DenseDesc *D, const int32_t *pre,
