@@ -7,7 +7,9 @@
  * throughout; where the matrix has to look like gb's (row format, known
  * pivot = first row per lead, columns = pivot columns first), it does.
  *
- *  * Monomials live in one table: exponent vectors (one byte per variable),
+ *  * Monomials live in one table: exponent vectors (16 bits per variable,
+ *    at most 32767 each, the range of gb's int16 exp_t, data.h:63;
+ *    a product leaving that range ends the run with GBGPU_ERANGE),
  *    degree, a divisibility mask (bit i set when x_i occurs) and an
  *    additive hash h(e) = sum_i r_i e_i, so h(a b) = h(a) + h(b) and a
  *    product is looked up without hashing its exponents again.
@@ -42,34 +44,43 @@ static double now_ms(void) {
 
 /* ------------------------------------------------------------ monomials */
 
+/* exponents: 16 bits each, at most EX_MAX; four of them fill a 64-bit
+ * word, and the sum of two in-range words has no carry between lanes */
+typedef uint16_t ex_t;
+#define EX_MAX 32767
+#define EX_HIGH 0x8000800080008000ull
+
 typedef struct {
     int nv, nvp, order;
-    uint8_t *ev;     /* nvp bytes per monomial */
+    int erange;      /* set when a product left [0, EX_MAX] */
+    ex_t *ev;        /* nvp exponents per monomial */
     uint32_t *deg;
     uint64_t *hv;
     uint64_t *dm;
     int32_t n, cap;
     int32_t *slot;   /* id + 1, 0 = empty */
     uint64_t smask;
-    uint64_t rnd[64];
-    uint8_t *tmp;    /* nvp scratch bytes */
+    uint64_t *rnd;   /* nv hash weights */
+    ex_t *tmp;       /* nvp scratch exponents */
 } mt_t;
 
 static int mt_init(mt_t *t, int nv, int order) {
     memset(t, 0, sizeof(*t));
     t->nv = nv;
-    t->nvp = (nv + 7) & ~7;
+    t->nvp = (nv + 3) & ~3;
     t->order = order;
     t->cap = 1 << 12;
-    t->ev = (uint8_t *)calloc((size_t)t->cap, (size_t)t->nvp);
+    t->ev = (ex_t *)calloc((size_t)t->cap * (size_t)t->nvp, sizeof(ex_t));
     t->deg = (uint32_t *)malloc((size_t)t->cap * sizeof(uint32_t));
     t->hv = (uint64_t *)malloc((size_t)t->cap * sizeof(uint64_t));
     t->dm = (uint64_t *)malloc((size_t)t->cap * sizeof(uint64_t));
     t->smask = (1u << 13) - 1;
     t->slot = (int32_t *)calloc((size_t)t->smask + 1, sizeof(int32_t));
-    t->tmp = (uint8_t *)calloc(1, (size_t)t->nvp);
+    t->tmp = (ex_t *)calloc((size_t)t->nvp, sizeof(ex_t));
+    t->rnd = (uint64_t *)malloc((size_t)nv * sizeof(uint64_t));
+    if (!t->rnd) return -1;
     uint64_t x = 0x9E3779B97F4A7C15ull;
-    for (int i = 0; i < 64; ++i) {
+    for (int i = 0; i < nv; ++i) {
         /* splitmix64 */
         x += 0x9E3779B97F4A7C15ull;
         uint64_t z = x;
@@ -82,6 +93,7 @@ static int mt_init(mt_t *t, int nv, int order) {
 
 static void mt_free(mt_t *t) {
     free(t->ev); free(t->deg); free(t->hv); free(t->dm); free(t->slot); free(t->tmp);
+    free(t->rnd);
     memset(t, 0, sizeof(*t));
 }
 
@@ -90,7 +102,7 @@ static inline uint64_t mt_pos(uint64_t h) { return h ^ (h >> 29) ^ (h >> 43); }
 static int mt_grow(mt_t *t) {
     if (t->n + 1 >= t->cap) {
         int32_t nc = t->cap * 2;
-        uint8_t *ev = (uint8_t *)realloc(t->ev, (size_t)nc * (size_t)t->nvp);
+        ex_t *ev = (ex_t *)realloc(t->ev, (size_t)nc * (size_t)t->nvp * sizeof(ex_t));
         uint32_t *dg = (uint32_t *)realloc(t->deg, (size_t)nc * sizeof(uint32_t));
         uint64_t *hv = (uint64_t *)realloc(t->hv, (size_t)nc * sizeof(uint64_t));
         uint64_t *dm = (uint64_t *)realloc(t->dm, (size_t)nc * sizeof(uint64_t));
@@ -117,14 +129,14 @@ static int mt_grow(mt_t *t) {
     return 0;
 }
 
-/* id of the monomial with exponents e (nvp bytes, zero padded) and hash h */
-static int32_t mt_get(mt_t *t, const uint8_t *e, uint64_t h) {
+/* id of the monomial with exponents e (nvp of them, zero padded) and hash h */
+static int32_t mt_get(mt_t *t, const ex_t *e, uint64_t h) {
     uint64_t p = mt_pos(h) & t->smask;
     for (;;) {
         const int32_t s = t->slot[p];
         if (!s) break;
         const int32_t id = s - 1;
-        if (t->hv[id] == h && memcmp(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp) == 0)
+        if (t->hv[id] == h && memcmp(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp * sizeof(ex_t)) == 0)
             return id;
         p = (p + 1) & t->smask;
     }
@@ -133,12 +145,12 @@ static int32_t mt_get(mt_t *t, const uint8_t *e, uint64_t h) {
     p = mt_pos(h) & t->smask;
     while (t->slot[p]) p = (p + 1) & t->smask;
     const int32_t id = t->n++;
-    memcpy(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp);
+    memcpy(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp * sizeof(ex_t));
     uint32_t d = 0;
     uint64_t m = 0;
     for (int i = 0; i < t->nv; ++i) {
         d += e[i];
-        if (e[i]) m |= 1ull << i;
+        if (e[i]) m |= 1ull << (i & 63);
     }
     t->deg[id] = d;
     t->hv[id] = h;
@@ -149,13 +161,13 @@ static int32_t mt_get(mt_t *t, const uint8_t *e, uint64_t h) {
 
 /* id of an existing monomial, -1 when absent (no insertion: safe to call
  * from many threads while nothing is inserted) */
-static int32_t mt_find(const mt_t *t, const uint8_t *e, uint64_t h) {
+static int32_t mt_find(const mt_t *t, const ex_t *e, uint64_t h) {
     uint64_t p = mt_pos(h) & t->smask;
     for (;;) {
         const int32_t s = t->slot[p];
         if (!s) return -1;
         const int32_t id = s - 1;
-        if (t->hv[id] == h && memcmp(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp) == 0)
+        if (t->hv[id] == h && memcmp(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp * sizeof(ex_t)) == 0)
             return id;
         p = (p + 1) & t->smask;
     }
@@ -168,7 +180,7 @@ static int mt_reserve(mt_t *t, int64_t extra) {
     if (need > t->cap) {
         int32_t nc = t->cap;
         while (nc < need) nc *= 2;
-        uint8_t *ev = (uint8_t *)realloc(t->ev, (size_t)nc * (size_t)t->nvp);
+        ex_t *ev = (ex_t *)realloc(t->ev, (size_t)nc * (size_t)t->nvp * sizeof(ex_t));
         uint32_t *dg = (uint32_t *)realloc(t->deg, (size_t)nc * sizeof(uint32_t));
         uint64_t *hv = (uint64_t *)realloc(t->hv, (size_t)nc * sizeof(uint64_t));
         uint64_t *dm = (uint64_t *)realloc(t->dm, (size_t)nc * sizeof(uint64_t));
@@ -201,7 +213,7 @@ static int mt_reserve(mt_t *t, int64_t extra) {
  * a thread meeting a claimed slot waits for it.  Ids come from an atomic
  * counter; one at or above lim is not stored (the slot is released and
  * *over set) -- the caller falls back to the serial path. */
-static int32_t mt_get_par(mt_t *t, const uint8_t *e, uint64_t h, int32_t lim, int *over) {
+static int32_t mt_get_par(mt_t *t, const ex_t *e, uint64_t h, int32_t lim, int *over) {
     uint64_t p = mt_pos(h) & t->smask;
     for (;;) {
         int32_t s = __atomic_load_n(&t->slot[p], __ATOMIC_ACQUIRE);
@@ -215,12 +227,12 @@ static int32_t mt_get_par(mt_t *t, const uint8_t *e, uint64_t h, int32_t lim, in
                     __atomic_store_n(over, 1, __ATOMIC_RELAXED);
                     return -1;
                 }
-                memcpy(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp);
+                memcpy(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp * sizeof(ex_t));
                 uint32_t d = 0;
                 uint64_t m = 0;
                 for (int i = 0; i < t->nv; ++i) {
                     d += e[i];
-                    if (e[i]) m |= 1ull << i;
+                    if (e[i]) m |= 1ull << (i & 63);
                 }
                 t->deg[id] = d;
                 t->hv[id] = h;
@@ -233,37 +245,47 @@ static int32_t mt_get_par(mt_t *t, const uint8_t *e, uint64_t h, int32_t lim, in
         while (s == -1) s = __atomic_load_n(&t->slot[p], __ATOMIC_ACQUIRE);
         if (s == 0) continue;  /* released by an overflowing claim: look again */
         const int32_t id = s - 1;
-        if (t->hv[id] == h && memcmp(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp) == 0)
+        if (t->hv[id] == h && memcmp(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp * sizeof(ex_t)) == 0)
             return id;
         p = (p + 1) & t->smask;
     }
 }
 
-static uint64_t mt_hash(const mt_t *t, const uint8_t *e) {
+static uint64_t mt_hash(const mt_t *t, const ex_t *e) {
     uint64_t h = 0;
     for (int i = 0; i < t->nv; ++i) h += t->rnd[i] * e[i];
     return h;
 }
 
-static inline const uint8_t *EV(const mt_t *t, int32_t a) { return t->ev + (size_t)a * t->nvp; }
+static inline const ex_t *EV(const mt_t *t, int32_t a) { return t->ev + (size_t)a * t->nvp; }
 
+/* a b; -1 with t->erange set when an exponent leaves [0, EX_MAX] */
 static int32_t mt_mul(mt_t *t, int32_t a, int32_t b) {
-    const uint8_t *ea = EV(t, a), *eb = EV(t, b);
-    for (int i = 0; i < t->nvp; ++i) t->tmp[i] = (uint8_t)(ea[i] + eb[i]);
+    const ex_t *ea = EV(t, a), *eb = EV(t, b);
+    int hi = 0;
+    for (int i = 0; i < t->nvp; ++i) {
+        const unsigned x = (unsigned)ea[i] + eb[i];
+        hi |= x > EX_MAX;
+        t->tmp[i] = (ex_t)x;
+    }
+    if (hi) {
+        t->erange = 1;
+        return -1;
+    }
     return mt_get(t, t->tmp, t->hv[a] + t->hv[b]);
 }
 
 /* b / a (a divides b) */
 static int32_t mt_div(mt_t *t, int32_t b, int32_t a) {
-    const uint8_t *ea = EV(t, a), *eb = EV(t, b);
-    for (int i = 0; i < t->nvp; ++i) t->tmp[i] = (uint8_t)(eb[i] - ea[i]);
+    const ex_t *ea = EV(t, a), *eb = EV(t, b);
+    for (int i = 0; i < t->nvp; ++i) t->tmp[i] = (ex_t)(eb[i] - ea[i]);
     return mt_get(t, t->tmp, t->hv[b] - t->hv[a]);
 }
 
 static inline int mt_divides(const mt_t *t, int32_t a, int32_t b) {
     if (t->dm[a] & ~t->dm[b]) return 0;
     if (t->deg[a] > t->deg[b]) return 0;
-    const uint8_t *ea = EV(t, a), *eb = EV(t, b);
+    const ex_t *ea = EV(t, a), *eb = EV(t, b);
     for (int i = 0; i < t->nv; ++i)
         if (ea[i] > eb[i]) return 0;
     return 1;
@@ -272,7 +294,7 @@ static inline int mt_divides(const mt_t *t, int32_t a, int32_t b) {
 /* > 0 when a > b in the monomial order */
 static inline int mt_cmp(const mt_t *t, int32_t a, int32_t b) {
     if (a == b) return 0;
-    const uint8_t *ea = EV(t, a), *eb = EV(t, b);
+    const ex_t *ea = EV(t, a), *eb = EV(t, b);
     if (t->order == 0) {
         if (t->deg[a] != t->deg[b]) return t->deg[a] > t->deg[b] ? 1 : -1;
         for (int i = t->nv - 1; i >= 0; --i)
@@ -280,6 +302,16 @@ static inline int mt_cmp(const mt_t *t, int32_t a, int32_t b) {
         return 0;
     }
     for (int i = 0; i < t->nv; ++i)
+        if (ea[i] != eb[i]) return ea[i] > eb[i] ? 1 : -1;
+    return 0;
+}
+
+/* > 0 when gb's initial_input_cmp puts a after b (order.c:38-95) */
+static inline int input_cmp(const mt_t *t, int32_t a, int32_t b) {
+    if (t->order != 0) return mt_cmp(t, a, b);
+    if (t->deg[a] != t->deg[b]) return t->deg[a] > t->deg[b] ? 1 : -1;
+    const ex_t *ea = EV(t, a), *eb = EV(t, b);
+    for (int i = t->nv - 1; i >= 0; --i)
         if (ea[i] != eb[i]) return ea[i] > eb[i] ? 1 : -1;
     return 0;
 }
@@ -359,13 +391,13 @@ static uint32_t inv_mod(uint32_t a, uint32_t p) {
  * rule in insert_and_update_spairs (update.c:96-112). */
 static inline int lcm_is(const mt_t *T, int32_t a, int32_t b, int32_t l) {
     /* lcm(a, b) == l, on the exponents */
-    const uint8_t *ea = EV(T, a), *eb = EV(T, b), *el = EV(T, l);
+    const ex_t *ea = EV(T, a), *eb = EV(T, b), *el = EV(T, l);
     for (int v = 0; v < T->nv; ++v)
         if ((ea[v] > eb[v] ? ea[v] : eb[v]) != el[v]) return 0;
     return 1;
 }
 
-static inline int ediv(const uint8_t *a, const uint8_t *b, int nv) {
+static inline int ediv(const ex_t *a, const ex_t *b, int nv) {
     for (int v = 0; v < nv; ++v)
         if (a[v] > b[v]) return 0;
     return 1;
@@ -385,24 +417,24 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
     /* candidate pairs (i, k): lcm exponents, degree and mask kept locally;
      * only the pairs that survive get a monomial id */
     const size_t nn = nl ? (size_t)nl : 1;
-    uint8_t *le = (uint8_t *)malloc(nn * (size_t)nvp);
+    ex_t *le = (ex_t *)malloc(nn * (size_t)nvp * sizeof(ex_t));
     uint32_t *ld = (uint32_t *)malloc(nn * sizeof(uint32_t));
     uint64_t *lm = (uint64_t *)malloc(nn * sizeof(uint64_t));
     uint8_t *st = (uint8_t *)malloc(nn);  /* 0 dropped, 1 kept, 2 coprime */
     if (!le || !ld || !lm || !st) { rc = -1; goto out; }
     {
-        const uint8_t *ek = EV(T, lk);
+        const ex_t *ek = EV(T, lk);
         for (int32_t q = 0; q < nl; ++q) {
             const int32_t i = B->lm[q];
             const int32_t li = B->e[i].m[0];
-            const uint8_t *ei = EV(T, li);
-            uint8_t *e = le + (size_t)q * nvp;
+            const ex_t *ei = EV(T, li);
+            ex_t *e = le + (size_t)q * nvp;
             uint32_t d = 0;
             uint64_t m = 0;
             for (int v = 0; v < nvp; ++v) {
                 e[v] = ei[v] > ek[v] ? ei[v] : ek[v];
                 d += e[v];
-                if (e[v]) m |= 1ull << v;
+                if (e[v]) m |= 1ull << (v & 63);
             }
             ld[q] = d;
             lm[q] = m;
@@ -430,12 +462,15 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
             free(drop);
         }
     }
-    /* new pairs: a pair is dropped when another pair's lcm properly
-     * divides its lcm, or equals it and the other pair comes later;
-     * coprime pairs take part in this and are dropped afterwards.  (Taken
-     * in order with dropped pairs left out, the Gebauer-Moeller loop drops
-     * exactly these pairs: a divisor that is itself dropped has a divisor
-     * that is not.)  Each candidate is checked on its own, in parallel. */
+    /* new pairs, gb's rule (update.c:157-195): with the candidates sorted
+     * by lcm and, for equal lcms, by basis index, a pair is dropped when
+     * an earlier one's lcm divides its lcm (hash.c:351-389) -- so of
+     * several pairs with one lcm the one with the oldest element stays --
+     * and the survivors whose leads are coprime are dropped afterwards
+     * (hash.c:553-557).  Coprime pairs still drop the pairs above them.
+     * (Dropped pairs need not be left out as divisors: a dropped divisor
+     * has an earlier divisor of its own.)  Each candidate is checked on its
+     * own, in parallel. */
     {
         uint8_t *keep = (uint8_t *)malloc(nn);
         if (!keep) { rc = -1; goto out; }
@@ -443,11 +478,12 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
         for (int32_t q = 0; q < nl; ++q) {
             keep[q] = st[q];
             if (st[q] != 1) continue;
-            const uint8_t *eq = le + (size_t)q * nvp;
+            const ex_t *eq = le + (size_t)q * nvp;
             for (int32_t r = 0; r < nl; ++r) {
                 if (r == q || st[r] == 0 || ld[r] > ld[q] || (lm[r] & ~lm[q])) continue;
-                const uint8_t *er = le + (size_t)r * nvp;
-                if (ld[r] == ld[q] ? (r > q && memcmp(er, eq, (size_t)nvp) == 0) : ediv(er, eq, nv)) {
+                const ex_t *er = le + (size_t)r * nvp;
+                if (ld[r] == ld[q] ? (r < q && memcmp(er, eq, (size_t)nvp * sizeof(ex_t)) == 0)
+                                   : ediv(er, eq, nv)) {
                     keep[q] = 0;
                     break;
                 }
@@ -458,15 +494,7 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
     }
     for (int32_t q = 0; q < nl; ++q) {
         if (st[q] != 1) continue;
-        /* a coprime pair with the same lcm removes the whole group */
-        int cop = 0;
-        const uint8_t *eq = le + (size_t)q * nvp;
-        for (int32_t r = 0; r < nl; ++r)
-            if (st[r] == 2 && ld[r] == ld[q] && memcmp(le + (size_t)r * nvp, eq, (size_t)nvp) == 0) {
-                cop = 1;
-                break;
-            }
-        if (cop) continue;
+        const ex_t *eq = le + (size_t)q * nvp;
         const int32_t i = B->lm[q];
         const int32_t l = mt_get(T, eq, mt_hash(T, eq));
         if (l < 0) { rc = -1; goto out; }
@@ -706,11 +734,12 @@ static int row_stub(rows_t *R, const basis_t *B, int32_t e, int32_t mul) {
 static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
     mt_t *T = &F->T;
     const basis_t *B = &F->B;
-    const int nw = T->nvp / 8;
+    const int nw = T->nvp / 4;
     int64_t nmiss = 0;
-#pragma omp parallel reduction(+ : nmiss)
+    uint64_t high = 0;
+#pragma omp parallel reduction(+ : nmiss) reduction(| : high)
     {
-        uint64_t tmp[8];
+        uint64_t tmp[nw];
 #pragma omp for schedule(dynamic, 8)
         for (int64_t r = r0; r < r1; ++r) {
             const row_t *x = &R->r[r];
@@ -720,11 +749,26 @@ static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
             int32_t *o = R->mon + x->off;
             for (int32_t t = 0; t < f->len; ++t) {
                 const uint64_t *et = (const uint64_t *)EV(T, f->m[t]);
-                for (int w = 0; w < nw; ++w) tmp[w] = em[w] + et[w];  /* no carries: exponents < 256 */
-                o[t] = mt_find(T, (const uint8_t *)tmp, hm + T->hv[f->m[t]]);
+                /* four 16-bit exponents per word, each <= EX_MAX: no carries
+                 * between lanes; a lane above EX_MAX shows in its top bit */
+                uint64_t hi = 0;
+                for (int w = 0; w < nw; ++w) {
+                    tmp[w] = em[w] + et[w];
+                    hi |= tmp[w];
+                }
+                if (hi & EX_HIGH) {
+                    high |= hi;
+                    o[t] = 0;
+                    continue;
+                }
+                o[t] = mt_find(T, (const ex_t *)tmp, hm + T->hv[f->m[t]]);
                 nmiss += o[t] < 0;
             }
         }
+    }
+    if (high & EX_HIGH) {
+        T->erange = 1;
+        return -1;
     }
     if (nmiss) {
         /* new monomials: inserted by all threads into reserved room (the
@@ -737,7 +781,7 @@ static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
             const int32_t lim = (int32_t)(T->n + room);
 #pragma omp parallel
             {
-                uint64_t tmp[8];
+                uint64_t tmp[nw];
 #pragma omp for schedule(dynamic, 8)
                 for (int64_t r = r0; r < r1; ++r) {
                     const row_t *x = &R->r[r];
@@ -749,7 +793,7 @@ static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
                         if (o[t] >= 0) continue;
                         const uint64_t *et = (const uint64_t *)EV(T, f->m[t]);
                         for (int w = 0; w < nw; ++w) tmp[w] = em[w] + et[w];
-                        o[t] = mt_get_par(T, (const uint8_t *)tmp, hm + T->hv[f->m[t]], lim, &over);
+                        o[t] = mt_get_par(T, (const ex_t *)tmp, hm + T->hv[f->m[t]], lim, &over);
                     }
                 }
             }
@@ -975,6 +1019,16 @@ static int f4_round(f4_t *F) {
         if (update_pairs(T, B, P, B->n - 1, lo)) goto done;
     }
     F->out->update_ms += now_ms() - tp;
+    {
+        /* the round's record: what gb's info_level 2 line shows */
+        const int32_t r = F->out->rounds;
+        int64_t *tr = (int64_t *)realloc(F->out->trace,
+                                         (size_t)(r + 1) * GBGPU_F4_TRACE_W * sizeof(int64_t));
+        if (!tr) goto done;
+        F->out->trace = tr;
+        int64_t *x = tr + (size_t)r * GBGPU_F4_TRACE_W;
+        x[0] = dmin; x[1] = nsel; x[2] = m.nr; x[3] = nc; x[4] = ncl; x[5] = nnz; x[6] = out.np;
+    }
     if (F->info > 1)
         printf("%3u  %6lld sel  %8lld pairs  %7d x %-7d (%lld spair rows)  %6.2f%%  %5d new  %8.1f ms LA\n",
                dmin, (long long)nsel, (long long)P->n, m.nr, nc, (long long)nspair_rows,
@@ -1001,7 +1055,7 @@ static int f4_reduce(f4_t *F) {
     gbgpu_rows_t out = {0};
     int32_t one;
     {
-        uint8_t *z = (uint8_t *)calloc(1, (size_t)T->nvp);
+        ex_t *z = (ex_t *)calloc((size_t)T->nvp, sizeof(ex_t));
         if (!z) goto done;
         one = mt_get(T, z, 0);
         free(z);
@@ -1053,7 +1107,7 @@ static int cmp_i32(const void *a, const void *b) {
 int gbgpu_f4(const gbgpu_f4_in_t *in, gbgpu_f4_la_t la, void *user, gbgpu_f4_out_t *out) {
     if (!in || !la || !out) return GBGPU_EINVAL;
     memset(out, 0, sizeof(*out));
-    if (in->nr_vars < 1 || in->nr_vars > 64 || in->nr_gens < 0 || in->field_char < 2 ||
+    if (in->nr_vars < 1 || in->nr_vars > GBGPU_F4_MAX_VARS || in->nr_gens < 0 || in->field_char < 2 ||
         (in->mon_order != 0 && in->mon_order != 1) || !in->lens || !in->exps || !in->cfs)
         return GBGPU_EINVAL;
     const double t_start = now_ms();
@@ -1079,15 +1133,15 @@ int gbgpu_f4(const gbgpu_f4_in_t *in, gbgpu_f4_la_t la, void *user, gbgpu_f4_out
             int32_t *ord = (int32_t *)malloc((size_t)(len ? len : 1) * sizeof(int32_t));
             if (!mm || !cc || !ord) { free(mm); free(cc); free(ord); goto done; }
             for (int32_t t = 0; t < len; ++t) {
-                uint8_t *e = F.T.tmp;
-                memset(e, 0, (size_t)F.T.nvp);
+                ex_t *e = F.T.tmp;
+                memset(e, 0, (size_t)F.T.nvp * sizeof(ex_t));
                 for (int i = 0; i < nv; ++i) {
                     const int32_t x = in->exps[(off + t) * nv + i];
-                    if (x < 0 || x > 255) { free(mm); free(cc); free(ord); rc = GBGPU_ERANGE; goto done; }
-                    e[i] = (uint8_t)x;
+                    if (x < 0 || x > EX_MAX) { free(mm); free(cc); free(ord); rc = GBGPU_ERANGE; goto done; }
+                    e[i] = (ex_t)x;
                 }
-                uint8_t ebuf[64];
-                memcpy(ebuf, e, (size_t)F.T.nvp);
+                ex_t ebuf[F.T.nvp];
+                memcpy(ebuf, e, (size_t)F.T.nvp * sizeof(ex_t));
                 mm[t] = mt_get(&F.T, ebuf, mt_hash(&F.T, ebuf));
                 if (mm[t] < 0) { free(mm); free(cc); free(ord); goto done; }
                 int64_t c = (int64_t)in->cfs[off + t] % (int64_t)F.p;
@@ -1131,7 +1185,11 @@ int gbgpu_f4(const gbgpu_f4_in_t *in, gbgpu_f4_la_t la, void *user, gbgpu_f4_out
             if (basis_push(&F.B, f)) { free(f.m); free(f.c); goto done; }
         }
     }
-    /* smallest lead first, then installed one by one (f4.c:258-275) */
+    /* the input order of gb's initial_input_cmp (order.c:38-95, sorted at
+     * f4.c:258-259), then installed one by one (f4.c:272-275): for lex,
+     * increasing lead; for DRL, increasing lead degree and, within a
+     * degree, DECREASING lead (that comparator returns e_a - e_b at the
+     * last differing variable, the opposite of the DRL comparison) */
     {
         const int32_t n0 = F.B.n;
         int32_t *ord = (int32_t *)malloc((size_t)(n0 ? n0 : 1) * sizeof(int32_t));
@@ -1141,7 +1199,7 @@ int gbgpu_f4(const gbgpu_f4_in_t *in, gbgpu_f4_la_t la, void *user, gbgpu_f4_out
         for (int32_t a = 1; a < n0; ++a) {
             const int32_t x = ord[a];
             int32_t b = a - 1;
-            while (b >= 0 && mt_cmp(&F.T, F.B.e[ord[b]].m[0], F.B.e[x].m[0]) > 0) {
+            while (b >= 0 && input_cmp(&F.T, F.B.e[ord[b]].m[0], F.B.e[x].m[0]) > 0) {
                 ord[b + 1] = ord[b];
                 --b;
             }
@@ -1201,7 +1259,7 @@ int gbgpu_f4(const gbgpu_f4_in_t *in, gbgpu_f4_la_t la, void *user, gbgpu_f4_out
             int32_t *oe = out->bexp + toff[q] * nv;
             for (int32_t a = 0; a < f->len; ++a) {
                 oc[a] = f->c[a];
-                const uint8_t *e = EV(&F.T, f->m[a]);
+                const ex_t *e = EV(&F.T, f->m[a]);
                 for (int i = 0; i < nv; ++i) oe[(int64_t)a * nv + i] = e[i];
             }
         }
@@ -1210,6 +1268,8 @@ int gbgpu_f4(const gbgpu_f4_in_t *in, gbgpu_f4_la_t la, void *user, gbgpu_f4_out
     }
     rc = GBGPU_OK;
 done:
+    /* an exponent past gb's int16 range is reported, never wrapped */
+    if (rc != GBGPU_OK && F.T.erange) rc = GBGPU_ERANGE;
     for (int32_t i = 0; i < F.B.n; ++i) {
         free(F.B.e[i].m);
         free(F.B.e[i].c);
@@ -1230,9 +1290,11 @@ void gbgpu_f4_out_free(gbgpu_f4_out_t *out) {
     free(out->blen);
     free(out->bexp);
     free(out->bcf);
+    free(out->trace);
     out->blen = NULL;
     out->bexp = NULL;
     out->bcf = NULL;
+    out->trace = NULL;
     out->bld = 0;
     out->nterms = 0;
 }

@@ -55,7 +55,11 @@ class _F4Out(C.Structure):
                 ("nterms", C.c_int64), ("rounds", C.c_int32), ("max_rows", C.c_int64),
                 ("max_cols", C.c_int64), ("max_nnz", C.c_int64), ("rows_total", C.c_int64),
                 ("la_ms", C.c_double), ("total_ms", C.c_double), ("select_ms", C.c_double),
-                ("symbolic_ms", C.c_double), ("build_ms", C.c_double), ("update_ms", C.c_double)]
+                ("symbolic_ms", C.c_double), ("build_ms", C.c_double), ("update_ms", C.c_double),
+                ("trace", C.POINTER(C.c_int64))]
+
+# fields of one round's record in _F4Out.trace (GBGPU_F4_TRACE_W in gbgpu_f4.h)
+F4_TRACE_FIELDS = ("degree", "pairs", "rows", "cols", "pivot_cols", "nnz", "new")
 
 
 class _F4Gpu(C.Structure):
@@ -120,7 +124,10 @@ def f4(system: PolySystem, la_fn: int, la_user, reduce_gb: int = 0, info_level: 
                      "max_nnz": out.max_nnz, "rows_total": out.rows_total, "la_ms": out.la_ms,
                      "total_ms": out.total_ms, "select_ms": out.select_ms,
                      "symbolic_ms": out.symbolic_ms, "build_ms": out.build_ms,
-                     "update_ms": out.update_ms})
+                     "update_ms": out.update_ms,
+                     "trace": ([dict(zip(F4_TRACE_FIELDS, map(int, row))) for row in
+                                np.ctypeslib.as_array(out.trace, shape=(out.rounds, len(F4_TRACE_FIELDS)))]
+                               if out.rounds and out.trace else [])})
     L.gbgpu_f4_out_free(C.byref(out))
     return b
 

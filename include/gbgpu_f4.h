@@ -25,11 +25,16 @@
 extern "C" {
 #endif
 
+/* variables per system: gb has no limit of its own; this bound only keeps
+ * an exponent vector (2 bytes per variable) a sane size */
+#define GBGPU_F4_MAX_VARS 4096
+
 typedef struct {
-    int32_t nr_vars;     /* <= 64 */
+    int32_t nr_vars;     /* 1 .. GBGPU_F4_MAX_VARS */
     int32_t nr_gens;
     const int32_t *lens; /* nr_gens */
-    const int32_t *exps; /* nr_vars per term, degrees < 256 */
+    const int32_t *exps; /* nr_vars per term, each in [0, 32767] (gb's int16 exp_t);
+                            a run whose products leave that range returns GBGPU_ERANGE */
     const int32_t *cfs;  /* one per term, any sign (taken mod field_char) */
     int32_t field_char;  /* prime below 2^31 */
     int32_t mon_order;   /* 0: degree reverse lexicographical, 1: lexicographical */
@@ -51,7 +56,14 @@ typedef struct {
     double la_ms;                         /* wall time inside the callback */
     double total_ms;
     double select_ms, symbolic_ms, build_ms, update_ms;  /* host phases */
+    /* one record of GBGPU_F4_TRACE_W values per round (rounds records):
+     * degree, pairs selected, rows, columns, pivot columns, nonzeros, new
+     * rows -- the matrix shape and result gb prints at info_level 2
+     * (symbol.c:72-75, convert.c:131, la_ff_32.c:1845) */
+    int64_t *trace;
 } gbgpu_f4_out_t;
+
+#define GBGPU_F4_TRACE_W 7
 
 /* Linear algebra of one round: interreduce == 0 is gb's linear_algebra on
  * the matrix m (data.h:269-273: known pivots = first row per lead, result
