@@ -19,7 +19,8 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I", os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-function"]
 
-SOURCES = ["api.hip", "sparse.hip", "dense.hip", "rows.hip", "batch.hip", "dsparse.hip"]
+SOURCES = ["api.hip", "sparse.hip", "dense.hip", "rows.hip", "batch.hip", "dsparse.hip",
+           "noalloc.hip"]
 C_SOURCES = ["gb_adapter.c", "f4.c"]
 
 

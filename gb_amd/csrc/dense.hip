@@ -1007,37 +1007,81 @@ static int grid_of(int64_t n, int bs) {
     return (int)g;
 }
 
+// element counts of the scratch arrays of an m x n echelon form, in the
+// order of DenseWs; the digit planes are empty in the plain form
+static void dense_ws_counts(int32_t m, int32_t n, int form, uint32_t p, size_t cnt[10],
+                            size_t elt[10]) {
+    const int64_t ldr = ((int64_t)n + 63) & ~63ll;
+    const int nd = form == GBGPU_FORM_MFMA ? digits_for(p) : 0;
+    const size_t c[10] = {(size_t)m, (size_t)n, (size_t)m, (size_t)m, (size_t)m * PB,
+                          (size_t)PB * ldr, (size_t)nd * m * PB, (size_t)nd * ldr * PB, 2, 1};
+    const size_t e[10] = {4, 4, 4, 4, 4, 4, 1, 1, sizeof(Panel), sizeof(ListState)};
+    for (int i = 0; i < 10; ++i) {
+        cnt[i] = c[i];
+        elt[i] = e[i];
+    }
+}
+
+size_t dense_ws_bytes(int32_t m, int32_t n, int form, uint32_t p) {
+    size_t cnt[10], elt[10], tot = 0;
+    dense_ws_counts(m, n, form, p, cnt, elt);
+    for (int i = 0; i < 10; ++i) tot += (cnt[i] * elt[i] + 255) & ~(size_t)255;
+    return tot;
+}
+
+void dense_ws_carve(uint8_t *base, int32_t m, int32_t n, int form, uint32_t p, DenseWs *ws) {
+    size_t cnt[10], elt[10], off = 0;
+    dense_ws_counts(m, n, form, p, cnt, elt);
+    void *ptr[10];
+    for (int i = 0; i < 10; ++i) {
+        ptr[i] = cnt[i] ? (void *)(base + off) : nullptr;
+        off += (cnt[i] * elt[i] + 255) & ~(size_t)255;
+    }
+    ws->rowpiv = (int32_t *)ptr[0];
+    ws->colpiv = (int32_t *)ptr[1];
+    ws->active = (int32_t *)ptr[2];
+    ws->cand = (int32_t *)ptr[3];
+    ws->L = (uint32_t *)ptr[4];
+    ws->rp = (uint32_t *)ptr[5];
+    ws->dl = (int8_t *)ptr[6];
+    ws->dr = (int8_t *)ptr[7];
+    ws->pan = (Panel *)ptr[8];
+    ws->st = (ListState *)ptr[9];
+}
+
 int dense_rref(gbgpu_dense *d, int form) {
+    gbgpu_ctx *ctx = d->ctx;
+    dfree(d->orow);
+    dfree(d->olead);
+    d->rank = 0;
+    uint8_t *base = nullptr;
+    int rc;
+    if ((rc = dalloc(ctx, &base, dense_ws_bytes(d->m, d->n, form, d->p)))) return rc;
+    DenseWs ws;
+    dense_ws_carve(base, d->m, d->n, form, d->p, &ws);
+    rc = dense_rref_ws(d, form, ws);
+    if (!rc) rc = dense_output(d, ws.colpiv);
+    dfree(base);
+    if (rc) return rc;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]) == hipSuccess) ctx->phase_ms[1] = ms;
+    return GBGPU_OK;
+}
+
+int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws) {
     gbgpu_ctx *ctx = d->ctx;
     hipStream_t s = ctx->stream;
     const int32_t m = d->m, n = d->n;
     const Mod md = Mod::make(d->p);
     const bool big = d->p >= (1u << 17);
-    int rc;
-    dfree(d->orow);
-    dfree(d->olead);
-    d->rank = 0;
-    int32_t *rowpiv = nullptr, *colpiv = nullptr, *active = nullptr, *cand = nullptr;
-    uint32_t *L = nullptr, *rp = nullptr;
-    int8_t *dl = nullptr, *dr = nullptr;
-    Panel *pan = nullptr;
-    ListState *st = nullptr;
+    int32_t *rowpiv = ws.rowpiv, *colpiv = ws.colpiv, *active = ws.active, *cand = ws.cand;
+    uint32_t *L = ws.L, *rp = ws.rp;
+    int8_t *dl = ws.dl, *dr = ws.dr;
+    Panel *pan = ws.pan;
+    ListState *st = ws.st;
     const int64_t ldr = ((int64_t)n + 63) & ~63ll;
-    auto cleanup = [&]() {
-        dfree(rowpiv); dfree(colpiv); dfree(active); dfree(cand); dfree(L); dfree(rp);
-        dfree(dl); dfree(dr); dfree(pan); dfree(st);
-    };
     Digits dg;
     if (form == GBGPU_FORM_MFMA) digits_setup(dg, d->p, digits_for(d->p));
-    if ((rc = dalloc(ctx, &rowpiv, (size_t)m)) || (rc = dalloc(ctx, &colpiv, (size_t)n)) ||
-        (rc = dalloc(ctx, &active, (size_t)m)) || (rc = dalloc(ctx, &cand, (size_t)m)) ||
-        (rc = dalloc(ctx, &L, (size_t)m * PB)) || (rc = dalloc(ctx, &rp, (size_t)PB * ldr)) ||
-        (dg.nd && ((rc = dalloc(ctx, &dl, (size_t)dg.nd * m * PB)) ||
-                   (rc = dalloc(ctx, &dr, (size_t)dg.nd * ldr * PB)))) ||
-        (rc = dalloc(ctx, &pan, 2)) || (rc = dalloc(ctx, &st, 1))) {
-        cleanup();
-        return rc;
-    }
     dg.L = dl;
     dg.R = dr;
     dg.lplane = (int64_t)m * PB;
@@ -1061,7 +1105,7 @@ int dense_rref(gbgpu_dense *d, int form) {
             else k_update_plain<false><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, rp, ldr, L, active, md, r64);
         }
     };
-    if (hipEventRecord(ctx->ev[2], s) != hipSuccess) { cleanup(); return check_hip(ctx, hipGetLastError(), "event"); }
+    if (hipEventRecord(ctx->ev[2], s) != hipSuccess) { return check_hip(ctx, hipGetLastError(), "event"); }
     // Look-ahead: once the next panel's columns are updated, its select (one
     // workgroup) runs while the rest of this panel's Rp and update fill the
     // other CUs from the side stream.  The chain select - prune - Rp of the
@@ -1096,7 +1140,6 @@ int dense_rref(gbgpu_dense *d, int form) {
             update(s, cur, c0, c1, c2);
             if (hipEventRecord(ctx->sync_ev[0], s) != hipSuccess ||
                 hipStreamWaitEvent(x, ctx->sync_ev[0], 0) != hipSuccess) {
-                cleanup();
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
             if (c2 < n)
@@ -1105,14 +1148,12 @@ int dense_rref(gbgpu_dense *d, int form) {
             update(x, cur, c0, c0, c1);
             update(x, cur, c0, c2, n);
             if (hipEventRecord(ctx->sync_ev[1], x) != hipSuccess) {
-                cleanup();
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
             // the select reads the columns [c1, c2) of candidate rows, which
             // the side stream's work does not touch
             select(s, c1, nxt);
             if (hipStreamWaitEvent(s, ctx->sync_ev[1], 0) != hipSuccess) {
-                cleanup();
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
             k_prune_flags<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, n, c1, cand, st, active);
@@ -1121,24 +1162,18 @@ int dense_rref(gbgpu_dense *d, int form) {
             update(s, cur, c0, c0, n);
         }
         hipError_t e = hipGetLastError();
-        if (e != hipSuccess) { cleanup(); return check_hip(ctx, e, "dense panel launch"); }
+        if (e != hipSuccess) { return check_hip(ctx, e, "dense panel launch"); }
         if ((it + 1) % 8 == 0) {
             // stop once no candidate is left and the selected next panel is empty
             if (hipMemcpyAsync(&hs.done, &st->done, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
                 hipMemcpyAsync(&hs.k, &nxt->k, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess) {
-                cleanup();
                 return check_hip(ctx, hipGetLastError(), "dense panel sync");
             }
             if (hs.done && hs.k == 0) break;
         }
     }
     (void)hipEventRecord(ctx->ev[3], s);
-    rc = dense_output(d, colpiv);
-    cleanup();
-    if (rc) return rc;
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]) == hipSuccess) ctx->phase_ms[1] = ms;
     return GBGPU_OK;
 }
 

@@ -115,11 +115,27 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work = nullptr, int form = GBGPU_FOR
 
 // dense.hip: reduced echelon form of d (in place); form = GBGPU_FORM_*
 int dense_rref(gbgpu_dense *d, int form);
+
+struct Panel;
+struct ListState;
+// the device scratch of one echelon form
+struct DenseWs {
+    int32_t *rowpiv, *colpiv, *active, *cand;  // m, n, m, m
+    uint32_t *L, *rp;                          // m x 64, 64 x roundup(n, 64)
+    int8_t *dl, *dr;                           // digit planes (MFMA form)
+    Panel *pan;                                // two panel records
+    ListState *st;
+};
+// bytes of that scratch for an m x n block (pieces 256-byte aligned)
+size_t dense_ws_bytes(int32_t m, int32_t n, int form, uint32_t p);
+// lay the pieces out from base (dense_ws_bytes of room)
+void dense_ws_carve(uint8_t *base, int32_t m, int32_t n, int form, uint32_t p, DenseWs *ws);
+// the panel loop on scratch ws, no allocation: leaves ws.colpiv[c] = the
+// pivot row of column c (-1: none); times itself with ctx->ev[2], ev[3]
+int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws);
 // d->rank, d->orow, d->olead from the column -> pivot row map of an
 // eliminated block (decreasing lead order)
 int dense_output(gbgpu_dense *d, const int32_t *colpiv);
-
-struct Panel;
 
 // Signed base-256 digits for the int8 matrix cores.  With nd digits in
 // [-128, 127] the integers 127 S .. -128 S are reachable (S = 1 + 256 + ...
@@ -257,7 +273,6 @@ bool dense_route_pays(const gbgpu_mat *m);
 int run_dense_route(gbgpu_mat *m, const uint32_t *tinv);
 
 // dense.hip, batches: one dense block of a batch and its elimination state
-struct ListState;
 struct DenseDesc {
     uint32_t *a;
     int64_t ld;

@@ -173,6 +173,8 @@ def lib():
         "gbgpu_dense_free": ([vp], None),
         "gbgpu_dense_rref": ([vp, vp, C.c_int, pi32], C.c_int),
         "gbgpu_dense_download": ([vp, vp, vp, vp], C.c_int),
+        "gbgpu_dense_rref_noalloc_bytes": ([i32, i32, u32, C.c_int], C.c_size_t),
+        "gbgpu_dense_rref_noalloc": ([vp, vp, i32, i32, u32, C.c_int, vp, vp, pi32], C.c_int),
         "gbgpu_batch_upload": ([vp, C.POINTER(_Csr), i32, C.POINTER(vp)], C.c_int),
         "gbgpu_batch_la": ([vp, vp, C.c_int, pi32], C.c_int),
         "gbgpu_batch_rows_download": ([vp, vp, i32, C.POINTER(_Rows)], C.c_int),
@@ -285,6 +287,17 @@ class Context:
         b = f4(system, fn, C.cast(C.pointer(g), C.c_void_p), reduce_gb, info_level)
         b.stats["device_ms"] = g.device_ms
         return b
+
+    def dense_rref_noalloc(self, a_dev: int, m: int, n: int, p: int, form: int, rows_dev: int,
+                           lead_dev: int) -> int:
+        """gbgpu_dense_rref_noalloc on device buffers the caller owns (raw
+        device addresses): a (m x n, overwritten), rows (min(m, n) x n) and
+        lead (min(m, n)); returns the rank."""
+        r = C.c_int32(0)
+        self._check(lib().gbgpu_dense_rref_noalloc(self.h, C.c_void_p(a_dev), m, n, p, form,
+                                                   C.c_void_p(rows_dev), C.c_void_p(lead_dev),
+                                                   C.byref(r)), "dense rref (no allocation)")
+        return int(r.value)
 
     def dense(self, a: np.ndarray, p: int) -> "DeviceDense":
         a = np.ascontiguousarray(a, dtype=np.uint32)

@@ -22,6 +22,7 @@
 #ifndef GBGPU_H
 #define GBGPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -173,6 +174,25 @@ int gbgpu_dense_rref(gbgpu_ctx *ctx, gbgpu_dense *d, int form, int32_t *rank);
  * lead columns.  Either may be NULL. */
 int gbgpu_dense_download(gbgpu_ctx *ctx, gbgpu_dense *d, uint32_t *rows,
                          int32_t *lead);
+
+/* ---- the dense echelon form without allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (a serving
+ * loop or a harness that forbids hipMalloc).  The reduced echelon form of
+ * an m x n block mod p that the caller already holds in device memory
+ * (`a`: row-major, entries in [0, p), overwritten), by the kernels of
+ * gbgpu_dense_rref and with the result of gbgpu_dense_upload +
+ * gbgpu_dense_rref + gbgpu_dense_download: `rows` (device, min(m, n) x n)
+ * receives the rank full-width rows in decreasing lead order, `lead`
+ * (device, min(m, n)) their lead columns, *rank (host) the rank.
+ * Every scratch array comes from one static device array of
+ * GBGPU_NOALLOC_SCRATCH_MB MiB per device, which each device holds until
+ * the process ends; a block needing more (gbgpu_dense_rref_noalloc_bytes)
+ * is refused with GBGPU_ERANGE.  No hipMalloc or hipFree is called.
+ * Calls that overlap on one device share that array and race. */
+#define GBGPU_NOALLOC_SCRATCH_MB 64
+size_t gbgpu_dense_rref_noalloc_bytes(int32_t m, int32_t n, uint32_t p, int form);
+int gbgpu_dense_rref_noalloc(gbgpu_ctx *ctx, uint32_t *a, int32_t m, int32_t n, uint32_t p,
+                             int form, uint32_t *rows, int32_t *lead, int32_t *rank);
 
 /* ---- batches: la_option 1 over many matrices, each modulo its own prime.
  * No counterpart in the reference (its rational path is fraction-free over
