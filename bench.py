@@ -18,10 +18,15 @@ is timed beside it.  The JSON line also carries:
     entries streamed + row inputs + dense outputs) over its event-timed
     average duration, against 8 TB/s HBM;
   * cpu_baseline: the CPU oracle (the C restatement of the reference's
-    routine, OpenMP) on the same matrix on this host;
-  * extra: the full la_option 1 step (sparse pass + dense echelon form) on
-    the same matrix in both forms, and the dense 4096 x 8192 block
-    elimination in modular GOPS.
+    routine, OpenMP) on the same matrix on this host: median of 5 timed runs
+    on the host's CPU share, and a 1-thread median of 3;
+  * extra: the end-to-end F4 runs on 14 dense quadratics in 14 variables
+    (plain and MFMA forms, and the same run with the oracle's linear
+    algebra on the host's cores), the full la_option 1 step on the same
+    matrix in both forms, the dense 4096 x 8192 block elimination in
+    modular GOPS, and the 64-matrix batch.
+Every libgbgpu call synchronises its stream before it returns, so the host
+clock around a call covers its device work.
 """
 from __future__ import annotations
 
@@ -46,13 +51,42 @@ HBM_PEAK_GBS = 8000.0
 PMC_TRAFFIC_BYTES = 13.40e9 + 0.770e9
 
 
-def _sync():
+# the CPU share of one GPU on the benchmark host: worker pools never exceed it
+CPU_SHARE = 16
+
+
+def cpu_threads() -> int:
+    n = os.cpu_count() or 1
+    return int(os.environ.get("BENCH_CPU_THREADS", min(CPU_SHARE, n)))
+
+
+def cpu_model() -> str:
     try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
         pass
+    return "unknown"
+
+
+class _RoundLinesToStderr:
+    """Send what the C code prints on stdout (the F4 host's per-round lines)
+    to stderr, so that stdout carries the JSON line only."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        import ctypes
+        ctypes.CDLL(None).fflush(None)
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
 
 
 def dense_ops(m: int, n: int, r: int) -> float:
@@ -62,17 +96,20 @@ def dense_ops(m: int, n: int, r: int) -> float:
     return 2.0 * m * n * r - r * r * (m + n) + (2.0 / 3.0) * r ** 3 + r * r * (n - r)
 
 
-def cpu_baseline(m, nthreads: int):
-    """The oracle's sparse pass (C + OpenMP) over the same matrix."""
+def cpu_baseline(m, nthreads: int, runs: int) -> float:
+    """Median wall time of the oracle's sparse pass (C + OpenMP) over the
+    same matrix, after one warm-up run."""
     from oracle import oracle
     oracle.sparse_pass_work(m, nthreads=nthreads)  # warm-up (page-in)
-    t0 = time.perf_counter()
-    oracle.sparse_pass_work(m, nthreads=nthreads)
-    t = time.perf_counter() - t0
-    return t
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        oracle.sparse_pass_work(m, nthreads=nthreads)
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2]
 
 
-def f4_runs(nvars: int, ctx, with_cpu: bool, forms=("mfma", "plain"), cpu_vars: int = 12,
+def f4_runs(nvars: int, ctx, with_cpu: bool, forms=("mfma", "plain"), cpu_vars: int = 14,
             info: int = 0) -> dict:
     """End to end F4 (the F4 host, every round's linear algebra on the GPU)
     on a seeded dense quadratic system: nvars quadratics in nvars
@@ -90,10 +127,10 @@ def f4_runs(nvars: int, ctx, with_cpu: bool, forms=("mfma", "plain"), cpu_vars: 
     for form, name in ((native.FORM_MFMA, "mfma"), (native.FORM_PLAIN, "plain")):
         if name not in forms:
             continue
-        _sync()
+        print(f"bench: F4 on {nvars} variables, {name} form", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        b = ctx.f4(s, form=form, info_level=info)
-        _sync()
+        with _RoundLinesToStderr():
+            b = ctx.f4(s, form=form, info_level=info)
         runs[name] = b
         st = b.stats
         res[name] = {"total_s": time.perf_counter() - t0, "la_s": st["la_ms"] / 1e3,
@@ -116,14 +153,21 @@ def f4_runs(nvars: int, ctx, with_cpu: bool, forms=("mfma", "plain"), cpu_vars: 
         from oracle import oracle
         n_cpu = min(nvars, cpu_vars)
         s_cpu = s if n_cpu == nvars else synth.dense_quadratic_system(n_cpu, 65521, seed=1)
-        threads = int(os.environ.get("BENCH_CPU_THREADS", min(16, os.cpu_count() or 1)))
+        threads = cpu_threads()
+        print(f"bench: F4 on {n_cpu} variables, oracle LA on {threads} threads",
+              file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        c = oracle.f4(s_cpu, nthreads=threads, info_level=info)
+        with _RoundLinesToStderr():
+            c = oracle.f4(s_cpu, nthreads=threads, info_level=info)
         tc = time.perf_counter() - t0
-        cpu = {"vars": n_cpu, "threads": threads, "total_s": tc, "la_s": c.stats["la_ms"] / 1e3}
+        cpu = {"vars": n_cpu, "threads": threads, "total_s": tc, "la_s": c.stats["la_ms"] / 1e3,
+               "host_s": (c.stats["total_ms"] - c.stats["la_ms"]) / 1e3}
         if n_cpu == nvars:
             cpu["identical_to_gpu"] = bool(np.array_equal(c.lens, a.lens) and np.array_equal(c.exps, a.exps)
                                            and np.array_equal(c.cfs, a.cfs))
+            for name in runs:
+                res[f"{name}_vs_cpu_total"] = tc / res[name]["total_s"]
+                res[f"{name}_vs_cpu_la"] = cpu["la_s"] / res[name]["la_s"]
         else:
             g = ctx.f4(s_cpu, form=native.FORM_MFMA)
             cpu["gpu_mfma_total_s_same_system"] = g.stats["total_ms"] / 1e3
@@ -141,14 +185,15 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the extra workloads")
-    ap.add_argument("--f4-vars", type=int, default=12,
+    ap.add_argument("--f4-vars", type=int, default=14,
                     help="variables of the dense quadratic system run end to end by the F4 host "
                          "(14 = the 14-variable configuration; plain and MFMA forms)")
     ap.add_argument("--only-f4", action="store_true", help="only the end-to-end F4 runs")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-matrix batch extra")
     ap.add_argument("--f4-forms", default="mfma,plain", help="forms of the F4 runs")
-    ap.add_argument("--f4-info", action="store_true", help="print one line per F4 round")
-    ap.add_argument("--f4-cpu-vars", type=int, default=12,
+    ap.add_argument("--no-f4-info", action="store_true",
+                    help="no per-round lines of the F4 runs (on stderr)")
+    ap.add_argument("--f4-cpu-vars", type=int, default=14,
                     help="variables of the system the CPU (oracle LA) F4 run uses")
     args = ap.parse_args(argv)
     if args.gpus != 1:
@@ -163,7 +208,7 @@ def main(argv=None) -> int:
     if args.only_f4:
         print(json.dumps({"f4": f4_runs(args.f4_vars, ctx, not args.no_cpu,
                                         tuple(args.f4_forms.split(",")), args.f4_cpu_vars,
-                                        2 if args.f4_info else 0)}))
+                                        0 if args.no_f4_info else 2)}))
         return 0
     m = synth.f4_shaped(NR, NC, NPIV, P, seed=SEED)
     d = ctx.upload(m)
@@ -182,13 +227,11 @@ def main(argv=None) -> int:
     def timed(form, steps, warmup):
         for _ in range(warmup):
             d.sparse_pass(form)
-        _sync()
         kernel_ms = []
         t0 = time.perf_counter()
         for _ in range(steps):
             d.sparse_pass(form)
             kernel_ms.append(ctx.phase_ms(0))
-        _sync()
         return (time.perf_counter() - t0) / steps, sum(kernel_ms) / len(kernel_ms)
 
     # headline: the blocked form (MFMA build of the library); the plain
@@ -228,23 +271,29 @@ def main(argv=None) -> int:
                          "roofline_frac": bytes_per_launch / (kavg_plain / 1000.0) / 1e9 / HBM_PEAK_GBS}
 
     if not args.no_cpu:
-        threads = int(os.environ.get("BENCH_CPU_THREADS", min(16, os.cpu_count() or 1)))
-        tc = cpu_baseline(m, threads)
+        threads = cpu_threads()
+        print(f"bench: CPU baseline on {threads} threads", file=sys.stderr, flush=True)
+        tc = cpu_baseline(m, threads, 5)
+        tc1 = cpu_baseline(m, 1, 3)
         out["cpu_baseline"] = {"value": nrl / tc, "unit": "rows/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"the same matrix, all {nrl} to-reduce rows, one timed run after a warm-up"}
+                               "sample": f"the same matrix, all {nrl} to-reduce rows: median of 5 timed "
+                                         f"runs after a warm-up",
+                               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+                               "cpu_share": CPU_SHARE,
+                               "one_thread": {"value": nrl / tc1, "unit": "rows/s",
+                                              "sample": "median of 3 timed runs"},
+                               "gpu_vs_cpu": rows_per_s / (nrl / tc)}
 
     if not args.no_extra:
         extra = {}
         extra["f4"] = f4_runs(args.f4_vars, ctx, not args.no_cpu, tuple(args.f4_forms.split(",")),
-                              args.f4_cpu_vars, 2 if args.f4_info else 0)
+                              args.f4_cpu_vars, 0 if args.no_f4_info else 2)
         for form, name in ((native.FORM_PLAIN, "plain"), (native.FORM_MFMA, "mfma")):
             d2 = ctx.upload(m)
             d2.la_sparse_dense(form)  # warm-up
-            _sync()
             t0 = time.perf_counter()
             npiv = d2.la_sparse_dense(form)
-            _sync()
             extra[f"la_option1_{name}_ms"] = 1000.0 * (time.perf_counter() - t0)
             extra[f"la_option1_{name}_dense_ms"] = ctx.phase_ms(1)
             extra["la_option1_new_pivots"] = npiv
@@ -258,10 +307,8 @@ def main(argv=None) -> int:
                 dd.rref(form)
                 dd.free()
                 dd = ctx.dense(a, dp)
-                _sync()
                 t0 = time.perf_counter()
                 r = dd.rref(form)
-                _sync()
                 td = time.perf_counter() - t0
                 extra[f"{tag}_{name}_ms"] = 1000.0 * td
                 extra[f"{tag}_{name}_gops"] = dense_ops(4096, 8192, r) / td / 1e9
@@ -280,10 +327,8 @@ def main(argv=None) -> int:
             bt.la(form)  # device layout, tuning, graph capture
             ts, gs = [], []
             for _ in range(3):
-                _sync()
                 t0 = time.perf_counter()
                 bt.la(form)
-                _sync()
                 ts.append(time.perf_counter() - t0)
                 gs.append(ctx.phase_ms(2))
             tb = sorted(ts)[1]
@@ -296,11 +341,9 @@ def main(argv=None) -> int:
         singles = [ctx.upload(mm) for mm in mats]
         for dm in singles:
             dm.la_sparse_dense(native.FORM_MFMA)
-        _sync()
         t0 = time.perf_counter()
         for dm in singles:
             dm.la_sparse_dense(native.FORM_MFMA)
-        _sync()
         t1 = time.perf_counter() - t0
         if mats:
             extra["batch64_one_by_one_mfma_ms"] = 1000.0 * t1

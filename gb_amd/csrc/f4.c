@@ -938,6 +938,23 @@ static int take_row(f4_t *F, const gbgpu_rows_t *o, int32_t k, const int32_t *co
     return 0;
 }
 
+/* FNV-1a over the LA result (row lengths, columns, coefficients): the
+ * round's echelon form in 64 bits, for comparing runs round by round */
+static uint64_t rows_hash(const gbgpu_rows_t *o) {
+    uint64_t h = 0xCBF29CE484222325ull;
+#define MIX(v) do { h ^= (uint64_t)(uint32_t)(v); h *= 0x100000001B3ull; } while (0)
+    MIX(o->np);
+    for (int32_t k = 0; k < o->np; ++k) {
+        MIX(o->rp[k + 1] - o->rp[k]);
+        for (int64_t q = o->rp[k]; q < o->rp[k + 1]; ++q) {
+            MIX(o->ci[q]);
+            MIX(o->cf[q]);
+        }
+    }
+#undef MIX
+    return h;
+}
+
 static int f4_round(f4_t *F) {
     mt_t *T = &F->T;
     basis_t *B = &F->B;
@@ -1028,6 +1045,7 @@ static int f4_round(f4_t *F) {
         F->out->trace = tr;
         int64_t *x = tr + (size_t)r * GBGPU_F4_TRACE_W;
         x[0] = dmin; x[1] = nsel; x[2] = m.nr; x[3] = nc; x[4] = ncl; x[5] = nnz; x[6] = out.np;
+        x[7] = (int64_t)rows_hash(&out);
     }
     if (F->info > 1)
         printf("%3u  %6lld sel  %8lld pairs  %7d x %-7d (%lld spair rows)  %6.2f%%  %5d new  %8.1f ms LA\n",

@@ -226,7 +226,9 @@ int prep_buckets(gbgpu_mat *m, int force_wshift) {
 }
 
 // pivot entries per (window, group), derived from the per-pivot window
-// segments without atomics: one wave per group, lane t = pivot 64 g + t
+// segments without atomics: one wave per group, lane t = pivot 64 g + t.
+// An entry's top byte is the pivot's index modulo 256, i.e. within its
+// aligned chunk of four groups; the low 24 bits are its column.
 __global__ __launch_bounds__(64) void k_gbucket_count(int32_t ncl, int32_t ngroups, int32_t nwin,
                                                       const int64_t *seg, int64_t *gcnt) {
     const int lane = threadIdx.x;
@@ -246,7 +248,7 @@ __global__ __launch_bounds__(64) void k_gbucket_fill(int32_t ncl, int32_t ngroup
     const int lane = threadIdx.x;
     const int32_t g = blockIdx.x;
     const int32_t c = g * 64 + lane;
-    const uint32_t tag = (uint32_t)lane << 24;
+    const uint32_t tag = ((uint32_t)(g & 3) * 64u + (uint32_t)lane) << 24;
     for (int w = 0; w < nwin; ++w) {
         int64_t sb = 0;
         int len = 0;
@@ -540,24 +542,21 @@ __global__ __launch_bounds__(64) void k_group_inverse(int32_t ncl, int32_t wshif
     for (int k = 0; k < 64; ++k) out[k * 64 + lane] = Y[k][lane];
 }
 
-// add x_t * cf for every entry (t, column, cf) of one (window, group)
-// range: a coalesced stream, the multipliers of the group's 64 pivots in LDS
+// add x_t * cf for the entries [b, e) of the (window, group) layout: a
+// coalesced stream with eight loads in flight per lane; t is the entry's
+// pivot index within its chunk of four groups, x_t in sh_x[t]
 template <bool BIG>
-__device__ __forceinline__ void stream_group(uint64_t *acc, int32_t w0, uint32_t x, int64_t b,
-                                             int64_t e, const uint2 *gbucket, uint32_t *sh_x,
+__device__ __forceinline__ void stream_range(uint64_t *acc, int32_t w0, int64_t b, int64_t e,
+                                             const uint2 *gbucket, const uint32_t *sh_x,
                                              const Mod &md) {
     const int lane = threadIdx.x & 63;
-    if (!__any(x != 0) || e <= b) return;
-    sh_x[lane] = x;
-    __syncthreads();
     int64_t k = b + lane;
-    // four entries in flight per lane
-    for (; k + 192 < e; k += 256) {
-        uint2 en[4];
+    for (; k + 448 < e; k += 512) {
+        uint2 en[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) en[u] = gbucket[k + 64 * u];
+        for (int u = 0; u < 8; ++u) en[u] = gbucket[k + 64 * u];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
             const uint32_t xt = sh_x[en[u].x >> 24];
             atomicAdd(&acc[(en[u].x & 0xFFFFFFu) - (uint32_t)w0], addend<BIG>(xt, en[u].y, md));
         }
@@ -567,6 +566,19 @@ __device__ __forceinline__ void stream_group(uint64_t *acc, int32_t w0, uint32_t
         const uint32_t xt = sh_x[ent.x >> 24];
         atomicAdd(&acc[(ent.x & 0xFFFFFFu) - (uint32_t)w0], addend<BIG>(xt, ent.y, md));
     }
+}
+
+// one group's multipliers x (lane t: pivot 64 g + t) into their slot of
+// sh_x, then its entries [b, e)
+template <bool BIG>
+__device__ __forceinline__ void stream_group(uint64_t *acc, int32_t w0, uint32_t x, int slot,
+                                             int64_t b, int64_t e, const uint2 *gbucket,
+                                             uint32_t *sh_x, const Mod &md) {
+    const int lane = threadIdx.x & 63;
+    if (!__any(x != 0) || e <= b) return;
+    sh_x[slot * 64 + lane] = x;
+    __syncthreads();
+    stream_range<BIG>(acc, w0, b, e, gbucket, sh_x, md);
     __syncthreads();
 }
 
@@ -575,7 +587,7 @@ template <int WSHIFT>
 struct BlockedLds {
     uint64_t acc[1 << WSHIFT];
     uint32_t pb[(1 << WSHIFT) / 32];
-    uint32_t sh_x[64];
+    uint32_t sh_x[256];  // multipliers of one chunk of four groups
 };
 
 
@@ -612,14 +624,49 @@ __device__ __forceinline__ void blocked_row(const SpArgs &a, int q, uint32_t *xs
                 if (c >= w0 && c < w1) acc[c - w0] = rcf[k];
             }
             __syncthreads();
-            // groups of earlier windows (their multipliers are in xs); the
-            // last group of pivot columns may be partial
+            // groups of earlier windows (their multipliers are in xs; the
+            // last group of pivot columns may be partial), in aligned chunks
+            // of four groups: the entries of consecutive groups of one
+            // window are consecutive in the layout, so the chunk's groups
+            // with a nonzero multiplier form a few contiguous streams.  The
+            // next chunk's multipliers are loaded before this chunk streams.
             const int32_t jl = min(w0, a.ncl);
             const int32_t g_end = (jl + 63) >> 6;
-            for (int32_t gi = g_lead; gi < g_end; ++gi) {
-                const int32_t j = gi * 64 + lane;
-                const uint32_t x = j < jl ? xs[j] : 0u;
-                stream_group<BIG>(acc, w0, x, gs[gi], gs[gi + 1], gbucket, sh_x, a.md);
+            const int32_t jlo = g_lead << 6;
+            uint32_t xn[4];
+            int32_t c0 = g_lead & ~3;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int32_t j = (c0 + u) * 64 + lane;
+                xn[u] = (c0 < g_end && j >= jlo && j < jl) ? xs[j] : 0u;
+            }
+            for (; c0 < g_end; c0 += 4) {
+                uint32_t xv[4];
+                uint32_t nz = 0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    xv[u] = xn[u];
+                    nz |= (__any(xv[u] != 0) ? 1u : 0u) << u;
+                }
+                const int32_t c1 = c0 + 4;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int32_t j = (c1 + u) * 64 + lane;
+                    xn[u] = (c1 < g_end && j < jl) ? xs[j] : 0u;
+                }
+                if (nz == 0) continue;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) sh_x[u * 64 + lane] = xv[u];
+                __syncthreads();
+                // runs of consecutive groups with a nonzero multiplier
+                while (nz) {
+                    const int ga = __builtin_ctz(nz);
+                    int gb = ga;
+                    while (gb < 4 && ((nz >> gb) & 1u)) ++gb;
+                    nz &= ~(((1u << gb) - 1u) & ~((1u << ga) - 1u));
+                    stream_range<BIG>(acc, w0, gs[c0 + ga], gs[c0 + gb], gbucket, sh_x, a.md);
+                }
+                __syncthreads();
             }
             // this window's groups
             for (int32_t g = max(w0, g_lead << 6); g < lend; g += 64) {
@@ -648,7 +695,8 @@ __device__ __forceinline__ void blocked_row(const SpArgs &a, int q, uint32_t *xs
                 // also adds the group's own in-group entries: harmless (its
                 // accumulator values are consumed) and needed for right-hand
                 // columns sharing the last partial group
-                stream_group<BIG>(acc, w0, x, gs[g >> 6], gs[(g >> 6) + 1], gbucket, sh_x, a.md);
+                stream_group<BIG>(acc, w0, x, (g >> 6) & 3, gs[g >> 6], gs[(g >> 6) + 1], gbucket,
+                                  sh_x, a.md);
             }
             if (w1 > a.ncl) {
                 const int32_t cs = max(w0, a.ncl);

@@ -59,7 +59,7 @@ class _F4Out(C.Structure):
                 ("trace", C.POINTER(C.c_int64))]
 
 # fields of one round's record in _F4Out.trace (GBGPU_F4_TRACE_W in gbgpu_f4.h)
-F4_TRACE_FIELDS = ("degree", "pairs", "rows", "cols", "pivot_cols", "nnz", "new")
+F4_TRACE_FIELDS = ("degree", "pairs", "rows", "cols", "pivot_cols", "nnz", "new", "la_hash")
 
 
 class _F4Gpu(C.Structure):

@@ -59,11 +59,12 @@ typedef struct {
     /* one record of GBGPU_F4_TRACE_W values per round (rounds records):
      * degree, pairs selected, rows, columns, pivot columns, nonzeros, new
      * rows -- the matrix shape and result gb prints at info_level 2
-     * (symbol.c:72-75, convert.c:131, la_ff_32.c:1845) */
+     * (symbol.c:72-75, convert.c:131, la_ff_32.c:1845) -- and a 64-bit
+     * FNV-1a hash of the round's echelon form (the LA result rows) */
     int64_t *trace;
 } gbgpu_f4_out_t;
 
-#define GBGPU_F4_TRACE_W 7
+#define GBGPU_F4_TRACE_W 8
 
 /* Linear algebra of one round: interreduce == 0 is gb's linear_algebra on
  * the matrix m (data.h:269-273: known pivots = first row per lead, result

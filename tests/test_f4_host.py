@@ -156,7 +156,7 @@ def test_f4_host_katsura_12_matrices_per_round():
     (symbol.c:151-200).  The degree-9 row is gb's measured shape."""
     c = GOLDEN["katsura_12_drl_17_bit"]
     b = oracle.f4(system_of(c), nthreads=8, reduce_gb=0)
-    got = [tuple(r.values()) for r in b.stats["trace"]]
+    got = [tuple(r.values())[:7] for r in b.stats["trace"]]
     assert got == KATSURA_12_TRACE
     r9 = got[7]
     assert (r9[2], r9[3], r9[2] - r9[4], r9[6]) == (27707, 27798, 2064, 120)
