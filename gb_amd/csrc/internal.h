@@ -77,6 +77,7 @@ struct SpArgs {
     const int64_t *seg;
     const uint2 *bucket;
     uint32_t *xs;       // gridDim.x x ncl multipliers
+    uint32_t *xs2;      // their Shoup quotients (blocked pass, 32-bit accumulators)
     uint32_t *dprime;   // nrl x ncr
     int32_t *dnz;       // nrl
     int32_t *queue;     // work counter
@@ -96,7 +97,8 @@ struct BlockedArgs {
 int prep_split(gbgpu_mat *m, int32_t lim);
 // force_wshift: window of 2^force_wshift columns (0: the smallest that fits)
 int prep_buckets(gbgpu_mat *m, int force_wshift = 0);
-int prep_group_buckets(gbgpu_mat *m);
+// acc_shift: log2 of the accumulator size the entries' offsets are scaled by
+int prep_group_buckets(gbgpu_mat *m, int acc_shift = 3);
 // group buckets and the inverses of the pivot groups' diagonal blocks
 int prep_blocked(gbgpu_mat *m, uint32_t **tinv);
 // the kernel arguments of m (xs, queue, work left null)

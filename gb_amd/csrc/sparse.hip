@@ -243,6 +243,7 @@ __global__ __launch_bounds__(64) void k_gbucket_count(int32_t ncl, int32_t ngrou
 }
 
 __global__ __launch_bounds__(64) void k_gbucket_fill(int32_t ncl, int32_t ngroups, int32_t nwin,
+                                                     int32_t wshift, int32_t acc_shift,
                                                      const int64_t *seg, const uint2 *bucket,
                                                      const int64_t *gseg, uint2 *gbucket) {
     const int lane = threadIdx.x;
@@ -258,14 +259,15 @@ __global__ __launch_bounds__(64) void k_gbucket_fill(int32_t ncl, int32_t ngroup
         }
         const int incl = wave_incl_scan(len);
         const int64_t at = gseg[(int64_t)w * ngroups + g] + (incl - len);
+        const uint32_t w0 = (uint32_t)w << wshift;
         for (int k = 0; k < len; ++k) {
             const uint2 e = bucket[sb + k];
-            gbucket[at + k] = make_uint2(tag | e.x, e.y);
+            gbucket[at + k] = make_uint2(tag | ((e.x - w0) << acc_shift), e.y);
         }
     }
 }
 
-int prep_group_buckets(gbgpu_mat *m) {
+int prep_group_buckets(gbgpu_mat *m, int acc_shift) {
     gbgpu_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
     dfree(m->gseg);
@@ -283,8 +285,8 @@ int prep_group_buckets(gbgpu_mat *m) {
     if ((rc = exclusive_scan_i64(ctx, gcnt, m->gseg, nseg + 1, &total))) { dfree(gcnt); return rc; }
     if ((rc = dalloc(ctx, &m->gbucket, (size_t)total))) { dfree(gcnt); return rc; }
     if (ngroups > 0)
-        k_gbucket_fill<<<ngroups, 64, 0, s>>>(m->ncl, ngroups, m->nwin, m->seg, m->bucket, m->gseg,
-                                              m->gbucket);
+        k_gbucket_fill<<<ngroups, 64, 0, s>>>(m->ncl, ngroups, m->nwin, m->wshift, acc_shift, m->seg,
+                                              m->bucket, m->gseg, m->gbucket);
     GBGPU_HIP(ctx, hipGetLastError());
     GBGPU_HIP(ctx, hipStreamSynchronize(s));
     dfree(gcnt);
@@ -542,64 +544,118 @@ __global__ __launch_bounds__(64) void k_group_inverse(int32_t ncl, int32_t wshif
     for (int k = 0; k < 64; ++k) out[k * 64 + lane] = Y[k][lane];
 }
 
-// add x_t * cf for the entries [b, e) of the (window, group) layout: a
-// coalesced stream with eight loads in flight per lane; t is the entry's
-// pivot index within its chunk of four groups, x_t in sh_x[t]
-template <bool BIG>
-__device__ __forceinline__ void stream_range(uint64_t *acc, int32_t w0, int64_t b, int64_t e,
-                                             const uint2 *gbucket, const uint32_t *sh_x,
-                                             const Mod &md) {
-    const int lane = threadIdx.x & 63;
-    int64_t k = b + lane;
-    for (; k + 448 < e; k += 512) {
-        uint2 en[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) en[u] = gbucket[k + 64 * u];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const uint32_t xt = sh_x[en[u].x >> 24];
-            atomicAdd(&acc[(en[u].x & 0xFFFFFFu) - (uint32_t)w0], addend<BIG>(xt, en[u].y, md));
-        }
-    }
-    for (; k < e; k += 64) {
-        const uint2 ent = gbucket[k];
-        const uint32_t xt = sh_x[ent.x >> 24];
-        atomicAdd(&acc[(ent.x & 0xFFFFFFu) - (uint32_t)w0], addend<BIG>(xt, ent.y, md));
+// The entries of the (window, group) layout: x = (t << 24) | the byte
+// offset of the entry's column in the window's accumulator (column - w0
+// times the accumulator's size), y = the coefficient; t is the pivot's
+// index within its aligned chunk of four groups, its multiplier x_t in
+// sh_x[t].
+//
+// Accumulators: uint64 with the addends of common.h, or, for p < 2^16 when
+// (ncl + 1) (2p - 2) < 2^32 (the host checks it), uint32 with addends
+// below 2p: x c - floor(x' c / 2^32) p with x' = floor(x 2^32 / p) (Shoup's
+// precomputed quotient, kept beside x in sh_xp), which lies in [0, 2p) for
+// c < 2^32.  A column of a window receives at most one addend per pivot
+// plus the row's own entry.
+template <bool BIG, typename ACC>
+__device__ __forceinline__ void apply_entry(ACC *acc, uint2 en, const uint32_t *sh_x,
+                                            const uint32_t *sh_xp, const Mod &md) {
+    const uint32_t t = en.x >> 24;
+    ACC *dst = (ACC *)((char *)acc + (en.x & 0xFFFFFFu));
+    if constexpr (sizeof(ACC) == 4) {
+        const uint32_t x = sh_x[t], xp = sh_xp[t];
+        const uint32_t q = __umulhi(xp, en.y);
+        atomicAdd(dst, __umul24(x, en.y) - __umul24(q, md.p));
+    } else {
+        atomicAdd(dst, addend<BIG>(sh_x[t], en.y, md));
     }
 }
 
-// one group's multipliers x (lane t: pivot 64 g + t) into their slot of
-// sh_x, then its entries [b, e)
-template <bool BIG>
-__device__ __forceinline__ void stream_group(uint64_t *acc, int32_t w0, uint32_t x, int slot,
-                                             int64_t b, int64_t e, const uint2 *gbucket,
-                                             uint32_t *sh_x, const Mod &md) {
+// x' = floor(x 2^32 / p) for x < p < 2^16, in two exact 32-bit divisions
+__device__ __forceinline__ uint32_t shoup16(uint32_t x, uint32_t p) {
+    const uint32_t hi = (x << 16) / p, r = (x << 16) - hi * p;
+    return (hi << 16) + (r << 16) / p;
+}
+
+__device__ __forceinline__ void load_batch(uint2 (&r)[8], const uint2 *src) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) r[u] = src[64 * u];
+}
+
+template <bool BIG, typename ACC>
+__device__ __forceinline__ void apply_batch(ACC *acc, const uint2 (&r)[8], const uint32_t *sh_x,
+                                            const uint32_t *sh_xp, const Mod &md) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) apply_entry<BIG>(acc, r[u], sh_x, sh_xp, md);
+}
+
+// the entries [b, e): a coalesced stream in batches of 512 entries (eight
+// per lane); two register sets in turn, so that while one batch is applied
+// the next one's loads are in flight (every load of the steady loop is
+// unconditional)
+template <bool BIG, typename ACC>
+__device__ __forceinline__ void stream_range(ACC *acc, int64_t b, int64_t e, const uint2 *gbucket,
+                                             const uint32_t *sh_x, const uint32_t *sh_xp,
+                                             const Mod &md) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nb = (e - b) >> 9;  // full batches (wave-uniform)
+    const uint2 *src = gbucket + b + lane;
+    uint2 ra[8], rb[8];
+    if (nb > 0) load_batch(ra, src);
+    int64_t i = 0;
+    for (; i + 2 < nb; i += 2) {
+        load_batch(rb, src + 512 * (i + 1));
+        apply_batch<BIG>(acc, ra, sh_x, sh_xp, md);
+        load_batch(ra, src + 512 * (i + 2));
+        apply_batch<BIG>(acc, rb, sh_x, sh_xp, md);
+    }
+    if (nb - i == 2) {
+        load_batch(rb, src + 512 * (i + 1));
+        apply_batch<BIG>(acc, ra, sh_x, sh_xp, md);
+        apply_batch<BIG>(acc, rb, sh_x, sh_xp, md);
+    } else if (nb - i == 1) {
+        apply_batch<BIG>(acc, ra, sh_x, sh_xp, md);
+    }
+    for (int64_t k = b + (nb << 9) + lane; k < e; k += 64)
+        apply_entry<BIG>(acc, gbucket[k], sh_x, sh_xp, md);
+}
+
+// one group's multipliers x (lane t: pivot 64 g + t; xp their Shoup
+// quotients for 32-bit accumulators) into their slot, then its entries
+// [b, e)
+template <bool BIG, typename ACC>
+__device__ __forceinline__ void stream_group(ACC *acc, uint32_t x, uint32_t xp, int slot, int64_t b,
+                                             int64_t e, const uint2 *gbucket, uint32_t *sh_x,
+                                             uint32_t *sh_xp, const Mod &md) {
     const int lane = threadIdx.x & 63;
     if (!__any(x != 0) || e <= b) return;
     sh_x[slot * 64 + lane] = x;
+    if constexpr (sizeof(ACC) == 4) sh_xp[slot * 64 + lane] = xp;
     __syncthreads();
-    stream_range<BIG>(acc, w0, b, e, gbucket, sh_x, md);
+    stream_range<BIG>(acc, b, e, gbucket, sh_x, sh_xp, md);
     __syncthreads();
 }
 
 // LDS of one wave of the blocked pass
-template <int WSHIFT>
+template <int WSHIFT, typename ACC>
 struct BlockedLds {
-    uint64_t acc[1 << WSHIFT];
+    ACC acc[1 << WSHIFT];
     uint32_t pb[(1 << WSHIFT) / 32];
-    uint32_t sh_x[256];  // multipliers of one chunk of four groups
+    uint32_t sh_x[256];                           // multipliers of one chunk of four groups
+    uint32_t sh_xp[sizeof(ACC) == 4 ? 256 : 1];   // their Shoup quotients (32-bit accumulators)
 };
 
-
-template <int WSHIFT, bool BIG>
+template <int WSHIFT, bool BIG, typename ACC>
 __device__ __forceinline__ void blocked_row(const SpArgs &a, int q, uint32_t *xs,
-                                            BlockedLds<WSHIFT> &L, const uint32_t *tinv,
+                                            BlockedLds<WSHIFT, ACC> &L, const uint32_t *tinv,
                                             const int64_t *gseg, const uint2 *gbucket,
                                             int32_t ngroups) {
     constexpr int W = 1 << WSHIFT;
-    uint64_t *acc = L.acc;
+    constexpr bool A32 = sizeof(ACC) == 4;
+    ACC *acc = L.acc;
     uint32_t *pb = L.pb;
     uint32_t *sh_x = L.sh_x;
+    uint32_t *sh_xp = L.sh_xp;
+    uint32_t *xs2 = A32 ? a.xs2 + (xs - a.xs) : nullptr;  // this wave's Shoup quotients
     const int lane = threadIdx.x & 63;
     const uint32_t p = a.md.p;
     {
@@ -633,30 +689,38 @@ __device__ __forceinline__ void blocked_row(const SpArgs &a, int q, uint32_t *xs
             const int32_t jl = min(w0, a.ncl);
             const int32_t g_end = (jl + 63) >> 6;
             const int32_t jlo = g_lead << 6;
-            uint32_t xn[4];
+            uint32_t xn[4], xpn[4];
             int32_t c0 = g_lead & ~3;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int32_t j = (c0 + u) * 64 + lane;
-                xn[u] = (c0 < g_end && j >= jlo && j < jl) ? xs[j] : 0u;
+                const bool in = c0 < g_end && j >= jlo && j < jl;
+                xn[u] = in ? xs[j] : 0u;
+                xpn[u] = A32 && in ? xs2[j] : 0u;
             }
             for (; c0 < g_end; c0 += 4) {
-                uint32_t xv[4];
+                uint32_t xv[4], xpv[4];
                 uint32_t nz = 0;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     xv[u] = xn[u];
+                    xpv[u] = xpn[u];
                     nz |= (__any(xv[u] != 0) ? 1u : 0u) << u;
                 }
                 const int32_t c1 = c0 + 4;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int32_t j = (c1 + u) * 64 + lane;
-                    xn[u] = (c1 < g_end && j < jl) ? xs[j] : 0u;
+                    const bool in = c1 < g_end && j < jl;
+                    xn[u] = in ? xs[j] : 0u;
+                    xpn[u] = A32 && in ? xs2[j] : 0u;
                 }
                 if (nz == 0) continue;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) sh_x[u * 64 + lane] = xv[u];
+                for (int u = 0; u < 4; ++u) {
+                    sh_x[u * 64 + lane] = xv[u];
+                    if (A32) sh_xp[u * 64 + lane] = xpv[u];
+                }
                 __syncthreads();
                 // runs of consecutive groups with a nonzero multiplier
                 while (nz) {
@@ -664,7 +728,7 @@ __device__ __forceinline__ void blocked_row(const SpArgs &a, int q, uint32_t *xs
                     int gb = ga;
                     while (gb < 4 && ((nz >> gb) & 1u)) ++gb;
                     nz &= ~(((1u << gb) - 1u) & ~((1u << ga) - 1u));
-                    stream_range<BIG>(acc, w0, gs[c0 + ga], gs[c0 + gb], gbucket, sh_x, a.md);
+                    stream_range<BIG>(acc, gs[c0 + ga], gs[c0 + gb], gbucket, sh_x, sh_xp, a.md);
                 }
                 __syncthreads();
             }
@@ -690,13 +754,17 @@ __device__ __forceinline__ void blocked_row(const SpArgs &a, int q, uint32_t *xs
                     else sx += (uint64_t)ui * (uint64_t)tcol[i];
                 }
                 uint32_t x = a.md.reduce(sx);
-                if (c < lend) xs[c] = x;
-                else x = 0;
+                if (c >= lend) x = 0;
+                const uint32_t xp = A32 ? shoup16(x, p) : 0u;
+                if (c < lend) {
+                    xs[c] = x;
+                    if (A32) xs2[c] = xp;
+                }
                 // also adds the group's own in-group entries: harmless (its
                 // accumulator values are consumed) and needed for right-hand
                 // columns sharing the last partial group
-                stream_group<BIG>(acc, w0, x, (g >> 6) & 3, gs[g >> 6], gs[(g >> 6) + 1], gbucket,
-                                  sh_x, a.md);
+                stream_group<BIG>(acc, x, xp, (g >> 6) & 3, gs[g >> 6], gs[(g >> 6) + 1], gbucket,
+                                  sh_x, sh_xp, a.md);
             }
             if (w1 > a.ncl) {
                 const int32_t cs = max(w0, a.ncl);
@@ -714,11 +782,11 @@ __device__ __forceinline__ void blocked_row(const SpArgs &a, int q, uint32_t *xs
     }
 }
 
-template <int WSHIFT, bool BIG>
+template <int WSHIFT, bool BIG, typename ACC>
 __global__ __launch_bounds__(64) void k_sparse_blocked(SpArgs a, const uint32_t *tinv,
                                                        const int64_t *gseg, const uint2 *gbucket,
                                                        int32_t ngroups) {
-    __shared__ BlockedLds<WSHIFT> L;
+    __shared__ BlockedLds<WSHIFT, ACC> L;
     const int lane = threadIdx.x;
     uint32_t *xs = a.xs + (size_t)blockIdx.x * (size_t)(a.ncl > 0 ? a.ncl : 1);
     for (;;) {
@@ -726,7 +794,7 @@ __global__ __launch_bounds__(64) void k_sparse_blocked(SpArgs a, const uint32_t 
         const int q0 = atomicAdd(a.queue, lane == 0 ? 1 : 0);
         const int q = __builtin_amdgcn_readfirstlane(q0);
         if (q >= a.nrl) break;
-        blocked_row<WSHIFT, BIG>(a, a.nrl - 1 - q, xs, L, tinv, gseg, gbucket, ngroups);
+        blocked_row<WSHIFT, BIG, ACC>(a, a.nrl - 1 - q, xs, L, tinv, gseg, gbucket, ngroups);
     }
 }
 
@@ -755,7 +823,7 @@ __global__ __launch_bounds__(64) void k_sparse_blocked_batch(const BlockedArgs *
                                                              int32_t nmat, int32_t *queue,
                                                              uint32_t *xs_all,
                                                              int64_t xs_stride) {
-    __shared__ BlockedLds<WSHIFT> L;
+    __shared__ BlockedLds<WSHIFT, uint64_t> L;
     const int lane = threadIdx.x;
     uint32_t *xs = xs_all + (int64_t)blockIdx.x * xs_stride;
     const int32_t total = __builtin_amdgcn_readfirstlane(row_off[nmat]);
@@ -767,7 +835,7 @@ __global__ __launch_bounds__(64) void k_sparse_blocked_batch(const BlockedArgs *
         const BlockedArgs &b = mats[i];
         const int32_t nrl = __builtin_amdgcn_readfirstlane(b.a.nrl);
         const int32_t r = nrl - 1 - (q - __builtin_amdgcn_readfirstlane(row_off[i]));
-        blocked_row<WSHIFT, BIG>(b.a, r, xs, L, b.tinv, b.gseg, b.gbucket, b.ngroups);
+        blocked_row<WSHIFT, BIG, uint64_t>(b.a, r, xs, L, b.tinv, b.gseg, b.gbucket, b.ngroups);
     }
 }
 
@@ -808,18 +876,31 @@ static void launch_sparse(const SpArgs &a, bool big, bool count, int grid, hipSt
     }
 }
 
+// LDS bytes of one wave of the blocked pass
+template <typename ACC>
+static size_t blocked_lds_bytes(int wshift) {
+    switch (wshift) {
+        case 10: return sizeof(BlockedLds<10, ACC>);
+        case 11: return sizeof(BlockedLds<11, ACC>);
+        case 12: return sizeof(BlockedLds<12, ACC>);
+        default: return sizeof(BlockedLds<13, ACC>);
+    }
+}
+
 template <int WS>
 static void launch_blocked(const SpArgs &a, const uint32_t *tinv, const int64_t *gseg,
-                           const uint2 *gbucket, int32_t ngroups, bool big, int grid, hipStream_t s) {
-    if (big) k_sparse_blocked<WS, true><<<grid, 64, 0, s>>>(a, tinv, gseg, gbucket, ngroups);
-    else k_sparse_blocked<WS, false><<<grid, 64, 0, s>>>(a, tinv, gseg, gbucket, ngroups);
+                           const uint2 *gbucket, int32_t ngroups, bool big, bool acc32, int grid,
+                           hipStream_t s) {
+    if (big) k_sparse_blocked<WS, true, uint64_t><<<grid, 64, 0, s>>>(a, tinv, gseg, gbucket, ngroups);
+    else if (acc32) k_sparse_blocked<WS, false, uint32_t><<<grid, 64, 0, s>>>(a, tinv, gseg, gbucket, ngroups);
+    else k_sparse_blocked<WS, false, uint64_t><<<grid, 64, 0, s>>>(a, tinv, gseg, gbucket, ngroups);
 }
 
 void sparse_args(const gbgpu_mat *m, SpArgs *a) {
     a->nc = m->nc; a->ncl = m->ncl; a->ncr = m->nc - m->ncl; a->nrl = m->nrl; a->nwin = m->nwin;
     a->rp = m->rp; a->ci = m->ci; a->rcf = m->rcf; a->cp = m->cp; a->cf = m->cf;
     a->upiv = m->upiv; a->pbits = m->pbits; a->seg = m->seg; a->bucket = m->bucket;
-    a->xs = nullptr; a->dprime = m->dprime; a->dnz = m->dnz;
+    a->xs = nullptr; a->xs2 = nullptr; a->dprime = m->dprime; a->dnz = m->dnz;
     a->queue = nullptr; a->work = nullptr;
     a->md = Mod::make(m->p);
 }
@@ -878,8 +959,16 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
     // columns (when at most 63 of them cover the matrix) fit more waves per
     // CU than 2048-column ones (blocked: 16 against 9, 12 % faster on a
     // 20000 x 30000 matrix)
+    // the blocked form with 32-bit accumulators (p < 2^16 and at most
+    // (2^32 - 1) / (2p - 2) addends per column: ncl pivots and the row's own
+    // entry) keeps 2048 columns in the space of 1024: half the windows, so
+    // half the passes over the earlier groups' entries, at the same
+    // occupancy
+    const bool acc32 = form == GBGPU_FORM_MFMA && work == nullptr && m->p < (1u << 16) &&
+                       ((uint64_t)m->ncl + 1) * (2ull * m->p - 2) < (1ull << 32);
     int wshift = 0;
     if (((int64_t)m->nc + 1023) / 1024 <= 63) wshift = 10;
+    if (acc32 && ((int64_t)m->nc + 2047) / 2048 <= 63) wshift = 11;
     if ((rc = prep_buckets(m, wshift))) return rc;
     const int32_t ncr = m->nc - m->ncl;
     dfree(m->dprime); dfree(m->dnz); dfree(m->dlist);
@@ -893,7 +982,7 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
     const bool dense = blocked && dense_route_pays(m);
     uint32_t *tinv = nullptr;
     const int32_t ngroups = (m->ncl + 63) / 64;
-    if (blocked && !dense && (rc = prep_group_buckets(m))) return rc;
+    if (blocked && !dense && (rc = prep_group_buckets(m, acc32 ? 2 : 3))) return rc;
     if (blocked && ngroups > 0) {
         if ((rc = dalloc(ctx, &tinv, (size_t)ngroups * 4096))) return rc;
         k_group_inverse<<<ngroups, 64, 0, s>>>(m->ncl, m->wshift, m->pbits, m->seg, m->bucket,
@@ -902,13 +991,16 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
     }
     // waves in flight: LDS of one wave is W*8 bytes + staging, 160 KB per CU
     const int W = 1 << m->wshift;
-    const int per_cu = (160 * 1024) / (W * 8 + (blocked ? 0 : STAGE * 8) + 2048);
+    const int lds = blocked ? (int)(acc32 ? blocked_lds_bytes<uint32_t>(m->wshift) : blocked_lds_bytes<uint64_t>(m->wshift))
+                            : W * 8 + STAGE * 8 + 2048;
+    const int per_cu = (160 * 1024) / lds;
     int grid = ctx->num_cu * (per_cu > 0 ? per_cu : 1);
     if (grid > m->nrl) grid = m->nrl > 0 ? m->nrl : 1;
     uint32_t *xs = nullptr;
     int32_t *queue = nullptr;
     // per-wave multipliers of the row-parallel kernels (the dense route has none)
-    const size_t nxs = dense ? 1 : (size_t)grid * (size_t)(m->ncl > 0 ? m->ncl : 1);
+    // (with 32-bit accumulators followed by as many Shoup quotients)
+    const size_t nxs = dense ? 1 : (size_t)grid * (size_t)(m->ncl > 0 ? m->ncl : 1) * (acc32 ? 2 : 1);
     if ((rc = dalloc(ctx, &xs, nxs))) { dfree(tinv); return rc; }
     if ((rc = dalloc(ctx, &queue, 6))) { dfree(xs); dfree(tinv); return rc; }
     GBGPU_HIP(ctx, hipMemsetAsync(queue, 0, 6 * sizeof(int32_t), s));
@@ -917,6 +1009,7 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
     a.rp = m->rp; a.ci = m->ci; a.rcf = m->rcf; a.cp = m->cp; a.cf = m->cf;
     a.upiv = m->upiv; a.pbits = m->pbits; a.seg = m->seg; a.bucket = m->bucket;
     a.xs = xs; a.dprime = m->dprime; a.dnz = m->dnz; a.queue = queue;
+    a.xs2 = acc32 ? xs + (size_t)grid * (size_t)(m->ncl > 0 ? m->ncl : 1) : nullptr;
     a.work = (unsigned long long *)(queue + 2);  // 8-byte aligned inside the 24-byte block
     a.md = Mod::make(m->p);
     const bool big = m->p >= (1u << 17);
@@ -925,10 +1018,10 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
         if ((rc = run_dense_route(m, tinv))) { dfree(xs); dfree(queue); dfree(tinv); return rc; }
     } else if (m->nrl > 0 && blocked) {
         switch (m->wshift) {
-            case 10: launch_blocked<10>(a, tinv, m->gseg, m->gbucket, ngroups, big, grid, s); break;
-            case 11: launch_blocked<11>(a, tinv, m->gseg, m->gbucket, ngroups, big, grid, s); break;
-            case 12: launch_blocked<12>(a, tinv, m->gseg, m->gbucket, ngroups, big, grid, s); break;
-            default: launch_blocked<13>(a, tinv, m->gseg, m->gbucket, ngroups, big, grid, s); break;
+            case 10: launch_blocked<10>(a, tinv, m->gseg, m->gbucket, ngroups, big, acc32, grid, s); break;
+            case 11: launch_blocked<11>(a, tinv, m->gseg, m->gbucket, ngroups, big, acc32, grid, s); break;
+            case 12: launch_blocked<12>(a, tinv, m->gseg, m->gbucket, ngroups, big, acc32, grid, s); break;
+            default: launch_blocked<13>(a, tinv, m->gseg, m->gbucket, ngroups, big, acc32, grid, s); break;
         }
     } else if (m->nrl > 0) {
         switch (m->wshift) {
