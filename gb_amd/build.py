@@ -21,7 +21,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
 
 SOURCES = ["api.hip", "sparse.hip", "dense.hip", "rows.hip", "batch.hip", "dsparse.hip",
            "noalloc.hip"]
-C_SOURCES = ["gb_adapter.c", "f4.c"]
+C_SOURCES = ["gb_adapter.c", "f4.c", "f4_multi.c"]
 
 
 def _compile(src: str) -> str:

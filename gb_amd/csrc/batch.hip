@@ -73,6 +73,8 @@ struct TuneKey {
                std::tie(o.form, o.nmat, o.rows, o.nnz, o.max_nc);
     }
 };
+// batches with fewer rows to reduce than this are not tuned
+constexpr int64_t SMALL_BATCH_ROWS = 20000;
 std::mutex g_tune_mu;
 std::map<TuneKey, std::pair<int, int>> g_tuned;  // -> (wshift, waves per CU)
 
@@ -176,7 +178,7 @@ int tune(gbgpu_batch *b, int form, int *best_ws, int *best_wpc) {
     float best = -1;
     int rc;
     for (int ws = 10; ws <= 13; ++ws) {
-        if ((((int64_t)max_nc + (1 << ws) - 1) >> ws) > 63) continue;
+        if ((((int64_t)max_nc + (1 << ws) - 1) >> ws) > 63 && ws < 13) continue;
         if ((rc = layout_sparse(b, form, ws))) return rc;
         const int mw = max_waves(ws, form);
         for (int wpc : {mw, std::max(1, mw / 2)}) {
@@ -311,6 +313,14 @@ int prepare(gbgpu_batch *b, int form) {
             ws = it->second.first;
             wpc = it->second.second;
         }
+    }
+    if (!ws && key.rows < SMALL_BATCH_ROWS) {
+        // a small batch (the images of a multimodular round, say) runs in
+        // less time than timing the candidates would take: the smallest
+        // window that covers the widest matrix in 63, at full occupancy
+        ws = 10;
+        while ((((int64_t)key.max_nc + (1 << ws) - 1) >> ws) > 63 && ws < 13) ++ws;
+        wpc = max_waves(ws, form);
     }
     if (!ws) {
         if ((rc = tune(b, form, &ws, &wpc))) return rc;

@@ -578,7 +578,7 @@ static int marks_fit(marks_t *M, int32_t n) {
     return 0;
 }
 
-static const mt_t *g_sort_t;
+static _Thread_local const mt_t *g_sort_t;  /* per thread: images may run side by side */
 static int cmp_mon_desc(const void *a, const void *b) {
     return -mt_cmp(g_sort_t, *(const int32_t *)a, *(const int32_t *)b);
 }
@@ -1363,6 +1363,7 @@ int64_t gbgpu_f4_julia(int32_t *bld, int32_t **blen, int32_t **bexp, void **bcf,
         return 0;
     }
     gbgpu_finalize(ctx);
+    free(out.trace);
     *bld = out.bld;
     *blen = out.blen;
     *bexp = out.bexp;

@@ -140,19 +140,21 @@ __global__ void k_pivlen(int32_t ncl, const int32_t *first, const int64_t *rp, i
     if (c < ncl && lane == 32) pbits[c >> 5] = (uint32_t)(bal >> 32);
 }
 
-// one wave per pivot column: histogram by window, scan, scatter
+// one wave per pivot column: histogram by window (counters 0 .. nwin rounded
+// up to whole chunks of 64, in dynamic LDS), scan chunk by chunk, scatter
+constexpr int MAX_WIN = 4096;
 __global__ __launch_bounds__(64) void k_bucket(int32_t ncl, int32_t nwin, int32_t wshift,
                                                const int32_t *first, const int64_t *rp,
                                                const int32_t *ci, const int32_t *rcf,
                                                const int64_t *cp, const uint32_t *cf,
                                                const int64_t *pbase, int64_t *seg, uint2 *bucket) {
-    __shared__ int32_t cnt[64];
-    __shared__ int32_t pos[64];
+    extern __shared__ int32_t pos[];
     const int lane = threadIdx.x;
+    const int32_t nw64 = (nwin + 1 + 63) & ~63;  // counters 0 .. nwin, whole chunks
     for (int32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
         const int32_t r = first[c];
         const int64_t base = pbase[c];
-        cnt[lane] = 0;
+        for (int32_t i = lane; i < nw64; i += 64) pos[i] = 0;
         __syncthreads();
         int64_t b = 0, e = 0;
         const uint32_t *rc = nullptr;
@@ -161,12 +163,19 @@ __global__ __launch_bounds__(64) void k_bucket(int32_t ncl, int32_t nwin, int32_
             e = rp[r + 1];
             rc = cf + cp[rcf[r]] - rp[r];
         }
-        for (int64_t k = b + lane; k < e; k += 64) atomicAdd(&cnt[ci[k] >> wshift], 1);
+        for (int64_t k = b + lane; k < e; k += 64) atomicAdd(&pos[ci[k] >> wshift], 1);
         __syncthreads();
-        const int v = lane < nwin ? cnt[lane] : 0;
-        const int incl = wave_incl_scan(v);
-        pos[lane] = incl - v;
-        if (lane <= nwin) seg[(int64_t)lane * ncl + c] = base + (incl - v);
+        // exclusive scan of the counts; seg[w] for w = 0 .. nwin
+        int32_t carry = 0;
+        for (int32_t w0 = 0; w0 < nw64; w0 += 64) {
+            const int32_t w = w0 + lane;
+            const int v = w < nwin ? pos[w] : 0;
+            const int incl = wave_incl_scan(v);
+            const int32_t start = carry + incl - v;
+            pos[w] = start;
+            if (w <= nwin) seg[(int64_t)w * ncl + c] = base + start;
+            carry += __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+        }
         __syncthreads();
         for (int64_t k = b + lane; k < e; k += 64) {
             const int32_t col = ci[k];
@@ -185,12 +194,13 @@ int prep_buckets(gbgpu_mat *m, int force_wshift) {
     dfree(m->bucket);
     dfree(m->pbits);
     // unless a width is forced: the smallest window (2048 .. 8192 columns)
-    // that keeps nwin <= 63
+    // that keeps nwin <= 63, else 8192-column windows, as many as it takes
     int wshift = 11;
     while (((int64_t)m->nc + (1 << wshift) - 1) >> wshift > 63 && wshift < 13) ++wshift;
     if (force_wshift) wshift = force_wshift;
-    const int nwin = (int)(((int64_t)m->nc + (1 << wshift) - 1) >> wshift);
-    if (nwin > 63) return set_error(ctx, GBGPU_ERANGE, "too many columns for the sparse pass");
+    const int64_t nwin64 = ((int64_t)m->nc + (1 << wshift) - 1) >> wshift;
+    if (nwin64 > MAX_WIN) return set_error(ctx, GBGPU_ERANGE, "too many columns for the sparse pass");
+    const int nwin = (int)nwin64;
     m->wshift = wshift;
     m->nwin = nwin;
     const int32_t ncl = m->ncl;
@@ -215,7 +225,8 @@ int prep_buckets(gbgpu_mat *m, int force_wshift) {
     }
     if (ncl > 0) {
         const int g = ncl < 65536 * 4 ? ncl : 65536 * 4;
-        k_bucket<<<g, 64, 0, s>>>(ncl, nwin, wshift, m->first, m->rp, m->ci, m->rcf, m->cp, m->cf,
+        const size_t lds = (size_t)((nwin + 1 + 63) & ~63) * sizeof(int32_t);
+        k_bucket<<<g, 64, lds, s>>>(ncl, nwin, wshift, m->first, m->rp, m->ci, m->rcf, m->cp, m->cf,
                                   pbase, m->seg, m->bucket);
     }
     GBGPU_HIP(ctx, hipGetLastError());

@@ -62,6 +62,10 @@ class _F4Out(C.Structure):
 F4_TRACE_FIELDS = ("degree", "pairs", "rows", "cols", "pivot_cols", "nnz", "new", "la_hash")
 
 
+class _F4MultiStats(C.Structure):
+    _fields_ = [("batches", C.c_int64), ("matrices", C.c_int64), ("device_ms", C.c_double)]
+
+
 class _F4Gpu(C.Structure):
     _fields_ = [("ctx", C.c_void_p), ("form", C.c_int32), ("device_ms", C.c_double)]
 
@@ -105,20 +109,32 @@ def f4(system: PolySystem, la_fn: int, la_user, reduce_gb: int = 0, info_level: 
     """Runs the F4 host with the linear-algebra callback at address la_fn
     (a gbgpu_f4_la_t) and its user pointer."""
     L = lib()
+    inp, _arrays = _f4_in(system, reduce_gb, info_level)
+    out = _F4Out()
+    rc = L.gbgpu_f4(C.byref(inp), C.c_void_p(la_fn), la_user, C.byref(out))
+    if rc != 0:
+        raise GpuError(f"gbgpu_f4 failed ({rc})")
+    b = _basis(out, system.nv)
+    L.gbgpu_f4_out_free(C.byref(out))
+    return b
+
+
+def _f4_in(system: PolySystem, reduce_gb: int, info_level: int):
     lens = np.ascontiguousarray(system.lens, np.int32)
     exps = np.ascontiguousarray(system.exps, np.int32).reshape(-1)
     cfs = np.ascontiguousarray(system.cfs, np.int32)
     pi = C.POINTER(C.c_int32)
     inp = _F4In(system.nv, len(lens), lens.ctypes.data_as(pi), exps.ctypes.data_as(pi),
                 cfs.ctypes.data_as(pi), system.p, system.order, 1, reduce_gb, info_level)
-    out = _F4Out()
-    rc = L.gbgpu_f4(C.byref(inp), C.c_void_p(la_fn), la_user, C.byref(out))
-    if rc != 0:
-        raise GpuError(f"gbgpu_f4 failed ({rc})")
+    return inp, (lens, exps, cfs)
+
+
+def _basis(out: "_F4Out", nv: int) -> Basis:
     nt = int(out.nterms)
-    b = Basis(nv=system.nv,
+    system_nv = nv
+    return Basis(nv=system_nv,
               lens=np.ctypeslib.as_array(out.blen, shape=(out.bld,)).copy() if out.bld else np.zeros(0, np.int32),
-              exps=np.ctypeslib.as_array(out.bexp, shape=(nt * system.nv,)).copy().reshape(nt, system.nv) if nt else np.zeros((0, system.nv), np.int32),
+              exps=np.ctypeslib.as_array(out.bexp, shape=(nt * nv,)).copy().reshape(nt, nv) if nt else np.zeros((0, nv), np.int32),
               cfs=np.ctypeslib.as_array(out.bcf, shape=(nt,)).copy() if nt else np.zeros(0, np.uint32),
               stats={"rounds": out.rounds, "max_rows": out.max_rows, "max_cols": out.max_cols,
                      "max_nnz": out.max_nnz, "rows_total": out.rows_total, "la_ms": out.la_ms,
@@ -128,8 +144,6 @@ def f4(system: PolySystem, la_fn: int, la_user, reduce_gb: int = 0, info_level: 
                      "trace": ([dict(zip(F4_TRACE_FIELDS, map(int, row))) for row in
                                 np.ctypeslib.as_array(out.trace, shape=(out.rounds, len(F4_TRACE_FIELDS)))]
                                if out.rounds and out.trace else [])})
-    L.gbgpu_f4_out_free(C.byref(out))
-    return b
 
 
 def declared_symbols() -> list[str]:
@@ -181,6 +195,8 @@ def lib():
         "gbgpu_batch_info": ([vp, pi32, i32], C.c_int),
         "gbgpu_batch_free": ([vp], None),
         "gbgpu_f4": ([C.POINTER(_F4In), vp, vp, C.POINTER(_F4Out)], C.c_int),
+        "gbgpu_f4_multimod": ([vp, C.POINTER(_F4In), i32, i32, C.POINTER(_F4Out),
+                               C.POINTER(_F4MultiStats)], C.c_int),
         "gbgpu_f4_out_free": ([C.POINTER(_F4Out)], None),
     }
     for name, (args, res) in sig.items():
@@ -287,6 +303,30 @@ class Context:
         b = f4(system, fn, C.cast(C.pointer(g), C.c_void_p), reduce_gb, info_level)
         b.stats["device_ms"] = g.device_ms
         return b
+
+    def f4_multimod(self, systems, form: int = FORM_MFMA, reduce_gb: int = 0):
+        """F4 of several images (one PolySystem per prime) in lockstep, every
+        round's matrices of all images in one batched launch
+        (gbgpu_f4_multimod).  Returns (bases, stats)."""
+        L = lib()
+        n = len(systems)
+        ins = (_F4In * max(n, 1))()
+        keep = []
+        for i, s in enumerate(systems):
+            ins[i], arrs = _f4_in(s, reduce_gb, 0)
+            keep.append(arrs)
+        outs = (_F4Out * max(n, 1))()
+        st = _F4MultiStats()
+        rc = L.gbgpu_f4_multimod(self.h, ins, n, form, outs, C.byref(st))
+        try:
+            if rc != 0:
+                raise GpuError(f"gbgpu_f4_multimod failed ({rc}): {L.gbgpu_last_error(self.h).decode()}")
+            bases = [_basis(outs[i], systems[i].nv) for i in range(n)]
+        finally:
+            for i in range(n):
+                L.gbgpu_f4_out_free(C.byref(outs[i]))
+        return bases, {"batches": int(st.batches), "matrices": int(st.matrices),
+                       "device_ms": float(st.device_ms)}
 
     def dense_rref_noalloc(self, a_dev: int, m: int, n: int, p: int, form: int, rows_dev: int,
                            lead_dev: int) -> int:

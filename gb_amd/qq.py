@@ -22,7 +22,12 @@ bases the multimodular way instead:
    so its leading coefficient is positive (the form gb's QQ output has).
 
 The F4 runner is a callable ``run(system) -> Basis``; the product path passes
-the GPU runner (:func:`gpu_runner`).  The reduced basis (reduce_gb = 1) is
+the GPU runner (:func:`gpu_runner`).  With ``run_many(systems) -> [Basis]``
+(:func:`gpu_batch_runner`) the images are computed a block of primes at a
+time, the F4 runs of a block in lockstep with every round's matrices of all
+its images reduced in one batched launch; the images are then consumed in
+prime order exactly as one at a time, so the lift and the primes it uses are
+the same.  The reduced basis (reduce_gb = 1) is
 unique, so its lift is independent of how each image was computed; the
 non-reduced basis is the one gb's own F4 order of operations produces, which
 the F4 host reproduces for every prime (its bases equal gb's over prime
@@ -140,13 +145,36 @@ def _integral(lens: Sequence[int], fr: Sequence[Fraction]) -> List[int]:
     return out
 
 
-def groebner_qq(run: Callable[[native.PolySystem], native.Basis], nv: int, lens, exps, num, den,
-                order: int = 0, primes: Optional[Sequence[int]] = None,
-                max_primes: int = 256) -> QQBasis:
+def _images(run, run_many, make, primes_ok: Iterator[int]):
+    """(p, image basis) in prime order: one at a time with ``run``, or in
+    blocks of 3, 4, 8, 16, ... primes with ``run_many`` (two images to lift
+    and one to check, then doubling)."""
+    if run_many is None:
+        for p in primes_ok:
+            yield p, run(make(p))
+        return
+    size = 3
+    while True:
+        block = []
+        for p in primes_ok:
+            block.append(p)
+            if len(block) == size:
+                break
+        if not block:
+            return
+        for p, b in zip(block, run_many([make(p) for p in block])):
+            yield p, b
+        size = 4 if size == 3 else 2 * size
+
+
+def groebner_qq(run: Optional[Callable[[native.PolySystem], native.Basis]], nv: int, lens, exps,
+                num, den, order: int = 0, primes: Optional[Sequence[int]] = None,
+                max_primes: int = 256, run_many=None) -> QQBasis:
     """Basis over QQ of the system with coefficients num[i] / den[i] (den > 0
     or negative; zero denominators are an error), from the images the
-    runner computes.  Raises RuntimeError if max_primes images do not
-    settle the lift."""
+    runner computes (``run`` one image at a time, or ``run_many`` a block of
+    images at once).  Raises RuntimeError if max_primes images do not settle
+    the lift."""
     num = [int(x) for x in num]
     den = [int(x) for x in den]
     if len(num) != len(den) or len(num) != int(np.sum(lens)):
@@ -154,16 +182,22 @@ def groebner_qq(run: Callable[[native.PolySystem], native.Basis], nv: int, lens,
     if any(d == 0 for d in den):
         raise ValueError("zero denominator")
     source = iter(primes) if primes is not None else primes_below()
+
+    def usable() -> Iterator[int]:
+        n = 0
+        for q in source:
+            if n >= max_primes:
+                return
+            if any(d % q == 0 for d in den):
+                continue
+            n += 1
+            yield q
+
     groups = {}        # support -> [modulus, CRT residues (ints), primes]
     cand = None        # (support, fractions): the current lift of the largest group
     used = 0
-    for p in source:
-        if used >= max_primes:
-            break
-        if any(d % p == 0 for d in den):
-            continue
+    for p, b in _images(run, run_many, lambda q: _image(nv, lens, exps, num, den, q, order), usable()):
         used += 1
-        b = run(_image(nv, lens, exps, num, den, p, order))
         key = (tuple(int(x) for x in b.lens), np.asarray(b.exps, np.int32).tobytes())
         cf = [int(x) for x in b.cfs]
         if cand is not None and cand[0] == key and all(
@@ -197,3 +231,15 @@ def gpu_runner(ctx: "native.Context", form: int = native.FORM_MFMA, reduce_gb: i
     def run(system: native.PolySystem) -> native.Basis:
         return ctx.f4(system, form=form, reduce_gb=reduce_gb)
     return run
+
+
+def gpu_batch_runner(ctx: "native.Context", form: int = native.FORM_MFMA, reduce_gb: int = 1):
+    """Blocks of F4 images on the GPU in lockstep (gbgpu_f4_multimod): every
+    round's matrices of the block's images in one batched launch.  The
+    runner's ``stats`` list gets each block's launch counts."""
+    def run_many(systems):
+        bases, st = ctx.f4_multimod(systems, form=form, reduce_gb=reduce_gb)
+        run_many.stats.append(dict(st, images=len(systems)))
+        return bases
+    run_many.stats = []
+    return run_many

@@ -87,6 +87,24 @@ typedef struct {
 } gbgpu_f4_gpu_t;
 int gbgpu_f4_gpu_la(void *user, const gbgpu_csr_t *m, int interreduce, gbgpu_rows_t *out);
 
+/* F4 of one system modulo several primes (the images of a multimodular run),
+ * in lockstep: ins[i] is image i (its own field_char), outs[i] its result.
+ * Each image runs the F4 host in a host thread of its own; in every round
+ * the matrices of all images still running go through one batched launch
+ * (gbgpu_batch_la: la_option 1 on every matrix, each modulo its prime), in
+ * `form`; the final interreduction (reduce_gb) runs image by image.  The
+ * host's OpenMP threads are shared out over the images.  No counterpart in
+ * the reference, whose rational path is fraction-free over GMP
+ * (la_qq.c:271-362).  Returns GBGPU_OK or the first error; outs are freed
+ * with gbgpu_f4_out_free. */
+typedef struct {
+    int64_t batches;     /* batched launches */
+    int64_t matrices;    /* matrices they reduced */
+    double device_ms;    /* their graphs' device time plus the interreductions' */
+} gbgpu_f4_multi_stats_t;
+int gbgpu_f4_multimod(gbgpu_ctx *ctx, const gbgpu_f4_in_t *ins, int32_t nimg, int32_t form,
+                      gbgpu_f4_out_t *outs, gbgpu_f4_multi_stats_t *stats);
+
 /* f4_julia's signature (f4.c:190-212) on the GPU back end: device 0 (or
  * GBGPU_DEVICE), form from la_option (101 plain, otherwise MFMA, which
  * also maps la_option 1/2/42/43/44 to the exact routine).  ht_size,

@@ -250,3 +250,15 @@ def test_full_size_sparse_pass_sampled(gpu_lib, p, form):
         assert np.array_equal(got[by_src[full]], exp[k])
     zero_sampled = set(int(x) for x in sample) - set(int(keep[s]) for s in esrc)
     assert not (zero_sampled & set(by_src)), "GPU row nonzero but oracle row zero"
+
+
+@pytest.mark.parametrize("p", [65521, 2147483647])
+@pytest.mark.parametrize("form", FORMS)
+def test_wide_matrix_more_than_63_windows(gpu_lib, p, form):
+    """600,000 columns: more than 63 windows of the widest (8192-column)
+    window the sparse pass uses, so the pivot layout is bucketed into 74
+    windows (gb itself has no column limit)."""
+    m = _f4(400, 600000, 250, p, seed=5, red_nnz=300)
+    exp = oracle.la_sparse_dense(m, nthreads=4)
+    got = gpu_lib.la_ff(m, la_option=1, form=form)
+    assert got.equals(exp)

@@ -147,3 +147,28 @@ def test_denominators_and_errors():
         qq.groebner_qq(_cpu_runner(1), 2, lens, exps, [1, -1, 1, 2], [1, 0, 1, 5])
     with pytest.raises(ValueError):
         qq.groebner_qq(_cpu_runner(1), 2, lens, exps, [1, -1, 1], [1, 3, 1])
+
+
+def test_blocks_of_images_give_the_same_lift():
+    """Images computed a block of primes at a time (the batched runner's
+    shape: blocks of 3, 4, 8, ...) are consumed in prime order exactly as
+    one at a time: the same lift from the same primes, and the first block
+    already settles it (two images to lift, one to check)."""
+    case = GOLD["qq_example"]
+    one = _solve(case, _cpu_runner(case["reduce_gb"]))
+    blocks = []
+
+    def run_many(systems):
+        blocks.append(len(systems))
+        return [oracle.f4(s, nthreads=4, reduce_gb=case["reduce_gb"]) for s in systems]
+
+    n = case["nr_gens"]
+    many = qq.groebner_qq(None, case["nr_vars"], case["lens"][:n], case["exps"], case["num"],
+                          case["den"], order=case["mon_order"], run_many=run_many)
+    assert many.cfs == one.cfs and many.lens == one.lens and many.primes == one.primes
+    assert blocks == [3]
+    # another sequence of primes lifts to the same basis
+    many = qq.groebner_qq(None, case["nr_vars"], case["lens"][:n], case["exps"], case["num"],
+                          case["den"], order=case["mon_order"], run_many=run_many,
+                          primes=list(islice(qq.primes_below(2 ** 30), 10)))
+    assert many.cfs == one.cfs
