@@ -45,10 +45,11 @@ NR, NC, NPIV = 20000, 30000, 12000
 SEED = 2024
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_sparse_blocked launch on this workload, from rocprofv3
-# FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00), each scaled by what the same
-# counter reads for a plain 256 MiB device copy; the commands and raw counts
-# are in profiles/k_sparse_blocked_f4_20000x30000_p65521.md
-PMC_TRAFFIC_BYTES = 13.40e9 + 0.770e9
+# FETCH_SIZE (8,068,585 KiB, x2.00) and WRITE_SIZE (940,417 KiB, x1.00), each
+# scaled by what the same counter reads for a plain 256 MiB device copy
+# (131,088 and 262,144 KiB); the commands and raw counts are in
+# profiles/k_sparse_blocked_f4_20000x30000_p65521.md
+PMC_TRAFFIC_BYTES = 8068585 * 1024 * 2.0 + 940417 * 1024 * 1.0
 
 
 # the CPU share of one GPU on the benchmark host: worker pools never exceed it
@@ -223,6 +224,13 @@ def main(argv=None) -> int:
     first[idx] = True
     red_nnz = int(lens[~first].sum())
     bytes_per_launch = 8.0 * red_nnz + 8.0 * nent + 4.0 * m.ncr * nrl
+    # compulsory bytes of the reduction kernel: every byte it must touch
+    # once -- the pivot layout (8 bytes per pivot entry, leads excluded),
+    # the groups' inverses (64 x 64 words per 64 pivot columns), the rows to
+    # reduce, and the D' rows it writes
+    piv_entries = int(lens[first].sum()) - int(first.sum())
+    compulsory = (8.0 * piv_entries + 4.0 * 4096 * ((m.ncl + 63) // 64) + 8.0 * red_nnz
+                  + 4.0 * m.ncr * nrl)
 
     def timed(form, steps, warmup):
         for _ in range(warmup):
@@ -263,7 +271,14 @@ def main(argv=None) -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": PMC_TRAFFIC_BYTES,
                      "kernel": "k_sparse_blocked", "kernel_ms": kavg,
                      "bytes_per_launch": bytes_per_launch,
-                     "pivots_applied": napp, "pivot_entries": nent},
+                     "pivots_applied": napp, "pivot_entries": nent,
+                     # algorithmic bytes count every pivot entry each time a
+                     # row streams it (caches serve the repeats); HBM traffic
+                     # and compulsory bytes beside them
+                     "compulsory_bytes": compulsory,
+                     "traffic_over_compulsory": PMC_TRAFFIC_BYTES / compulsory,
+                     "hbm_gbs": PMC_TRAFFIC_BYTES / (kavg / 1000.0) / 1e9,
+                     "hbm_frac": PMC_TRAFFIC_BYTES / (kavg / 1000.0) / 1e9 / HBM_PEAK_GBS},
     }
 
     out["plain_form"] = {"rows_per_s": nrl / t_plain, "ms_per_step": 1000.0 * t_plain,
