@@ -23,6 +23,7 @@
 // itself the first time a batch of that shape runs in a form; the choice is
 // kept per process.
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -57,6 +58,8 @@ struct gbgpu_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int32_t replays = 0;
+    int32_t nodes = 0;              // kernel launches the graph holds
+    bool eager = false;             // launched on the stream, no graph
     ~gbgpu_batch();
 };
 
@@ -330,6 +333,14 @@ int prepare(gbgpu_batch *b, int form) {
     if ((rc = layout_sparse(b, form, ws))) return rc;
     if ((rc = set_grid(b, wpc))) return rc;
     if ((rc = layout_dense(b, form))) return rc;
+    // GBGPU_BATCH_EAGER=1: the same launches straight on the stream, each
+    // run (for tracing tools that follow stream launches one by one)
+    const char *eg = getenv("GBGPU_BATCH_EAGER");
+    b->eager = eg && eg[0] == '1';
+    if (b->eager) {
+        b->form = form;
+        return GBGPU_OK;
+    }
     // capture the run
     hipStream_t s = ctx->stream;
     GBGPU_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -342,6 +353,9 @@ int prepare(gbgpu_batch *b, int form) {
     }
     if (e != hipSuccess) return check_hip(ctx, e, "hipStreamEndCapture");
     b->graph = g;
+    size_t nn = 0;
+    GBGPU_HIP(ctx, hipGraphGetNodes(g, nullptr, &nn));
+    b->nodes = (int32_t)nn;
     GBGPU_HIP(ctx, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
     b->form = form;
     return GBGPU_OK;
@@ -390,7 +404,11 @@ int gbgpu_batch_la(gbgpu_ctx *ctx, gbgpu_batch *b, int form, int32_t *np) {
     }
     hipStream_t s = ctx->stream;
     GBGPU_HIP(ctx, hipEventRecord(ctx->ev[4], s));
-    GBGPU_HIP(ctx, hipGraphLaunch(b->exec, s));
+    if (b->eager) {
+        if ((rc = enqueue_run(b, form))) return rc;
+    } else {
+        GBGPU_HIP(ctx, hipGraphLaunch(b->exec, s));
+    }
     GBGPU_HIP(ctx, hipEventRecord(ctx->ev[5], s));
     GBGPU_HIP(ctx, hipStreamSynchronize(s));
     float ms = 0;
@@ -427,8 +445,9 @@ int gbgpu_batch_rows_download(gbgpu_ctx *ctx, gbgpu_batch *b, int32_t i, gbgpu_r
 
 int gbgpu_batch_info(gbgpu_batch *b, int32_t *out, int32_t nout) {
     if (!b || !out) return GBGPU_EINVAL;
-    const int32_t v[5] = {(int32_t)b->mats.size(), b->wshift, b->waves_per_cu, b->grid, b->replays};
-    for (int32_t i = 0; i < nout && i < 5; ++i) out[i] = v[i];
+    const int32_t v[6] = {(int32_t)b->mats.size(), b->wshift, b->waves_per_cu, b->grid, b->replays,
+                          b->nodes};
+    for (int32_t i = 0; i < nout && i < 6; ++i) out[i] = v[i];
     return GBGPU_OK;
 }
 

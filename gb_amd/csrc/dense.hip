@@ -25,6 +25,9 @@
 // bound k (p-1)^2 < 2^53 (p < 2^17) or, for larger p, 16-bit operand
 // splitting (four exact products recombined in integers).
 #include <climits>
+#include <mutex>
+#include <string>
+#include <unordered_map>
 
 #include "internal.h"
 
@@ -1321,8 +1324,42 @@ size_t dense_panel_bytes() { return sizeof(Panel); }
 size_t dense_liststate_bytes() { return sizeof(ListState); }
 
 int batch_dense_init(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat) {
+    int rc;
+    if (nmat > 0 && (rc = check_launch(ctx, (const void *)k_batch_dense_init, nmat, 256, "batch init")))
+        return rc;
     if (nmat > 0) k_batch_dense_init<<<nmat, 256, 0, ctx->stream>>>(D);
     GBGPU_HIP(ctx, hipGetLastError());
+    return GBGPU_OK;
+}
+
+// a launch's shape against the device's limits, checked on the host before
+// it is issued (or captured): a nonzero grid, a block the kernel was
+// compiled for, grid x block within the 32-bit dispatch size, the kernel's
+// static LDS within the per-workgroup limit
+int check_launch(gbgpu_ctx *ctx, const void *fn, int64_t grid, int block, const char *what) {
+    static std::mutex mu;
+    static std::unordered_map<const void *, hipFuncAttributes> seen;
+    static int lds_max = 0;
+    hipFuncAttributes at;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = seen.find(fn);
+        if (it == seen.end()) {
+            GBGPU_HIP(ctx, hipFuncGetAttributes(&at, fn));
+            seen[fn] = at;
+            if (!lds_max)
+                GBGPU_HIP(ctx, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                                     ctx->device));
+        } else {
+            at = it->second;
+        }
+    }
+    if (grid < 1 || block < 1 || block > at.maxThreadsPerBlock || grid * (int64_t)block >= (1ll << 32) ||
+        (int64_t)at.sharedSizeBytes > (int64_t)lds_max)
+        return set_error(ctx, GBGPU_EDEVICE,
+                         std::string("launch outside the device limits: ") + what + " grid " +
+                             std::to_string(grid) + " block " + std::to_string(block) + " lds " +
+                             std::to_string(at.sharedSizeBytes));
     return GBGPU_OK;
 }
 
@@ -1334,6 +1371,21 @@ int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int
     const int32_t *hpre = htab + (int64_t)step * 4 * (nmat + 1);
     const int32_t nrp = hpre[nmat], nga = hpre[(nmat + 1) + nmat];
     const int32_t nup = hpre[2 * (nmat + 1) + nmat], npr = hpre[3 * (nmat + 1) + nmat];
+    int rc;
+    const void *upd = form == GBGPU_FORM_MFMA
+                          ? (nd == 1 ? (const void *)k_batch_update_mfma<1>
+                             : nd == 2 ? (const void *)k_batch_update_mfma<2>
+                             : nd == 3 ? (const void *)k_batch_update_mfma<3>
+                                       : (const void *)k_batch_update_mfma<4>)
+                          : (big ? (const void *)k_batch_update_plain<true>
+                                 : (const void *)k_batch_update_plain<false>);
+    if ((rc = check_launch(ctx, (const void *)k_batch_select, nmat, SEL_T, "batch select")) ||
+        (nrp > 0 && (rc = check_launch(ctx, (const void *)k_batch_rp, nrp, 256, "batch rp"))) ||
+        (nga > 0 && (rc = check_launch(ctx, (const void *)k_batch_gather, nga, 256, "batch gather"))) ||
+        (nup > 0 && (rc = check_launch(ctx, upd, nup, 256, "batch update"))) ||
+        (npr > 0 && (rc = check_launch(ctx, (const void *)k_batch_prune_flags, npr, 256, "batch prune"))) ||
+        (npr > 0 && (rc = check_launch(ctx, (const void *)k_batch_prune_compact, nmat, 256, "batch prune"))))
+        return rc;
     k_batch_select<<<nmat, SEL_T, 0, s>>>(D, c0);
     if (nrp > 0) k_batch_rp<<<nrp, 256, 0, s>>>(D, pre, nmat, c0);
     if (nga > 0) k_batch_gather<<<nga, 256, 0, s>>>(D, pre + (nmat + 1), nmat, c0);

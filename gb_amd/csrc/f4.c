@@ -554,11 +554,13 @@ static int rows_push(rows_t *R, row_t x) {
     return 0;
 }
 
-/* round-local marks on monomials */
+/* marks on monomials: round-local, plus the reducer search's memory */
 typedef struct {
     int32_t *stamp;   /* round of the last mark */
     int32_t *col;     /* column index in this round; -1 unprocessed */
     uint8_t *piv;     /* has a row with this lead */
+    int32_t *red;     /* reducer found by the last search (-1: none) */
+    int32_t *red_n;   /* basis size at that search (-1: never searched) */
     int32_t cap;
 } marks_t;
 
@@ -569,11 +571,18 @@ static int marks_fit(marks_t *M, int32_t n) {
     int32_t *s = (int32_t *)realloc(M->stamp, (size_t)nc * sizeof(int32_t));
     int32_t *c = (int32_t *)realloc(M->col, (size_t)nc * sizeof(int32_t));
     uint8_t *p = (uint8_t *)realloc(M->piv, (size_t)nc);
+    int32_t *r = (int32_t *)realloc(M->red, (size_t)nc * sizeof(int32_t));
+    int32_t *rn = (int32_t *)realloc(M->red_n, (size_t)nc * sizeof(int32_t));
     if (s) M->stamp = s;
     if (c) M->col = c;
     if (p) M->piv = p;
-    if (!s || !c || !p) return -1;
-    for (int32_t i = M->cap; i < nc; ++i) M->stamp[i] = -1;
+    if (r) M->red = r;
+    if (rn) M->red_n = rn;
+    if (!s || !c || !p || !r || !rn) return -1;
+    for (int32_t i = M->cap; i < nc; ++i) {
+        M->stamp[i] = -1;
+        M->red_n[i] = -1;
+    }
     M->cap = nc;
     return 0;
 }
@@ -703,13 +712,44 @@ static int build_csr(f4_t *F, rows_t *R, int32_t nc, int32_t ncl, gbgpu_csr_t *m
     return 0;
 }
 
+/* The reducer of a monomial: the first element of the reducer list (in
+ * installation order) whose lead divides it -- gb's find_multiplied_reducer
+ * (symbol.c:151-200).  Monomials recur from round to round, so the answer
+ * is remembered with the basis size it was found at: elements only join
+ * the list at its end, so a remembered reducer stays the first divisor while
+ * it is still in the list (for DRL, until it turns redundant), and a
+ * monomial that had none need only be tried against the elements installed
+ * since. */
 static int find_reducer(f4_t *F, int32_t mon) {
     const mt_t *T = &F->T;
-    for (int32_t q = 0; q < F->B.nlm; ++q) {
-        const int32_t i = F->B.lm[q];
-        if (mt_divides(T, F->B.e[i].m[0], mon)) return i;
+    const basis_t *B = &F->B;
+    int32_t q0 = 0;
+    const int32_t upto = F->M.red_n[mon];
+    if (upto >= 0) {
+        const int32_t e = F->M.red[mon];
+        if (e >= 0 && (T->order != 0 || !B->red[e])) return e;
+        if (e < 0) {
+            /* the list is in installation order: its first element >= upto */
+            int32_t lo = 0, hi = B->nlm;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (B->lm[mid] < upto) lo = mid + 1;
+                else hi = mid;
+            }
+            q0 = lo;
+        }
     }
-    return -1;
+    int32_t found = -1;
+    for (int32_t q = q0; q < B->nlm; ++q) {
+        const int32_t i = B->lm[q];
+        if (mt_divides(T, B->e[i].m[0], mon)) {
+            found = i;
+            break;
+        }
+    }
+    F->M.red[mon] = found;
+    F->M.red_n[mon] = B->n;
+    return found;
 }
 
 /* A round's rows are added in batches: stubs (element, multiplier) first,
@@ -1234,12 +1274,14 @@ int gbgpu_f4(const gbgpu_f4_in_t *in, gbgpu_f4_la_t la, void *user, gbgpu_f4_out
         if ((rc = f4_round(&F)) != GBGPU_OK) goto done;
         out->rounds++;
     }
-    /* drop the redundant elements (f4.c:330-338) */
+    /* drop the redundant elements (f4.c:330-338); the reducer search
+     * forgets what it found, since the list now changes in the middle */
     {
         int32_t w = 0;
         for (int32_t q = 0; q < F.B.nlm; ++q)
             if (!F.B.red[F.B.lm[q]]) F.B.lm[w++] = F.B.lm[q];
         F.B.nlm = w;
+        for (int32_t i = 0; i < F.M.cap; ++i) F.M.red_n[i] = -1;
     }
     if (in->reduce_gb == 1 && F.B.nlm > 0) {
         if ((rc = f4_reduce(&F)) != GBGPU_OK) goto done;
@@ -1294,7 +1336,7 @@ done:
     }
     free(F.B.e); free(F.B.red); free(F.B.lm);
     free(F.P.p);
-    free(F.M.stamp); free(F.M.col); free(F.M.piv);
+    free(F.M.stamp); free(F.M.col); free(F.M.piv); free(F.M.red); free(F.M.red_n);
     free(F.R.r); free(F.R.mon);
     free(F.W.rp); free(F.W.ci); free(F.W.rcf); free(F.W.eid); free(F.W.cp); free(F.W.cf);
     free(F.W.key);

@@ -298,6 +298,8 @@ void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int for
 int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int32_t *tab,
                      const int32_t *htab, int32_t step, int form, bool big, int nd);
 size_t dense_panel_bytes();      // sizeof(Panel)
+// a launch's grid, block and static LDS against the device's limits
+int check_launch(gbgpu_ctx *ctx, const void *fn, int64_t grid, int block, const char *what);
 size_t dense_liststate_bytes();  // sizeof(ListState)
 
 // rows.hip: rows of a dense RREF to CSR (columns shifted by col_shift);

@@ -935,8 +935,13 @@ int batch_sparse_launch(gbgpu_ctx *ctx, const BlockedArgs *mats, const int32_t *
                         int32_t nmat, int32_t *queue, uint32_t *xs, int64_t xs_stride, int wshift,
                         bool big, int form, int grid) {
     hipStream_t s = ctx->stream;
+    int rc;
 #define GBGPU_BATCH_SP(WS)                                                                       \
     do {                                                                                         \
+        const void *fn = form == GBGPU_FORM_MFMA                                                 \
+            ? (big ? (const void *)k_sparse_blocked_batch<WS, true> : (const void *)k_sparse_blocked_batch<WS, false>) \
+            : (big ? (const void *)k_sparse_plain_batch<WS, true> : (const void *)k_sparse_plain_batch<WS, false>);   \
+        if ((rc = check_launch(ctx, fn, grid, 64, "batch sparse pass"))) return rc;            \
         if (form == GBGPU_FORM_MFMA) {                                                           \
             if (big) k_sparse_blocked_batch<WS, true><<<grid, 64, 0, s>>>(mats, row_off, nmat, queue, xs, xs_stride); \
             else k_sparse_blocked_batch<WS, false><<<grid, 64, 0, s>>>(mats, row_off, nmat, queue, xs, xs_stride); \

@@ -215,7 +215,10 @@ int gbgpu_batch_la(gbgpu_ctx *ctx, gbgpu_batch *b, int form, int32_t *np);
 /* Rows of matrix i's result, as gbgpu_rows_download. */
 int gbgpu_batch_rows_download(gbgpu_ctx *ctx, gbgpu_batch *b, int32_t i,
                               gbgpu_rows_t *out);
-/* out[0..nout): matrices, window shift, waves per CU, sparse grid, replays. */
+/* out[0..nout): matrices, window shift, waves per CU, sparse grid, replays,
+ * kernel launches in the graph.  GBGPU_BATCH_EAGER=1 in the environment
+ * issues the same launches straight on the stream instead of through the
+ * graph (for tools that trace stream launches). */
 int gbgpu_batch_info(gbgpu_batch *b, int32_t *out, int32_t nout);
 void gbgpu_batch_free(gbgpu_batch *b);
 

@@ -109,3 +109,28 @@ def test_batch_64_uneven_31bit(gpu_lib):
         assert npm[i] == exp.np
         assert got_m[i].equals(exp)
         assert b.rows(i).equals(exp)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_batch_eager_launches_equal_the_graph(gpu_lib, form, monkeypatch):
+    """GBGPU_BATCH_EAGER=1 issues the batch's launches straight on the
+    stream (for tracing tools): the same rows as the graph replay.  Every
+    launch is checked against the device limits on the host first; the graph
+    holds one node per launch."""
+    primes = [2147483647, 65521, 251, 32003]
+    mats = _small_batch(primes, 21)
+    exp = [oracle.la_sparse_dense(m) for m in mats]
+    b = gpu_lib.batch(mats)
+    b.la(form)
+    info = b.info()
+    assert info["graph_nodes"] > 0
+    b.free()
+    monkeypatch.setenv("GBGPU_BATCH_EAGER", "1")
+    e = gpu_lib.batch(mats)
+    for _ in range(2):
+        npv = e.la(form)
+        for i in range(len(mats)):
+            assert npv[i] == exp[i].np
+            assert e.rows(i).equals(exp[i])
+    assert e.info()["graph_nodes"] == 0
+    e.free()
