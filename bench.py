@@ -191,6 +191,7 @@ def main(argv=None) -> int:
                          "(14 = the 14-variable configuration; plain and MFMA forms)")
     ap.add_argument("--only-f4", action="store_true", help="only the end-to-end F4 runs")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-matrix batch extra")
+    ap.add_argument("--no-f4", action="store_true", help="skip the end-to-end F4 extra")
     ap.add_argument("--f4-forms", default="mfma,plain", help="forms of the F4 runs")
     ap.add_argument("--no-f4-info", action="store_true",
                     help="no per-round lines of the F4 runs (on stderr)")
@@ -302,8 +303,9 @@ def main(argv=None) -> int:
 
     if not args.no_extra:
         extra = {}
-        extra["f4"] = f4_runs(args.f4_vars, ctx, not args.no_cpu, tuple(args.f4_forms.split(",")),
-                              args.f4_cpu_vars, 0 if args.no_f4_info else 2)
+        if not args.no_f4:
+            extra["f4"] = f4_runs(args.f4_vars, ctx, not args.no_cpu, tuple(args.f4_forms.split(",")),
+                                  args.f4_cpu_vars, 0 if args.no_f4_info else 2)
         for form, name in ((native.FORM_PLAIN, "plain"), (native.FORM_MFMA, "mfma")):
             d2 = ctx.upload(m)
             d2.la_sparse_dense(form)  # warm-up
