@@ -34,6 +34,8 @@
 #include <string.h>
 #include <time.h>
 
+#include <omp.h>
+
 #include "../../include/gbgpu_f4.h"
 
 static double now_ms(void) {
@@ -58,7 +60,7 @@ typedef struct {
     uint64_t *hv;
     uint64_t *dm;
     int32_t n, cap;
-    int32_t *slot;   /* id + 1, 0 = empty */
+    uint64_t *slot;  /* high 32 bits of the hash, then id + 1 (0 = empty) */
     uint64_t smask;
     uint64_t *rnd;   /* nv hash weights */
     ex_t *tmp;       /* nvp scratch exponents */
@@ -75,7 +77,7 @@ static int mt_init(mt_t *t, int nv, int order) {
     t->hv = (uint64_t *)malloc((size_t)t->cap * sizeof(uint64_t));
     t->dm = (uint64_t *)malloc((size_t)t->cap * sizeof(uint64_t));
     t->smask = (1u << 13) - 1;
-    t->slot = (int32_t *)calloc((size_t)t->smask + 1, sizeof(int32_t));
+    t->slot = (uint64_t *)calloc((size_t)t->smask + 1, sizeof(uint64_t));
     t->tmp = (ex_t *)calloc((size_t)t->nvp, sizeof(ex_t));
     t->rnd = (uint64_t *)malloc((size_t)nv * sizeof(uint64_t));
     if (!t->rnd) return -1;
@@ -99,6 +101,34 @@ static void mt_free(mt_t *t) {
 
 static inline uint64_t mt_pos(uint64_t h) { return h ^ (h >> 29) ^ (h >> 43); }
 
+/* a slot holds the monomial's hash tag beside its id: a probe passes over
+ * slots of other monomials without reading their exponents */
+#define SLOT_TAG 0xFFFFFFFF00000000ull
+#define SLOT_CLAIMED (~0ull)
+static inline uint64_t slot_of(uint64_t h, int32_t id) { return (h & SLOT_TAG) | (uint32_t)(id + 1); }
+static inline int32_t slot_id(uint64_t s) { return (int32_t)(uint32_t)s - 1; }
+
+/* exponent vectors are nvp 16-bit lanes, nvp a multiple of 4: compared as
+ * 64-bit words */
+static inline int ev_eq(const ex_t *a, const ex_t *b, int nvp) {
+    const uint64_t *x = (const uint64_t *)a, *y = (const uint64_t *)b;
+    uint64_t d = 0;
+    for (int w = 0; w < nvp / 4; ++w) d |= x[w] ^ y[w];
+    return d == 0;
+}
+
+/* a fresh slot array of nm + 1 slots holding the first n monomials */
+static uint64_t *slots_build(const uint64_t *hv, int32_t n, uint64_t nm) {
+    uint64_t *sl = (uint64_t *)calloc((size_t)nm + 1, sizeof(uint64_t));
+    if (!sl) return NULL;
+    for (int32_t i = 0; i < n; ++i) {
+        uint64_t p = mt_pos(hv[i]) & nm;
+        while (sl[p]) p = (p + 1) & nm;
+        sl[p] = slot_of(hv[i], i);
+    }
+    return sl;
+}
+
 static int mt_grow(mt_t *t) {
     if (t->n + 1 >= t->cap) {
         int32_t nc = t->cap * 2;
@@ -115,13 +145,8 @@ static int mt_grow(mt_t *t) {
     }
     if ((uint64_t)t->n * 2 >= t->smask) {
         const uint64_t nm = t->smask * 2 + 1;
-        int32_t *sl = (int32_t *)calloc((size_t)nm + 1, sizeof(int32_t));
+        uint64_t *sl = slots_build(t->hv, t->n, nm);
         if (!sl) return -1;
-        for (int32_t i = 0; i < t->n; ++i) {
-            uint64_t p = mt_pos(t->hv[i]) & nm;
-            while (sl[p]) p = (p + 1) & nm;
-            sl[p] = i + 1;
-        }
         free(t->slot);
         t->slot = sl;
         t->smask = nm;
@@ -133,11 +158,10 @@ static int mt_grow(mt_t *t) {
 static int32_t mt_get(mt_t *t, const ex_t *e, uint64_t h) {
     uint64_t p = mt_pos(h) & t->smask;
     for (;;) {
-        const int32_t s = t->slot[p];
+        const uint64_t s = t->slot[p];
         if (!s) break;
-        const int32_t id = s - 1;
-        if (t->hv[id] == h && memcmp(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp * sizeof(ex_t)) == 0)
-            return id;
+        if (((s ^ h) & SLOT_TAG) == 0 && ev_eq(t->ev + (size_t)slot_id(s) * t->nvp, e, t->nvp))
+            return slot_id(s);
         p = (p + 1) & t->smask;
     }
     if (mt_grow(t)) return -1;
@@ -155,20 +179,19 @@ static int32_t mt_get(mt_t *t, const ex_t *e, uint64_t h) {
     t->deg[id] = d;
     t->hv[id] = h;
     t->dm[id] = m;
-    t->slot[p] = id + 1;
+    t->slot[p] = slot_of(h, id);
     return id;
 }
 
 /* id of an existing monomial, -1 when absent (no insertion: safe to call
  * from many threads while nothing is inserted) */
-static int32_t mt_find(const mt_t *t, const ex_t *e, uint64_t h) {
+static inline int32_t mt_find(const mt_t *t, const ex_t *e, uint64_t h) {
     uint64_t p = mt_pos(h) & t->smask;
     for (;;) {
-        const int32_t s = t->slot[p];
+        const uint64_t s = t->slot[p];
         if (!s) return -1;
-        const int32_t id = s - 1;
-        if (t->hv[id] == h && memcmp(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp * sizeof(ex_t)) == 0)
-            return id;
+        if (((s ^ h) & SLOT_TAG) == 0 && ev_eq(t->ev + (size_t)slot_id(s) * t->nvp, e, t->nvp))
+            return slot_id(s);
         p = (p + 1) & t->smask;
     }
 }
@@ -194,13 +217,8 @@ static int mt_reserve(mt_t *t, int64_t extra) {
     if ((uint64_t)need * 2 > t->smask) {
         uint64_t nm = t->smask;
         while ((uint64_t)need * 2 > nm) nm = nm * 2 + 1;
-        int32_t *sl = (int32_t *)calloc((size_t)nm + 1, sizeof(int32_t));
+        uint64_t *sl = slots_build(t->hv, t->n, nm);
         if (!sl) return -1;
-        for (int32_t i = 0; i < t->n; ++i) {
-            uint64_t p = mt_pos(t->hv[i]) & nm;
-            while (sl[p]) p = (p + 1) & nm;
-            sl[p] = i + 1;
-        }
         free(t->slot);
         t->slot = sl;
         t->smask = nm;
@@ -209,17 +227,17 @@ static int mt_reserve(mt_t *t, int64_t extra) {
 }
 
 /* Insertion from many threads at once, after mt_reserve: a slot is
- * claimed by compare-and-swap (0 -> -1), filled, then published as id + 1;
+ * claimed by compare-and-swap (0 -> SLOT_CLAIMED), filled, then published;
  * a thread meeting a claimed slot waits for it.  Ids come from an atomic
  * counter; one at or above lim is not stored (the slot is released and
  * *over set) -- the caller falls back to the serial path. */
 static int32_t mt_get_par(mt_t *t, const ex_t *e, uint64_t h, int32_t lim, int *over) {
     uint64_t p = mt_pos(h) & t->smask;
     for (;;) {
-        int32_t s = __atomic_load_n(&t->slot[p], __ATOMIC_ACQUIRE);
+        uint64_t s = __atomic_load_n(&t->slot[p], __ATOMIC_ACQUIRE);
         if (s == 0) {
-            int32_t expect = 0;
-            if (__atomic_compare_exchange_n(&t->slot[p], &expect, -1, 0, __ATOMIC_ACQ_REL,
+            uint64_t expect = 0;
+            if (__atomic_compare_exchange_n(&t->slot[p], &expect, SLOT_CLAIMED, 0, __ATOMIC_ACQ_REL,
                                             __ATOMIC_ACQUIRE)) {
                 const int32_t id = __atomic_fetch_add(&t->n, 1, __ATOMIC_RELAXED);
                 if (id >= lim) {
@@ -237,16 +255,15 @@ static int32_t mt_get_par(mt_t *t, const ex_t *e, uint64_t h, int32_t lim, int *
                 t->deg[id] = d;
                 t->hv[id] = h;
                 t->dm[id] = m;
-                __atomic_store_n(&t->slot[p], id + 1, __ATOMIC_RELEASE);
+                __atomic_store_n(&t->slot[p], slot_of(h, id), __ATOMIC_RELEASE);
                 return id;
             }
             s = expect;
         }
-        while (s == -1) s = __atomic_load_n(&t->slot[p], __ATOMIC_ACQUIRE);
+        while (s == SLOT_CLAIMED) s = __atomic_load_n(&t->slot[p], __ATOMIC_ACQUIRE);
         if (s == 0) continue;  /* released by an overflowing claim: look again */
-        const int32_t id = s - 1;
-        if (t->hv[id] == h && memcmp(t->ev + (size_t)id * t->nvp, e, (size_t)t->nvp * sizeof(ex_t)) == 0)
-            return id;
+        if (((s ^ h) & SLOT_TAG) == 0 && ev_eq(t->ev + (size_t)slot_id(s) * t->nvp, e, t->nvp))
+            return slot_id(s);
         p = (p + 1) & t->smask;
     }
 }
@@ -424,22 +441,23 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
     if (!le || !ld || !lm || !st) { rc = -1; goto out; }
     {
         const ex_t *ek = EV(T, lk);
+        const uint64_t mk = T->dm[lk];
+        const uint32_t dk = T->deg[lk];
+#pragma omp parallel for schedule(static) if (nl > 2048)
         for (int32_t q = 0; q < nl; ++q) {
             const int32_t i = B->lm[q];
             const int32_t li = B->e[i].m[0];
             const ex_t *ei = EV(T, li);
             ex_t *e = le + (size_t)q * nvp;
             uint32_t d = 0;
-            uint64_t m = 0;
             for (int v = 0; v < nvp; ++v) {
                 e[v] = ei[v] > ek[v] ? ei[v] : ek[v];
                 d += e[v];
-                if (e[v]) m |= 1ull << (v & 63);
             }
             ld[q] = d;
-            lm[q] = m;
+            lm[q] = T->dm[li] | mk;  /* the variables of the lcm */
             /* pairs with redundant elements (kept as reducers in lex) are not formed */
-            st[q] = B->red[i] ? 0 : (d == T->deg[li] + T->deg[lk]) ? 2 : 1;
+            st[q] = B->red[i] ? 0 : (d == T->deg[li] + dk) ? 2 : 1;
         }
     }
     /* old pairs: chain criterion (marks in parallel, then compaction) */
@@ -470,20 +488,40 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
      * (hash.c:553-557).  Coprime pairs still drop the pairs above them.
      * (Dropped pairs need not be left out as divisors: a dropped divisor
      * has an earlier divisor of its own.)  Each candidate is checked on its
-     * own, in parallel. */
+     * own, in parallel, against the candidates before it in (degree, index)
+     * order -- the only ones that can drop it -- lowest degree first. */
     {
         uint8_t *keep = (uint8_t *)malloc(nn);
-        if (!keep) { rc = -1; goto out; }
+        int32_t *byd = (int32_t *)malloc(nn * sizeof(int32_t));
+        int32_t *pos = (int32_t *)malloc(nn * sizeof(int32_t));
+        if (!keep || !byd || !pos) { free(keep); free(byd); free(pos); rc = -1; goto out; }
+        uint32_t dlo = UINT32_MAX, dhi = 0;
+        for (int32_t q = 0; q < nl; ++q) {
+            if (ld[q] < dlo) dlo = ld[q];
+            if (ld[q] > dhi) dhi = ld[q];
+        }
+        /* counting sort by degree, stable in the index */
+        const uint32_t nd = nl ? dhi - dlo + 2 : 1;
+        int32_t *cnt = (int32_t *)calloc(nd, sizeof(int32_t));
+        if (!cnt) { free(keep); free(byd); free(pos); rc = -1; goto out; }
+        for (int32_t q = 0; q < nl; ++q) cnt[ld[q] - dlo + 1]++;
+        for (uint32_t d = 1; d < nd; ++d) cnt[d] += cnt[d - 1];
+        for (int32_t q = 0; q < nl; ++q) {
+            pos[q] = cnt[ld[q] - dlo]++;
+            byd[pos[q]] = q;
+        }
+        free(cnt);
 #pragma omp parallel for schedule(dynamic, 16) if (nl > 256)
         for (int32_t q = 0; q < nl; ++q) {
             keep[q] = st[q];
             if (st[q] != 1) continue;
             const ex_t *eq = le + (size_t)q * nvp;
-            for (int32_t r = 0; r < nl; ++r) {
-                if (r == q || st[r] == 0 || ld[r] > ld[q] || (lm[r] & ~lm[q])) continue;
+            const uint64_t mq = lm[q];
+            for (int32_t a = 0; a < pos[q]; ++a) {
+                const int32_t r = byd[a];
+                if (st[r] == 0 || (lm[r] & ~mq)) continue;
                 const ex_t *er = le + (size_t)r * nvp;
-                if (ld[r] == ld[q] ? (r < q && memcmp(er, eq, (size_t)nvp * sizeof(ex_t)) == 0)
-                                   : ediv(er, eq, nv)) {
+                if (ld[r] == ld[q] ? ev_eq(er, eq, nvp) : ediv(er, eq, nv)) {
                     keep[q] = 0;
                     break;
                 }
@@ -491,6 +529,8 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
         }
         memcpy(st, keep, nn);
         free(keep);
+        free(byd);
+        free(pos);
     }
     for (int32_t q = 0; q < nl; ++q) {
         if (st[q] != 1) continue;
@@ -626,6 +666,17 @@ static int buf_fit(void **p, int64_t *cap, int64_t n, size_t elt) {
     return *p ? 0 : -1;
 }
 
+/* Products u t of one multiplier u with the monomials t of several
+ * elements: the rows of a round share few multipliers and their elements
+ * share most monomials, so most products recur.  Each thread keeps, per
+ * monomial t, the last product it formed with t and the (pass, u) it
+ * belongs to; a pass number is never reused, so nothing is ever cleared. */
+typedef struct {
+    uint64_t *key;  /* pass << 32 | (u + 1); 0 never matches */
+    int32_t *val;   /* u t */
+    int32_t cap;
+} pcache_t;
+
 typedef struct {
     mt_t T;
     basis_t B;
@@ -633,6 +684,9 @@ typedef struct {
     marks_t M;
     rows_t R;     /* the round's rows, reused */
     csrbuf_t W;   /* the round's matrix, reused */
+    pcache_t *pc; /* product caches, one per thread */
+    int32_t npc;
+    uint32_t pass;
     uint32_t p;
     int32_t round;
     gbgpu_f4_la_t la;
@@ -771,24 +825,77 @@ static int row_stub(rows_t *R, const basis_t *B, int32_t e, int32_t mul) {
     return rows_push(R, x);
 }
 
+/* a product cache per thread, each with room for every monomial */
+static int pcache_fit(f4_t *F, int32_t n) {
+    const int nth = omp_get_max_threads();
+    if (F->npc < nth) {
+        pcache_t *np = (pcache_t *)realloc(F->pc, (size_t)nth * sizeof(pcache_t));
+        if (!np) return -1;
+        memset(np + F->npc, 0, (size_t)(nth - F->npc) * sizeof(pcache_t));
+        F->pc = np;
+        F->npc = nth;
+    }
+    for (int i = 0; i < nth; ++i) {
+        pcache_t *c = &F->pc[i];
+        if (c->cap >= n) continue;
+        int32_t nc = c->cap ? c->cap : 4096;
+        while (nc < n) nc *= 2;
+        uint64_t *k = (uint64_t *)realloc(c->key, (size_t)nc * sizeof(uint64_t));
+        if (k) c->key = k;
+        int32_t *v = (int32_t *)realloc(c->val, (size_t)nc * sizeof(int32_t));
+        if (v) c->val = v;
+        if (!k || !v) return -1;
+        memset(c->key + c->cap, 0, (size_t)(nc - c->cap) * sizeof(uint64_t));
+        c->cap = nc;
+    }
+    return 0;
+}
+
+static int cmp_u64(const void *a, const void *b) {
+    const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : (x > y);
+}
+
+/* The monomials of rows r0 .. r1-1: every product u t looked up in the
+ * table while it is read only (rows taken by multiplier, so that the
+ * product caches hit), then the missing ones inserted by all threads. */
 static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
     mt_t *T = &F->T;
     const basis_t *B = &F->B;
     const int nw = T->nvp / 4;
+    const int64_t nr = r1 - r0;
+    if (nr <= 0) return 0;
+    /* the rows by multiplier */
+    uint64_t *ord = (uint64_t *)malloc((size_t)nr * sizeof(uint64_t));
+    if (!ord || pcache_fit(F, T->n)) {
+        free(ord);
+        return -1;
+    }
+    for (int64_t r = r0; r < r1; ++r)
+        ord[r - r0] = (uint64_t)(uint32_t)R->r[r].mul << 32 | (uint64_t)(r - r0);
+    qsort(ord, (size_t)nr, sizeof(uint64_t), cmp_u64);
     int64_t nmiss = 0;
     uint64_t high = 0;
+    const uint64_t pass = (uint64_t)(++F->pass) << 32;
 #pragma omp parallel reduction(+ : nmiss) reduction(| : high)
     {
         uint64_t tmp[nw];
+        pcache_t *pc = &F->pc[omp_get_thread_num()];
 #pragma omp for schedule(dynamic, 8)
-        for (int64_t r = r0; r < r1; ++r) {
-            const row_t *x = &R->r[r];
+        for (int64_t q = 0; q < nr; ++q) {
+            const row_t *x = &R->r[r0 + (int64_t)(uint32_t)ord[q]];
             const poly_t *f = &B->e[x->e];
             const uint64_t *em = (const uint64_t *)EV(T, x->mul);
             const uint64_t hm = T->hv[x->mul];
+            const uint64_t key = pass | (uint32_t)(x->mul + 1);
             int32_t *o = R->mon + x->off;
             for (int32_t t = 0; t < f->len; ++t) {
-                const uint64_t *et = (const uint64_t *)EV(T, f->m[t]);
+                const int32_t mt = f->m[t];
+                if (pc->key[mt] == key) {
+                    o[t] = pc->val[mt];
+                    continue;
+                }
+                const uint64_t *et = (const uint64_t *)EV(T, mt);
                 /* four 16-bit exponents per word, each <= EX_MAX: no carries
                  * between lanes; a lane above EX_MAX shows in its top bit */
                 uint64_t hi = 0;
@@ -801,53 +908,75 @@ static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
                     o[t] = 0;
                     continue;
                 }
-                o[t] = mt_find(T, (const ex_t *)tmp, hm + T->hv[f->m[t]]);
+                o[t] = mt_find(T, (const ex_t *)tmp, hm + T->hv[mt]);
+                pc->key[mt] = key;
+                pc->val[mt] = o[t];
                 nmiss += o[t] < 0;
             }
         }
     }
     if (high & EX_HIGH) {
+        free(ord);
         T->erange = 1;
         return -1;
     }
+    int rc = 0;
     if (nmiss) {
-        /* new monomials: inserted by all threads into reserved room (the
-         * misses count repeats, so the room is capped) */
+        /* new monomials: inserted by all threads into reserved room (each
+         * is missed at least once, so nmiss bounds their number; the
+         * room is capped) */
         int64_t room = nmiss;
         const int64_t cap_room = (int64_t)T->n * 2 > (1 << 22) ? (int64_t)T->n * 2 : (1 << 22);
         if (room > cap_room) room = cap_room;
-        int over = 0;
+        int done = 0;
         if (mt_reserve(T, room) == 0) {
             const int32_t lim = (int32_t)(T->n + room);
+            const uint64_t pass2 = (uint64_t)(++F->pass) << 32;
+            int over = 0;
 #pragma omp parallel
             {
                 uint64_t tmp[nw];
+                pcache_t *pc = &F->pc[omp_get_thread_num()];
 #pragma omp for schedule(dynamic, 8)
-                for (int64_t r = r0; r < r1; ++r) {
-                    const row_t *x = &R->r[r];
+                for (int64_t q = 0; q < nr; ++q) {
+                    const row_t *x = &R->r[r0 + (int64_t)(uint32_t)ord[q]];
                     const poly_t *f = &B->e[x->e];
                     const uint64_t *em = (const uint64_t *)EV(T, x->mul);
                     const uint64_t hm = T->hv[x->mul];
+                    const uint64_t key = pass2 | (uint32_t)(x->mul + 1);
                     int32_t *o = R->mon + x->off;
                     for (int32_t t = 0; t < f->len; ++t) {
                         if (o[t] >= 0) continue;
-                        const uint64_t *et = (const uint64_t *)EV(T, f->m[t]);
+                        const int32_t mt = f->m[t];
+                        if (pc->key[mt] == key && pc->val[mt] >= 0) {
+                            o[t] = pc->val[mt];
+                            continue;
+                        }
+                        const uint64_t *et = (const uint64_t *)EV(T, mt);
                         for (int w = 0; w < nw; ++w) tmp[w] = em[w] + et[w];
-                        o[t] = mt_get_par(T, (const ex_t *)tmp, hm + T->hv[f->m[t]], lim, &over);
+                        o[t] = mt_get_par(T, (const ex_t *)tmp, hm + T->hv[mt], lim, &over);
+                        pc->key[mt] = key;
+                        pc->val[mt] = o[t];
                     }
                 }
             }
             if (T->n > lim) T->n = lim;
+            done = !over;
         }
         /* whatever is left (no room, or the room ran out): one thread */
-        for (int64_t r = r0; r < r1; ++r) {
+        for (int64_t r = r0; r < r1 && !done && rc == 0; ++r) {
             const row_t *x = &R->r[r];
             const poly_t *f = &B->e[x->e];
             int32_t *o = R->mon + x->off;
             for (int32_t t = 0; t < f->len; ++t)
-                if (o[t] < 0 && (o[t] = mt_mul(T, x->mul, f->m[t])) < 0) return -1;
+                if (o[t] < 0 && (o[t] = mt_mul(T, x->mul, f->m[t])) < 0) {
+                    rc = -1;
+                    break;
+                }
         }
     }
+    free(ord);
+    if (rc) return rc;
     for (int64_t r = r0; r < r1; ++r) R->r[r].lead = R->mon[R->r[r].off];
     return 0;
 }
@@ -1340,6 +1469,11 @@ done:
     free(F.R.r); free(F.R.mon);
     free(F.W.rp); free(F.W.ci); free(F.W.rcf); free(F.W.eid); free(F.W.cp); free(F.W.cf);
     free(F.W.key);
+    for (int32_t i = 0; i < F.npc; ++i) {
+        free(F.pc[i].key);
+        free(F.pc[i].val);
+    }
+    free(F.pc);
     mt_free(&F.T);
     out->total_ms = now_ms() - t_start;
     return rc;
