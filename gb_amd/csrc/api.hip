@@ -103,7 +103,8 @@ void pool_free(void *ptr) {
 using namespace gbgpu;
 
 gbgpu_mat::~gbgpu_mat() {
-    dfree(rp); dfree(ci); dfree(rcf); dfree(cp); dfree(cf);
+    dfree(rp); dfree(ci); dfree(rcf); dfree(cp);
+    if (!cf_borrowed) dfree(cf);
     dfree(first); dfree(upiv);
     dfree(seg); dfree(bucket); dfree(pbits); dfree(gseg); dfree(gbucket);
     dfree(dprime); dfree(dnz); dfree(dlist);
@@ -197,12 +198,14 @@ double gbgpu_phase_ms(const gbgpu_ctx *ctx, int phase) {
     return ctx->phase_ms[phase];
 }
 
-static int check_csr_header(gbgpu_ctx *ctx, const gbgpu_csr_t *m) {
+static int check_csr_header(gbgpu_ctx *ctx, const gbgpu_csr_t *m, bool need_ci = true,
+                            bool need_cf = true) {
     if (!m) return set_error(ctx, GBGPU_EINVAL, "NULL matrix");
     if (m->nr < 0 || m->nc < 0 || m->ncl < 0 || m->ncl > m->nc || m->ncf < 0)
         return set_error(ctx, GBGPU_EINVAL, "bad matrix dimensions");
     if (m->p < 2 || m->p >= (1u << 31)) return set_error(ctx, GBGPU_EINVAL, "p must be in [2, 2^31)");
-    if ((m->nr > 0 && (!m->rp || !m->ci || !m->rcf)) || (m->ncf > 0 && (!m->cp || !m->cf)))
+    if ((m->nr > 0 && (!m->rp || (need_ci && !m->ci) || !m->rcf)) ||
+        (m->ncf > 0 && (!m->cp || (need_cf && !m->cf))))
         return set_error(ctx, GBGPU_EINVAL, "NULL array in matrix");
     return GBGPU_OK;
 }
@@ -235,11 +238,128 @@ __global__ void k_check_entries(int64_t nnz, const int32_t *ci, int32_t nc, int6
     if (bad) atomicOr(flag, bad);
 }
 
+// row q of a matrix given by monomial ids: mon[src[q] .. src[q] + len(q))
+// mapped through colmap to the column indices ci[rp[q] ..]; flag |= 1 for an
+// id outside the map.  One workgroup per row at a time, coalesced in t.
+__global__ void k_map_rows(int32_t nr, const int64_t *rp, const int64_t *src, const int32_t *mon,
+                           const int32_t *colmap, int32_t nmap, int32_t *ci, int32_t *flag) {
+    int bad = 0;
+    for (int32_t q = blockIdx.x; q < nr; q += gridDim.x) {
+        const int64_t b = rp[q], len = rp[q + 1] - b;
+        const int32_t *in = mon + src[q];
+        for (int64_t t = threadIdx.x; t < len; t += blockDim.x) {
+            const int32_t id = in[t];
+            if ((uint32_t)id >= (uint32_t)nmap) {
+                bad = 1;
+                ci[b + t] = -1;
+            } else {
+                ci[b + t] = colmap[id];
+            }
+        }
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
+// the monomial-id form of a matrix's column indices (gbgpu_mat_upload_rows)
+struct RowsById {
+    const int32_t *mon;
+    int64_t nmon;
+    const int64_t *src;
+    const int32_t *colmap;
+    int32_t nmap;
+    const gbgpu_cfstore *store;  // coefficient arrays already on the device, or null
+};
+
+// Coefficient arrays kept on the device (entries checked < p on append).
+struct gbgpu_cfstore {
+    gbgpu_ctx *ctx = nullptr;
+    uint32_t p = 0;
+    uint32_t *d = nullptr;
+    int64_t n = 0, cap = 0;
+    ~gbgpu_cfstore() { dfree(d); }
+};
+
+int gbgpu_cfstore_create(gbgpu_ctx *ctx, uint32_t p, gbgpu_cfstore **out) {
+    if (!ctx || !out) return GBGPU_EINVAL;
+    *out = nullptr;
+    if (p < 2 || p >= (1u << 31)) return set_error(ctx, GBGPU_EINVAL, "p must be in [2, 2^31)");
+    gbgpu_cfstore *c = new gbgpu_cfstore();
+    c->ctx = ctx;
+    c->p = p;
+    *out = c;
+    return GBGPU_OK;
+}
+
+int gbgpu_cfstore_append(gbgpu_cfstore *c, const uint32_t *cf, int64_t n, int64_t *off) {
+    if (!c || n < 0 || (n > 0 && !cf)) return GBGPU_EINVAL;
+    gbgpu_ctx *ctx = c->ctx;
+    for (int64_t i = 0; i < n; ++i)
+        if (cf[i] >= c->p) return set_error(ctx, GBGPU_EINVAL, "coefficient not reduced mod p");
+    GBGPU_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    if (c->n + n > c->cap) {
+        int64_t nc = c->cap ? c->cap : ((int64_t)1 << 20);
+        while (nc < c->n + n) nc += nc / 2;
+        uint32_t *nd = nullptr;
+        int rc = dalloc(ctx, &nd, (size_t)nc);
+        if (rc) return rc;
+        hipError_t e = c->n ? hipMemcpyAsync(nd, c->d, (size_t)c->n * 4, hipMemcpyDeviceToDevice, s)
+                            : hipSuccess;
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            dfree(nd);
+            return check_hip(ctx, e, "coefficient store");
+        }
+        dfree(c->d);
+        c->d = nd;
+        c->cap = nc;
+    }
+    if (n > 0) {
+        hipError_t e = hipMemcpyAsync(c->d + c->n, cf, (size_t)n * 4, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return check_hip(ctx, e, "coefficient store");
+    }
+    if (off) *off = c->n;
+    c->n += n;
+    return GBGPU_OK;
+}
+
+int64_t gbgpu_cfstore_size(const gbgpu_cfstore *c) { return c ? c->n : -1; }
+
+void gbgpu_cfstore_free(gbgpu_cfstore *c) { delete c; }
+
+static int mat_upload(gbgpu_ctx *ctx, const gbgpu_csr_t *h, const RowsById *byid, gbgpu_mat **out);
+
 int gbgpu_mat_upload(gbgpu_ctx *ctx, const gbgpu_csr_t *h, gbgpu_mat **out) {
+    return mat_upload(ctx, h, nullptr, out);
+}
+
+int gbgpu_mat_upload_rows(gbgpu_ctx *ctx, const gbgpu_csr_t *h, const int32_t *mon, int64_t nmon,
+                          const int64_t *src, const int32_t *colmap, int32_t nmap,
+                          const gbgpu_cfstore *store, gbgpu_mat **out) {
+    if (!ctx || !out || !h) return GBGPU_EINVAL;
+    if (nmon < 0 || nmap < 0 || (h->nr > 0 && (!mon || !src || !colmap)))
+        return set_error(ctx, GBGPU_EINVAL, "NULL or negative monomial-id arrays");
+    if (store && (store->ctx != ctx || store->p != h->p))
+        return set_error(ctx, GBGPU_EINVAL, "coefficient store of another context or prime");
+    const RowsById b = {mon, nmon, src, colmap, nmap, store};
+    return mat_upload(ctx, h, &b, out);
+}
+
+static int mat_upload(gbgpu_ctx *ctx, const gbgpu_csr_t *h, const RowsById *byid, gbgpu_mat **out) {
     if (!ctx || !out) return GBGPU_EINVAL;
     *out = nullptr;
     int rc;
-    if ((rc = check_csr_header(ctx, h)) || (rc = check_csr_body(ctx, h))) return rc;
+    const gbgpu_cfstore *store = byid ? byid->store : nullptr;
+    if ((rc = check_csr_header(ctx, h, byid == nullptr, store == nullptr)) ||
+        (rc = check_csr_body(ctx, h)))
+        return rc;
+    if (store && h->ncf > 0 && h->cp[h->ncf] > store->n)
+        return set_error(ctx, GBGPU_EINVAL, "coefficient array outside the store");
+    if (byid)
+        for (int32_t q = 0; q < h->nr; ++q)
+            if (byid->src[q] < 0 || byid->src[q] + (h->rp[q + 1] - h->rp[q]) > byid->nmon)
+                return set_error(ctx, GBGPU_EINVAL, "row outside the monomial-id array");
     if ((int64_t)h->nr * 8 > INT32_MAX || (int64_t)h->nc * 64 > INT32_MAX)
         return set_error(ctx, GBGPU_ERANGE, "matrix too large for this build");
     GBGPU_HIP(ctx, hipSetDevice(ctx->device));
@@ -251,35 +371,67 @@ int gbgpu_mat_upload(gbgpu_ctx *ctx, const gbgpu_csr_t *h, gbgpu_mat **out) {
     hipStream_t s = ctx->stream;
     if ((rc = dalloc(ctx, &m->rp, (size_t)m->nr + 1)) || (rc = dalloc(ctx, &m->ci, (size_t)m->nnz)) ||
         (rc = dalloc(ctx, &m->rcf, (size_t)m->nr)) || (rc = dalloc(ctx, &m->cp, (size_t)m->ncf + 1)) ||
-        (rc = dalloc(ctx, &m->cf, (size_t)m->npool))) {
+        (!store && (rc = dalloc(ctx, &m->cf, (size_t)m->npool)))) {
+        delete m;
+        return rc;
+    }
+    if (store) {
+        // the store's arrays, borrowed: checked when they were appended
+        m->cf = store->d;
+        m->cf_borrowed = true;
+    }
+    // monomial ids, row sources and the column map: device copies for the
+    // mapping kernel, freed after the synchronisation below
+    int32_t *dmon = nullptr, *dmap = nullptr;
+    int64_t *dsrc = nullptr;
+    if (byid && m->nr > 0 &&
+        ((rc = dalloc(ctx, &dmon, (size_t)byid->nmon)) || (rc = dalloc(ctx, &dsrc, (size_t)m->nr)) ||
+         (rc = dalloc(ctx, &dmap, (size_t)byid->nmap)))) {
+        dfree(dmon); dfree(dsrc); dfree(dmap);
         delete m;
         return rc;
     }
     hipError_t e = hipSuccess;
     if (m->nr > 0) {
         e = hipMemcpyAsync(m->rp, h->rp, ((size_t)m->nr + 1) * 8, hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(m->ci, h->ci, (size_t)m->nnz * 4, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && !byid)
+            e = hipMemcpyAsync(m->ci, h->ci, (size_t)m->nnz * 4, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && byid) {
+            e = hipMemcpyAsync(dmon, byid->mon, (size_t)byid->nmon * 4, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(dsrc, byid->src, (size_t)m->nr * 8, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess && byid->nmap > 0)
+                e = hipMemcpyAsync(dmap, byid->colmap, (size_t)byid->nmap * 4, hipMemcpyHostToDevice, s);
+        }
         if (e == hipSuccess) e = hipMemcpyAsync(m->rcf, h->rcf, (size_t)m->nr * 4, hipMemcpyHostToDevice, s);
     }
     if (e == hipSuccess && m->ncf > 0) {
         e = hipMemcpyAsync(m->cp, h->cp, ((size_t)m->ncf + 1) * 8, hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(m->cf, h->cf, (size_t)m->npool * 4, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && !store)
+            e = hipMemcpyAsync(m->cf, h->cf, (size_t)m->npool * 4, hipMemcpyHostToDevice, s);
     }
     // every column index and coefficient checked before any kernel uses them
     int32_t *flag = nullptr;
     int32_t hflag = 0;
     if (e == hipSuccess && (rc = dalloc(ctx, &flag, 1))) {
+        dfree(dmon); dfree(dsrc); dfree(dmap);
         delete m;
         return rc;
     }
     if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(int32_t), s);
+    if (e == hipSuccess && byid && m->nr > 0) {
+        const int32_t grid = m->nr < 8192 ? m->nr : 8192;
+        k_map_rows<<<grid, 256, 0, s>>>(m->nr, m->rp, dsrc, dmon, dmap, byid->nmap, m->ci, flag);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess && (m->nnz > 0 || m->npool > 0)) {
-        k_check_entries<<<1024, 256, 0, s>>>(m->nnz, m->ci, m->nc, m->npool, m->cf, m->p, flag);
+        k_check_entries<<<1024, 256, 0, s>>>(m->nnz, m->ci, m->nc, store ? 0 : m->npool, m->cf,
+                                             m->p, flag);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&hflag, flag, sizeof(int32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     dfree(flag);
+    dfree(dmon); dfree(dsrc); dfree(dmap);
     if (e != hipSuccess) {
         delete m;
         return check_hip(ctx, e, "upload");

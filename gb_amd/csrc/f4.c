@@ -652,7 +652,8 @@ typedef struct {
     int64_t *cp;
     uint32_t *cf;
     rkey_t *key;
-    int64_t crp, cci, crcf, ceid, ccp, ccf, ckey;
+    int64_t *src;  /* row q's monomial ids start at R->mon + src[q] */
+    int64_t crp, cci, crcf, ceid, ccp, ccf, ckey, csrc;
 } csrbuf_t;
 
 /* room for n elements of size elt; the old contents are not kept */
@@ -687,6 +688,11 @@ typedef struct {
     pcache_t *pc; /* product caches, one per thread */
     int32_t npc;
     uint32_t pass;
+    /* GPU back end: the elements' coefficient arrays on the device, element
+     * i's at cfoff[i] (cfoff[ncfs] = the store's size) */
+    gbgpu_cfstore *cfs;
+    int64_t *cfoff;
+    int32_t ncfs, cfoff_cap;
     uint32_t p;
     int32_t round;
     gbgpu_f4_la_t la;
@@ -695,9 +701,50 @@ typedef struct {
     int info;
 } f4_t;
 
+/* GPU back end: the basis elements' coefficient arrays not yet on the
+ * device, appended to the store in one piece */
+static int cfstore_sync(f4_t *F) {
+    const basis_t *B = &F->B;
+    gbgpu_f4_gpu_t *g = (gbgpu_f4_gpu_t *)F->user;
+    int rc;
+    if (!F->cfs && (rc = gbgpu_cfstore_create(g->ctx, F->p, &F->cfs)) != GBGPU_OK) return rc;
+    if (F->cfoff_cap < B->n + 1) {
+        int64_t *o = (int64_t *)realloc(F->cfoff, (size_t)(B->cap + 1) * sizeof(int64_t));
+        if (!o) return GBGPU_ENOMEM;
+        F->cfoff = o;
+        F->cfoff_cap = B->cap + 1;
+    }
+    if (F->ncfs == B->n) return GBGPU_OK;
+    int64_t n = 0;
+    for (int32_t i = F->ncfs; i < B->n; ++i) n += B->e[i].len;
+    uint32_t *tmp = (uint32_t *)malloc((size_t)(n ? n : 1) * sizeof(uint32_t));
+    if (!tmp) return GBGPU_ENOMEM;
+    int64_t w = 0;
+    for (int32_t i = F->ncfs; i < B->n; ++i) {
+        memcpy(tmp + w, B->e[i].c, (size_t)B->e[i].len * sizeof(uint32_t));
+        w += B->e[i].len;
+    }
+    int64_t off = 0;
+    rc = gbgpu_cfstore_append(F->cfs, tmp, n, &off);
+    free(tmp);
+    if (rc != GBGPU_OK) return rc;
+    for (int32_t i = F->ncfs; i < B->n; ++i) {
+        F->cfoff[i] = off;
+        off += B->e[i].len;
+    }
+    F->cfoff[B->n] = off;
+    F->ncfs = B->n;
+    return GBGPU_OK;
+}
+
 /* turn rows (element, multiplier) into gb's CSR over the given columns
- * (arrays in F->W) */
-static int build_csr(f4_t *F, rows_t *R, int32_t nc, int32_t ncl, gbgpu_csr_t *m) {
+ * (arrays in F->W).  byid: for the GPU back end, which forms the column
+ * indices on the device -- m->ci stays NULL and F->W.src gives each row's
+ * monomial ids in R->mon, mapped to columns by F->M.col
+ * (gbgpu_mat_upload_rows) -- and takes the coefficient arrays from the
+ * device store: row q uses element e's array (rcf = e, cp = F->cfoff, m->cf
+ * NULL; cfstore_sync first). */
+static int build_csr(f4_t *F, rows_t *R, int32_t nc, int32_t ncl, gbgpu_csr_t *m, int byid) {
     csrbuf_t *W = &F->W;
     int64_t nnz = 0;
     for (int64_t r = 0; r < R->n; ++r) nnz += R->r[r].len;
@@ -706,26 +753,45 @@ static int build_csr(f4_t *F, rows_t *R, int32_t nc, int32_t ncl, gbgpu_csr_t *m
     if (buf_fit((void **)&W->key, &W->ckey, R->n ? R->n : 1, sizeof(rkey_t)) ||
         buf_fit((void **)&W->eid, &W->ceid, F->B.n ? F->B.n : 1, sizeof(int32_t)) ||
         buf_fit((void **)&W->rp, &W->crp, R->n + 1, sizeof(int64_t)) ||
-        buf_fit((void **)&W->ci, &W->cci, nnz ? nnz : 1, sizeof(int32_t)) ||
+        (byid ? buf_fit((void **)&W->src, &W->csrc, R->n ? R->n : 1, sizeof(int64_t))
+              : buf_fit((void **)&W->ci, &W->cci, nnz ? nnz : 1, sizeof(int32_t))) ||
         buf_fit((void **)&W->rcf, &W->crcf, R->n ? R->n : 1, sizeof(int32_t)))
         return -1;
     rkey_t *key = W->key;
     int32_t *eid = W->eid, *ci = W->ci, *rcf = W->rcf;
     int64_t *rp = W->rp;
-    for (int32_t i = 0; i < F->B.n; ++i) eid[i] = -1;
     int32_t ncf = 0;
     int64_t npool = 0;
     for (int64_t r = 0; r < R->n; ++r) {
         key[r].key = F->M.col[R->r[r].lead];
         key[r].len = R->r[r].len;
         key[r].idx = r;
+    }
+    qsort(key, (size_t)R->n, sizeof(rkey_t), cmp_rkey);
+    if (byid) {
+        rp[0] = 0;
+        for (int64_t q = 0; q < R->n; ++q) {
+            const row_t *x = &R->r[key[q].idx];
+            rp[q + 1] = rp[q] + x->len;
+            rcf[q] = x->e;
+            W->src[q] = x->off;
+        }
+        m->nr = (int32_t)R->n;
+        m->nc = nc;
+        m->ncl = ncl;
+        m->ncf = F->ncfs;
+        m->p = F->p;
+        m->rp = rp; m->ci = NULL; m->rcf = rcf; m->cp = F->cfoff; m->cf = NULL;
+        return 0;
+    }
+    for (int32_t i = 0; i < F->B.n; ++i) eid[i] = -1;
+    for (int64_t r = 0; r < R->n; ++r) {
         const int32_t e = R->r[r].e;
         if (eid[e] < 0) {
             eid[e] = ncf++;
             npool += F->B.e[e].len;
         }
     }
-    qsort(key, (size_t)R->n, sizeof(rkey_t), cmp_rkey);
     if (buf_fit((void **)&W->cp, &W->ccp, ncf + 1, sizeof(int64_t)) ||
         buf_fit((void **)&W->cf, &W->ccf, npool ? npool : 1, sizeof(uint32_t)))
         return -1;
@@ -1092,6 +1158,25 @@ static int run_la(f4_t *F, const gbgpu_csr_t *m, int interreduce, gbgpu_rows_t *
     return rc;
 }
 
+/* the GPU back end (gbgpu_f4_gpu_la, la_option 1) on a matrix whose column
+ * indices the device forms from the rows' monomial ids (build_csr, byid) */
+static int run_la_byid(f4_t *F, const gbgpu_csr_t *m, gbgpu_rows_t *out) {
+    const double t0 = now_ms();
+    gbgpu_f4_gpu_t *g = (gbgpu_f4_gpu_t *)F->user;
+    gbgpu_mat *dm = NULL;
+    int32_t np = 0;
+    int rc = gbgpu_mat_upload_rows(g->ctx, m, F->R.mon, F->R.nmon, F->W.src, F->M.col, F->T.n,
+                                   F->cfs, &dm);
+    if (rc == GBGPU_OK) rc = gbgpu_la_sparse_dense(g->ctx, dm, g->form, &np);
+    if (rc == GBGPU_OK) {
+        g->device_ms += gbgpu_phase_ms(g->ctx, 2);
+        rc = gbgpu_rows_download(g->ctx, dm, out);
+    }
+    gbgpu_mat_free(dm);
+    F->out->la_ms += now_ms() - t0;
+    return rc;
+}
+
 /* new basis element from row k of the LA output (columns -> monomials) */
 static int take_row(f4_t *F, const gbgpu_rows_t *o, int32_t k, const int32_t *cols, poly_t *f) {
     const int64_t b = o->rp[k], e = o->rp[k + 1];
@@ -1184,7 +1269,11 @@ static int f4_round(f4_t *F) {
     qsort(cols + ncl, (size_t)(nc - ncl), sizeof(int32_t), cmp_mon_desc);
     for (int32_t c = 0; c < nc; ++c) F->M.col[cols[c]] = c;
     gbgpu_csr_t m;
-    if (build_csr(F, R, nc, ncl, &m)) goto done;
+    /* the GPU back end takes the rows as monomial ids */
+    const int byid = F->la == gbgpu_f4_gpu_la && F->user;
+    if (byid && (rc = cfstore_sync(F)) != GBGPU_OK) goto done;
+    rc = GBGPU_ENOMEM;
+    if (build_csr(F, R, nc, ncl, &m, byid)) goto done;
     int64_t nnz = m.rp[m.nr];
     F->out->build_ms += now_ms() - tp;
     if (m.nr > F->out->max_rows) F->out->max_rows = m.nr;
@@ -1192,7 +1281,7 @@ static int f4_round(f4_t *F) {
     if (nnz > F->out->max_nnz) F->out->max_nnz = nnz;
     F->out->rows_total += m.nr - ncl;
     const double t0 = now_ms();
-    if ((rc = run_la(F, &m, 0, &out)) != 0) goto done;
+    if ((rc = byid ? run_la_byid(F, &m, &out) : run_la(F, &m, 0, &out)) != 0) goto done;
     const double tla = now_ms() - t0;
     /* new elements in the order the LA returns them (decreasing lead) */
     rc = GBGPU_ENOMEM;
@@ -1261,7 +1350,7 @@ static int f4_reduce(f4_t *F) {
     qsort(cols, (size_t)nq, sizeof(int32_t), cmp_mon_desc);
     for (int64_t c = 0; c < nq; ++c) F->M.col[cols[c]] = (int32_t)c;
     gbgpu_csr_t m;
-    if (build_csr(F, R, (int32_t)nq, 0, &m)) goto done;
+    if (build_csr(F, R, (int32_t)nq, 0, &m, 0)) goto done;
     if ((rc = run_la(F, &m, 1, &out)) != 0) goto done;
     rc = GBGPU_ENOMEM;
     /* the reduced basis replaces the old one: the rows whose lead no
@@ -1468,12 +1557,14 @@ done:
     free(F.M.stamp); free(F.M.col); free(F.M.piv); free(F.M.red); free(F.M.red_n);
     free(F.R.r); free(F.R.mon);
     free(F.W.rp); free(F.W.ci); free(F.W.rcf); free(F.W.eid); free(F.W.cp); free(F.W.cf);
-    free(F.W.key);
+    free(F.W.key); free(F.W.src);
     for (int32_t i = 0; i < F.npc; ++i) {
         free(F.pc[i].key);
         free(F.pc[i].val);
     }
     free(F.pc);
+    gbgpu_cfstore_free(F.cfs);
+    free(F.cfoff);
     mt_free(&F.T);
     out->total_ms = now_ms() - t_start;
     return rc;

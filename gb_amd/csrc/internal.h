@@ -17,6 +17,7 @@ struct gbgpu_mat {
     int32_t *rcf = nullptr;
     int64_t *cp = nullptr;
     uint32_t *cf = nullptr;
+    bool cf_borrowed = false;   // cf belongs to a gbgpu_cfstore
     // known-pivot split (first row per lead = pivot, la_ff_32.c:1106-1116)
     int32_t split_lim = -1;     // leads were checked against [0, split_lim)
     int32_t *first = nullptr;   // nc: first row with that lead, INT32_MAX if none

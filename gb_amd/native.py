@@ -174,6 +174,12 @@ def lib():
         "gbgpu_phase_ms": ([vp, C.c_int], C.c_double),
         "gbgpu_mat_upload": ([vp, C.POINTER(_Csr), C.POINTER(vp)], C.c_int),
         "gbgpu_mat_free": ([vp], None),
+        "gbgpu_mat_upload_rows": ([vp, C.POINTER(_Csr), vp, i64, vp, vp, i32, vp, C.POINTER(vp)],
+                                  C.c_int),
+        "gbgpu_cfstore_create": ([vp, u32, C.POINTER(vp)], C.c_int),
+        "gbgpu_cfstore_append": ([vp, vp, i64, C.POINTER(C.c_int64)], C.c_int),
+        "gbgpu_cfstore_size": ([vp], C.c_int64),
+        "gbgpu_cfstore_free": ([vp], None),
         "gbgpu_sparse_pass": ([vp, vp, C.c_int, pi32], C.c_int),
         "gbgpu_sparse_pass_download": ([vp, vp, vp, vp], C.c_int),
         "gbgpu_sparse_pass_work": ([vp, vp, pi32, C.POINTER(C.c_uint64)], C.c_int),
@@ -274,6 +280,35 @@ class Context:
         self._check(lib().gbgpu_mat_upload(self.h, C.byref(s), C.byref(h)), "upload")
         return DeviceMatrix(self, h, m)
 
+    def upload_rows(self, m: F4Matrix, mon: np.ndarray, src: np.ndarray,
+                    colmap: np.ndarray, store: "CfStore | None" = None,
+                    cp: np.ndarray | None = None) -> "DeviceMatrix":
+        """Upload m with its column indices formed on the device
+        (gbgpu_mat_upload_rows): row q is mon[src[q] : src[q] + len(q)]
+        mapped through colmap; m.ci is not sent.  With a store, the
+        coefficient arrays are the store's at offsets cp (m.cp, m.cf unused)."""
+        s, keep = _csr(m)
+        s.ci = None
+        if store is not None:
+            cp = np.ascontiguousarray(cp, np.int64)
+            s.ncf = len(cp) - 1
+            s.cp = cp.ctypes.data
+            s.cf = None
+        mon = np.ascontiguousarray(mon, np.int32)
+        src = np.ascontiguousarray(src, np.int64)
+        colmap = np.ascontiguousarray(colmap, np.int32)
+        h = C.c_void_p()
+        self._check(lib().gbgpu_mat_upload_rows(self.h, C.byref(s), mon.ctypes.data, len(mon),
+                                                src.ctypes.data, colmap.ctypes.data, len(colmap),
+                                                store.h if store is not None else None,
+                                                C.byref(h)), "upload_rows")
+        return DeviceMatrix(self, h, m)
+
+    def cfstore(self, p: int) -> "CfStore":
+        h = C.c_void_p()
+        self._check(lib().gbgpu_cfstore_create(self.h, p, C.byref(h)), "cfstore")
+        return CfStore(self, h)
+
     def la_ff(self, m: F4Matrix, la_option: int = 1, form: int = FORM_PLAIN) -> SparseRows:
         s, keep = _csr(m)
         r = _Rows()
@@ -345,6 +380,34 @@ class Context:
         self._check(lib().gbgpu_dense_upload(self.h, a.shape[0], a.shape[1], p,
                                              a.ctypes.data, C.byref(h)), "dense upload")
         return DeviceDense(self, h, a.shape, p)
+
+
+class CfStore:
+    """Coefficient arrays kept on the device (gbgpu_cfstore)."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+
+    def append(self, cf: np.ndarray) -> int:
+        cf = np.ascontiguousarray(cf, np.uint32)
+        off = C.c_int64()
+        self.ctx._check(lib().gbgpu_cfstore_append(self.h, cf.ctypes.data, len(cf), C.byref(off)),
+                        "cfstore append")
+        return int(off.value)
+
+    def size(self) -> int:
+        return int(lib().gbgpu_cfstore_size(self.h))
+
+    def free(self):
+        if self.h:
+            lib().gbgpu_cfstore_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class DeviceMatrix:

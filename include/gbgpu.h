@@ -113,6 +113,29 @@ double gbgpu_phase_ms(const gbgpu_ctx *ctx, int phase);
 int gbgpu_mat_upload(gbgpu_ctx *ctx, const gbgpu_csr_t *m, gbgpu_mat **out);
 void gbgpu_mat_free(gbgpu_mat *m);
 
+/* Coefficient arrays kept on the device, appended once and shared by the
+ * matrices of later rounds (each basis element's array is sent once, not
+ * once per round).  Append checks every entry < p and returns the offset of
+ * the first; a matrix uploaded against the store must be freed before the
+ * next append (the store may move).  No counterpart in the reference (gb's
+ * matrices point into host coefficient arrays, hash.c:903-926). */
+typedef struct gbgpu_cfstore gbgpu_cfstore;
+int gbgpu_cfstore_create(gbgpu_ctx *ctx, uint32_t p, gbgpu_cfstore **out);
+int gbgpu_cfstore_append(gbgpu_cfstore *s, const uint32_t *cf, int64_t n, int64_t *off);
+int64_t gbgpu_cfstore_size(const gbgpu_cfstore *s);
+void gbgpu_cfstore_free(gbgpu_cfstore *s);
+
+/* The same matrix with its column indices formed on the device: m->ci is
+ * not read; row q's entries are the monomial ids mon[src[q] ..
+ * src[q] + rp[q+1] - rp[q]) and column = colmap[id] (ids in [0, nmap)).
+ * With a store, m->cf is not read either: m->cp are offsets into the store
+ * (m->cp[m->ncf] <= its size).  No counterpart in the reference: an F4 host
+ * that keeps its rows as monomial ids (gbgpu_f4 does) hands them over as
+ * they are instead of translating every entry on the CPU. */
+int gbgpu_mat_upload_rows(gbgpu_ctx *ctx, const gbgpu_csr_t *m, const int32_t *mon, int64_t nmon,
+                          const int64_t *src, const int32_t *colmap, int32_t nmap,
+                          const gbgpu_cfstore *store, gbgpu_mat **out);
+
 /* Replaces sparse_AB_CD_linear_algebra_ff_{8,16,32}
  * (la_ff_32.c:1081-1179, la_ff_16.c:737-836): reduce every to-reduce row
  * by the known pivots; the nonzero D' rows stay on the device.  *nrows
