@@ -443,7 +443,6 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
         const ex_t *ek = EV(T, lk);
         const uint64_t mk = T->dm[lk];
         const uint32_t dk = T->deg[lk];
-#pragma omp parallel for schedule(static) if (nl > 2048)
         for (int32_t q = 0; q < nl; ++q) {
             const int32_t i = B->lm[q];
             const int32_t li = B->e[i].m[0];
@@ -460,14 +459,13 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
             st[q] = B->red[i] ? 0 : (d == T->deg[li] + dk) ? 2 : 1;
         }
     }
-    /* old pairs: chain criterion (marks in parallel, then compaction) */
+    /* old pairs: chain criterion (marks, then compaction) */
     {
         const int64_t np = P->n;
         pair_t *pp = P->p;
         if (np > 0) {
             uint8_t *drop = (uint8_t *)malloc((size_t)np);
             if (!drop) { rc = -1; goto out; }
-#pragma omp parallel for schedule(static) if (np > 4096)
             for (int64_t q = 0; q < np; ++q) {
                 const pair_t x = pp[q];
                 drop[q] = mt_divides(T, lk, x.lcm) && !lcm_is(T, B->e[x.i].m[0], lk, x.lcm) &&
@@ -488,8 +486,10 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
      * (hash.c:553-557).  Coprime pairs still drop the pairs above them.
      * (Dropped pairs need not be left out as divisors: a dropped divisor
      * has an earlier divisor of its own.)  Each candidate is checked on its
-     * own, in parallel, against the candidates before it in (degree, index)
-     * order -- the only ones that can drop it -- lowest degree first. */
+     * own against the candidates before it in (degree, index) order -- the
+     * only ones that can drop it -- lowest degree first.  (One element's
+     * update is a few hundred microseconds at most: it stays on one thread;
+     * a fork and join per element costs more than it saves.) */
     {
         uint8_t *keep = (uint8_t *)malloc(nn);
         int32_t *byd = (int32_t *)malloc(nn * sizeof(int32_t));
@@ -511,7 +511,6 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
             byd[pos[q]] = q;
         }
         free(cnt);
-#pragma omp parallel for schedule(dynamic, 16) if (nl > 256)
         for (int32_t q = 0; q < nl; ++q) {
             keep[q] = st[q];
             if (st[q] != 1) continue;
@@ -673,8 +672,11 @@ static int buf_fit(void **p, int64_t *cap, int64_t n, size_t elt) {
  * monomial t, the last product it formed with t and the (pass, u) it
  * belongs to; a pass number is never reused, so nothing is ever cleared. */
 typedef struct {
-    uint64_t *key;  /* pass << 32 | (u + 1); 0 never matches */
-    int32_t *val;   /* u t */
+    uint64_t key;   /* pass << 32 | (u + 1); 0 never matches */
+    int64_t val;    /* u t */
+} pcent_t;
+typedef struct {
+    pcent_t *e;     /* per monomial t: one 16-byte entry, one cache access */
     int32_t cap;
 } pcache_t;
 
@@ -906,12 +908,10 @@ static int pcache_fit(f4_t *F, int32_t n) {
         if (c->cap >= n) continue;
         int32_t nc = c->cap ? c->cap : 4096;
         while (nc < n) nc *= 2;
-        uint64_t *k = (uint64_t *)realloc(c->key, (size_t)nc * sizeof(uint64_t));
-        if (k) c->key = k;
-        int32_t *v = (int32_t *)realloc(c->val, (size_t)nc * sizeof(int32_t));
-        if (v) c->val = v;
-        if (!k || !v) return -1;
-        memset(c->key + c->cap, 0, (size_t)(nc - c->cap) * sizeof(uint64_t));
+        pcent_t *e = (pcent_t *)realloc(c->e, (size_t)nc * sizeof(pcent_t));
+        if (!e) return -1;
+        c->e = e;
+        memset(c->e + c->cap, 0, (size_t)(nc - c->cap) * sizeof(pcent_t));
         c->cap = nc;
     }
     return 0;
@@ -922,18 +922,65 @@ static int cmp_u64(const void *a, const void *b) {
     return x < y ? -1 : (x > y);
 }
 
+/* marks of first sight, gathered per thread and queued together */
+typedef struct {
+    int32_t *v;
+    int64_t n, cap;
+    int bad;
+} mlist_t;
+
+/* first sight of monomial m in this round: whichever thread swaps the stamp
+ * queues it (the queue order does not matter: columns are sorted, reducers
+ * do not depend on the order) */
+static inline void mark_seen(f4_t *F, int32_t m, int32_t round, mlist_t *L) {
+    int32_t *stamp = F->M.stamp;
+    if (__atomic_load_n(&stamp[m], __ATOMIC_RELAXED) == round) return;
+    if (__atomic_exchange_n(&stamp[m], round, __ATOMIC_RELAXED) == round) return;
+    F->M.col[m] = -1;
+    F->M.piv[m] = 0;
+    if (L->n == L->cap) {
+        const int64_t nc = L->cap ? 2 * L->cap : 1024;
+        int32_t *nv = (int32_t *)realloc(L->v, (size_t)nc * sizeof(int32_t));
+        if (!nv) {
+            L->bad = 1;
+            return;
+        }
+        L->v = nv;
+        L->cap = nc;
+    }
+    L->v[L->n++] = m;
+}
+
+static int queue_push(int32_t **queue, int64_t *nq, int64_t *capq, const int32_t *v, int64_t n);
+static int rows_mark(f4_t *F, rows_t *R, int64_t r0, int64_t r1, int32_t **queue, int64_t *nq,
+                     int64_t *capq);
+
+/* a thread's marks into the queue (inside a parallel region) */
+static void mlist_flush(mlist_t *L, int32_t **queue, int64_t *nq, int64_t *capq, int *bad) {
+#pragma omp critical
+    {
+        if (L->bad || (L->n && queue_push(queue, nq, capq, L->v, L->n))) *bad = 1;
+    }
+    free(L->v);
+}
+
 /* The monomials of rows r0 .. r1-1: every product u t looked up in the
  * table while it is read only (rows taken by multiplier, so that the
- * product caches hit), then the missing ones inserted by all threads. */
-static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
+ * product caches hit), then the missing ones inserted by all threads.  A
+ * product met for the first time in this round is marked and queued as it
+ * is formed (a product cache hit was marked when it was first formed); the
+ * rows' leads are marked as pivots at the end. */
+static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1, int32_t **queue, int64_t *nq,
+                         int64_t *capq) {
     mt_t *T = &F->T;
     const basis_t *B = &F->B;
     const int nw = T->nvp / 4;
     const int64_t nr = r1 - r0;
+    const int32_t round = F->round;
     if (nr <= 0) return 0;
     /* the rows by multiplier */
     uint64_t *ord = (uint64_t *)malloc((size_t)nr * sizeof(uint64_t));
-    if (!ord || pcache_fit(F, T->n)) {
+    if (!ord || pcache_fit(F, T->n) || marks_fit(&F->M, T->n)) {
         free(ord);
         return -1;
     }
@@ -942,11 +989,13 @@ static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
     qsort(ord, (size_t)nr, sizeof(uint64_t), cmp_u64);
     int64_t nmiss = 0;
     uint64_t high = 0;
+    int bad = 0;
     const uint64_t pass = (uint64_t)(++F->pass) << 32;
 #pragma omp parallel reduction(+ : nmiss) reduction(| : high)
     {
         uint64_t tmp[nw];
         pcache_t *pc = &F->pc[omp_get_thread_num()];
+        mlist_t L = {NULL, 0, 0, 0};
 #pragma omp for schedule(dynamic, 8)
         for (int64_t q = 0; q < nr; ++q) {
             const row_t *x = &R->r[r0 + (int64_t)(uint32_t)ord[q]];
@@ -957,8 +1006,9 @@ static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
             int32_t *o = R->mon + x->off;
             for (int32_t t = 0; t < f->len; ++t) {
                 const int32_t mt = f->m[t];
-                if (pc->key[mt] == key) {
-                    o[t] = pc->val[mt];
+                const pcent_t ce = pc->e[mt];
+                if (ce.key == key) {
+                    o[t] = (int32_t)ce.val;
                     continue;
                 }
                 const uint64_t *et = (const uint64_t *)EV(T, mt);
@@ -974,20 +1024,23 @@ static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
                     o[t] = 0;
                     continue;
                 }
-                o[t] = mt_find(T, (const ex_t *)tmp, hm + T->hv[mt]);
-                pc->key[mt] = key;
-                pc->val[mt] = o[t];
-                nmiss += o[t] < 0;
+                const int32_t id = mt_find(T, (const ex_t *)tmp, hm + T->hv[mt]);
+                o[t] = id;
+                pc->e[mt].key = key;
+                pc->e[mt].val = id;
+                if (id >= 0) mark_seen(F, id, round, &L);
+                else ++nmiss;
             }
         }
+        mlist_flush(&L, queue, nq, capq, &bad);
     }
     if (high & EX_HIGH) {
         free(ord);
         T->erange = 1;
         return -1;
     }
-    int rc = 0;
-    if (nmiss) {
+    int rc = bad ? -1 : 0;
+    if (nmiss && rc == 0) {
         /* new monomials: inserted by all threads into reserved room (each
          * is missed at least once, so nmiss bounds their number; the
          * room is capped) */
@@ -995,7 +1048,7 @@ static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
         const int64_t cap_room = (int64_t)T->n * 2 > (1 << 22) ? (int64_t)T->n * 2 : (1 << 22);
         if (room > cap_room) room = cap_room;
         int done = 0;
-        if (mt_reserve(T, room) == 0) {
+        if (mt_reserve(T, room) == 0 && marks_fit(&F->M, (int32_t)(T->n + room)) == 0) {
             const int32_t lim = (int32_t)(T->n + room);
             const uint64_t pass2 = (uint64_t)(++F->pass) << 32;
             int over = 0;
@@ -1003,6 +1056,7 @@ static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
             {
                 uint64_t tmp[nw];
                 pcache_t *pc = &F->pc[omp_get_thread_num()];
+                mlist_t L = {NULL, 0, 0, 0};
 #pragma omp for schedule(dynamic, 8)
                 for (int64_t q = 0; q < nr; ++q) {
                     const row_t *x = &R->r[r0 + (int64_t)(uint32_t)ord[q]];
@@ -1014,36 +1068,49 @@ static int rows_products(f4_t *F, rows_t *R, int64_t r0, int64_t r1) {
                     for (int32_t t = 0; t < f->len; ++t) {
                         if (o[t] >= 0) continue;
                         const int32_t mt = f->m[t];
-                        if (pc->key[mt] == key && pc->val[mt] >= 0) {
-                            o[t] = pc->val[mt];
+                        const pcent_t ce = pc->e[mt];
+                        if (ce.key == key && ce.val >= 0) {
+                            o[t] = (int32_t)ce.val;
                             continue;
                         }
                         const uint64_t *et = (const uint64_t *)EV(T, mt);
                         for (int w = 0; w < nw; ++w) tmp[w] = em[w] + et[w];
-                        o[t] = mt_get_par(T, (const ex_t *)tmp, hm + T->hv[mt], lim, &over);
-                        pc->key[mt] = key;
-                        pc->val[mt] = o[t];
+                        const int32_t id =
+                            mt_get_par(T, (const ex_t *)tmp, hm + T->hv[mt], lim, &over);
+                        o[t] = id;
+                        pc->e[mt].key = key;
+                        pc->e[mt].val = id;
+                        if (id >= 0) mark_seen(F, id, round, &L);
                     }
                 }
+                mlist_flush(&L, queue, nq, capq, &bad);
             }
             if (T->n > lim) T->n = lim;
             done = !over;
+            if (bad) rc = -1;
         }
-        /* whatever is left (no room, or the room ran out): one thread */
-        for (int64_t r = r0; r < r1 && !done && rc == 0; ++r) {
-            const row_t *x = &R->r[r];
-            const poly_t *f = &B->e[x->e];
-            int32_t *o = R->mon + x->off;
-            for (int32_t t = 0; t < f->len; ++t)
-                if (o[t] < 0 && (o[t] = mt_mul(T, x->mul, f->m[t])) < 0) {
-                    rc = -1;
-                    break;
-                }
+        if (!done && rc == 0) {
+            /* whatever is left (no room, or the room ran out): one thread,
+             * then a marking pass over the rows */
+            for (int64_t r = r0; r < r1 && rc == 0; ++r) {
+                const row_t *x = &R->r[r];
+                const poly_t *f = &B->e[x->e];
+                int32_t *o = R->mon + x->off;
+                for (int32_t t = 0; t < f->len; ++t)
+                    if (o[t] < 0 && (o[t] = mt_mul(T, x->mul, f->m[t])) < 0) {
+                        rc = -1;
+                        break;
+                    }
+            }
+            if (rc == 0 && rows_mark(F, R, r0, r1, queue, nq, capq)) rc = -1;
         }
     }
     free(ord);
     if (rc) return rc;
-    for (int64_t r = r0; r < r1; ++r) R->r[r].lead = R->mon[R->r[r].off];
+    for (int64_t r = r0; r < r1; ++r) {
+        R->r[r].lead = R->mon[R->r[r].off];
+        F->M.piv[R->r[r].lead] = 1;
+    }
     return 0;
 }
 
@@ -1061,9 +1128,8 @@ static int queue_push(int32_t **queue, int64_t *nq, int64_t *capq, const int32_t
     return 0;
 }
 
-/* first sight of a monomial in this round: whichever thread swaps the
- * stamp queues it (the queue order does not matter: columns are sorted,
- * reducers do not depend on the order) */
+/* marking pass over rows r0 .. r1-1 (the serial insertion path of
+ * rows_products, whose products were not marked as they were formed) */
 static int rows_mark(f4_t *F, rows_t *R, int64_t r0, int64_t r1, int32_t **queue, int64_t *nq,
                      int64_t *capq) {
     if (marks_fit(&F->M, F->T.n)) return -1;
@@ -1130,7 +1196,7 @@ static int symbolic(f4_t *F, rows_t *R, int32_t **queue, int64_t *nq, int64_t *c
             if (mul < 0 || row_stub(R, B, e, mul)) { free(red); return -1; }
         }
         free(red);
-        if (rows_products(F, R, r0, R->n) || rows_mark(F, R, r0, R->n, queue, nq, capq)) return -1;
+        if (rows_products(F, R, r0, R->n, queue, nq, capq)) return -1;
         qdone = qend;
     }
     return 0;
@@ -1246,7 +1312,7 @@ static int f4_round(f4_t *F) {
         if (q > 0 && gen[q].e == gen[q - 1].e && gen[q].mul == gen[q - 1].mul) continue;
         if (row_stub(R, B, gen[q].e, gen[q].mul)) goto done;
     }
-    if (rows_products(F, R, 0, R->n) || rows_mark(F, R, 0, R->n, &queue, &nq, &capq)) goto done;
+    if (rows_products(F, R, 0, R->n, &queue, &nq, &capq)) goto done;
     const int64_t nspair_rows = R->n;
     F->out->select_ms += now_ms() - tp;
     tp = now_ms();
@@ -1339,7 +1405,7 @@ static int f4_reduce(f4_t *F) {
     }
     for (int32_t q = 0; q < B->nlm; ++q)
         if (row_stub(R, B, B->lm[q], one)) goto done;
-    if (rows_products(F, R, 0, R->n) || rows_mark(F, R, 0, R->n, &queue, &nq, &capq)) goto done;
+    if (rows_products(F, R, 0, R->n, &queue, &nq, &capq)) goto done;
     /* reducers for the tail monomials divisible by a lead, so that the
      * reduced rows have no such monomial left */
     if (symbolic(F, R, &queue, &nq, &capq)) goto done;
@@ -1558,10 +1624,7 @@ done:
     free(F.R.r); free(F.R.mon);
     free(F.W.rp); free(F.W.ci); free(F.W.rcf); free(F.W.eid); free(F.W.cp); free(F.W.cf);
     free(F.W.key); free(F.W.src);
-    for (int32_t i = 0; i < F.npc; ++i) {
-        free(F.pc[i].key);
-        free(F.pc[i].val);
-    }
+    for (int32_t i = 0; i < F.npc; ++i) free(F.pc[i].e);
     free(F.pc);
     gbgpu_cfstore_free(F.cfs);
     free(F.cfoff);
