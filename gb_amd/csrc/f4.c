@@ -34,6 +34,7 @@
 #include <string.h>
 #include <time.h>
 
+#include <emmintrin.h>
 #include <omp.h>
 
 #include "../../include/gbgpu_f4.h"
@@ -414,6 +415,38 @@ static inline int lcm_is(const mt_t *T, int32_t a, int32_t b, int32_t l) {
     return 1;
 }
 
+/* lcm of two exponent vectors (nvp lanes, each <= EX_MAX) into e; returns
+ * its degree, and in *up bit v & 63 set where a[v] > b[v].  SSE2: eight
+ * lanes at a time; the signed 16-bit max and compare are exact below 2^15. */
+static inline uint32_t lcm_lanes(const ex_t *a, const ex_t *b, ex_t *e, int nvp, uint64_t *up) {
+    const __m128i one = _mm_set1_epi16(1), zero = _mm_setzero_si128();
+    __m128i acc = zero;
+    uint64_t m = 0;
+    int v = 0;
+    for (; v + 8 <= nvp; v += 8) {
+        const __m128i x = _mm_loadu_si128((const __m128i *)(a + v));
+        const __m128i y = _mm_loadu_si128((const __m128i *)(b + v));
+        const __m128i mx = _mm_max_epi16(x, y);
+        _mm_storeu_si128((__m128i *)(e + v), mx);
+        acc = _mm_add_epi32(acc, _mm_madd_epi16(mx, one));
+        const __m128i gt = _mm_cmpgt_epi16(x, y);
+        m |= (uint64_t)(_mm_movemask_epi8(_mm_packs_epi16(gt, zero)) & 0xFF) << (v & 63);
+    }
+    if (v < nvp) {  /* nvp is a multiple of 4: one half vector left */
+        const __m128i x = _mm_loadl_epi64((const __m128i *)(a + v));
+        const __m128i y = _mm_loadl_epi64((const __m128i *)(b + v));
+        const __m128i mx = _mm_max_epi16(x, y);
+        _mm_storel_epi64((__m128i *)(e + v), mx);
+        acc = _mm_add_epi32(acc, _mm_madd_epi16(mx, one));
+        const __m128i gt = _mm_cmpgt_epi16(x, y);
+        m |= (uint64_t)(_mm_movemask_epi8(_mm_packs_epi16(gt, zero)) & 0xF) << (v & 63);
+    }
+    acc = _mm_add_epi32(acc, _mm_shuffle_epi32(acc, 0x4E));
+    acc = _mm_add_epi32(acc, _mm_shuffle_epi32(acc, 0xB1));
+    *up = m;
+    return (uint32_t)_mm_cvtsi128_si32(acc);
+}
+
 static inline int ediv(const ex_t *a, const ex_t *b, int nv) {
     for (int v = 0; v < nv; ++v)
         if (a[v] > b[v]) return 0;
@@ -431,30 +464,26 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
         B->red[k] = 1;
         return pairs_push(P, x);
     }
-    /* candidate pairs (i, k): lcm exponents, degree and mask kept locally;
-     * only the pairs that survive get a monomial id */
+    /* candidate pairs (i, k): lcm exponents and degree kept locally, and
+     * the variables of lcm / lead(k) (bit v & 63); only the pairs that
+     * survive get a monomial id */
     const size_t nn = nl ? (size_t)nl : 1;
     ex_t *le = (ex_t *)malloc(nn * (size_t)nvp * sizeof(ex_t));
     uint32_t *ld = (uint32_t *)malloc(nn * sizeof(uint32_t));
-    uint64_t *lm = (uint64_t *)malloc(nn * sizeof(uint64_t));
+    uint64_t *am = (uint64_t *)malloc(nn * sizeof(uint64_t));
     uint8_t *st = (uint8_t *)malloc(nn);  /* 0 dropped, 1 kept, 2 coprime */
-    if (!le || !ld || !lm || !st) { rc = -1; goto out; }
+    const uint32_t dk = T->deg[lk];
+    if (!le || !ld || !am || !st) { rc = -1; goto out; }
     {
         const ex_t *ek = EV(T, lk);
-        const uint64_t mk = T->dm[lk];
-        const uint32_t dk = T->deg[lk];
         for (int32_t q = 0; q < nl; ++q) {
             const int32_t i = B->lm[q];
             const int32_t li = B->e[i].m[0];
             const ex_t *ei = EV(T, li);
-            ex_t *e = le + (size_t)q * nvp;
-            uint32_t d = 0;
-            for (int v = 0; v < nvp; ++v) {
-                e[v] = ei[v] > ek[v] ? ei[v] : ek[v];
-                d += e[v];
-            }
+            uint64_t m;
+            const uint32_t d = lcm_lanes(ei, ek, le + (size_t)q * nvp, nvp, &m);
             ld[q] = d;
-            lm[q] = T->dm[li] | mk;  /* the variables of the lcm */
+            am[q] = m;
             /* pairs with redundant elements (kept as reducers in lex) are not formed */
             st[q] = B->red[i] ? 0 : (d == T->deg[li] + dk) ? 2 : 1;
         }
@@ -511,14 +540,27 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
             byd[pos[q]] = q;
         }
         free(cnt);
+        /* lcm(r) | lcm(q) iff lcm(r) / lead(k) divides lcm(q) / lead(k):
+         * its variables are a subset of q's.  A quotient of degree 1 is one
+         * variable x_v, which divides every quotient of higher degree that
+         * holds x_v: sv collects those variables (exact while nv <= 64, when
+         * bit v stands for x_v alone). */
+        uint64_t sv = 0;
+        if (nv <= 64)
+            for (int32_t q = 0; q < nl; ++q)
+                if (st[q] != 0 && ld[q] == dk + 1) sv |= am[q];
         for (int32_t q = 0; q < nl; ++q) {
             keep[q] = st[q];
             if (st[q] != 1) continue;
+            const uint64_t mq = am[q];
+            if (ld[q] > dk + 1 && (mq & sv)) {
+                keep[q] = 0;
+                continue;
+            }
             const ex_t *eq = le + (size_t)q * nvp;
-            const uint64_t mq = lm[q];
             for (int32_t a = 0; a < pos[q]; ++a) {
                 const int32_t r = byd[a];
-                if (st[r] == 0 || (lm[r] & ~mq)) continue;
+                if (st[r] == 0 || (am[r] & ~mq)) continue;
                 const ex_t *er = le + (size_t)r * nvp;
                 if (ld[r] == ld[q] ? ev_eq(er, eq, nvp) : ediv(er, eq, nv)) {
                     keep[q] = 0;
@@ -556,7 +598,7 @@ static int update_pairs(mt_t *T, basis_t *B, pairs_t *P, int32_t k, int32_t lo) 
 out:
     free(le);
     free(ld);
-    free(lm);
+    free(am);
     free(st);
     return rc;
 }

@@ -162,3 +162,33 @@ def test_f4_host_katsura_12_matrices_per_round():
     assert (r9[2], r9[3], r9[2] - r9[4], r9[6]) == (27707, 27798, 2064, 120)
     assert round(100.0 * r9[5] / (r9[2] * r9[3]), 3) == 1.204
     assert len(b.lens) == 1050 and int(np.sum(b.lens)) == 1481052
+
+
+@pytest.mark.parametrize("name", ["cyclic_6_drl_17_bit_la1_t1", "cyclic_5_lex_17_bit",
+                                  "red_gb_17_bit"])
+def test_f4_host_more_than_64_variables(name):
+    """gb's golden systems placed into a ring of 70 variables, the k-th
+    variable sent to an increasing index set that mixes indices below 64
+    with indices 64 and up (whose divisibility-mask bits v & 63 coincide
+    with the low ones'), the others unused: the monomial order on the used
+    variables is unchanged, so the basis is the golden basis with its
+    variables sent the same way."""
+    c = GOLDEN[name]
+    nv = c["nr_vars"]
+    big = 70
+    where = np.array(([0, 1, 2, 64, 65, 66, 67])[:nv])
+    assert np.all(np.diff(where) > 0)
+
+    def spread(flat, nterms):
+        e = np.zeros((nterms, big), np.int64)
+        e[:, where] = np.array(flat).reshape(nterms, nv)
+        return e.reshape(-1)
+
+    n = sum(c["lens"])
+    s = native.PolySystem(nv=big, p=c["field_char"], lens=np.array(c["lens"]),
+                          exps=spread(c["exps"], n), cfs=np.array(c["cfs"]),
+                          order=c["mon_order"])
+    b = oracle.f4(s, nthreads=2, reduce_gb=c["reduce_gb"])
+    assert list(b.lens) == c["tlen"]
+    assert list(b.cfs) == c["tcf"]
+    assert np.array_equal(b.exps.reshape(-1), spread(c["texp"], sum(c["tlen"])))
