@@ -79,38 +79,33 @@ __device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p) 
 // found so far (E) and a tile of candidate rows (rows that are no pivot yet,
 // kept in the list cand[head, count) in any order) loaded into the free
 // slots in list order, each with a unit transform.  The tile is first
-// reduced by E (normalised and reduced within the panel).  Then, repeatedly,
-// the leftmost column in which an unselected tile row is nonzero is the next
-// pivot column, the lowest slot nonzero there gives the pivot row, with
-// pivot value a, and every other slot x is replaced by a x - x[col] * pivot
-// row, without division: no inverse sits on the per-pivot path, the pivot
-// row becomes a row of E where it is, and the column is cleared everywhere
-// else.  At the end of a tile the E rows are normalised, their inverses
-// computed in parallel.  Tile rows that are never selected end up zero on
-// the panel.  A row's transform only ever takes in pivot rows, so the
-// transform T of E over the selected rows gives E = T * A[selected rows,
-// panel], and Rp = T * A[selected rows, c0:n] is in reduced form with an
-// identity on the panel's pivot columns.  Selected rows are swapped to the
-// front of the candidate list.
+// reduced by E (normalised and reduced within the panel).  Then the panel's
+// columns are swept from the left in sub-panels of SW columns:
+//
+//   chain (one wave, lane = slot, the sub-panel's entries in registers):
+//     for each column c of the sub-panel, the lowest unselected tile row
+//     nonzero there is the pivot (columns left of c are zero in every
+//     unselected row), with pivot value a; every other slot x becomes
+//     a x - x[c] * pivot row, without division: no inverse sits on the
+//     per-pivot path.  Each slot's new row is kept as a combination of the
+//     rows at the start of the sub-panel: D[s] * row s + sum_c C[s][c] *
+//     (row of the pivot of column c);
+//   apply (every wave, lane = slot, a wave per column): the same
+//     combination on every column, the sub-panel's and the transform part
+//     included -- a 64 x 64 by 64 x SW product per sub-panel instead of a
+//     barrier per pivot.  (The chain only keeps the entries right of the
+//     current column: every other row is scaled by the pivot value, which
+//     the combination carries.)
+//
+// At the end of a tile the E rows are normalised, their inverses computed
+// in parallel.  Tile rows that are never selected end up zero on the panel.
+// A row's transform only ever takes in pivot rows, so the transform T of E
+// over the selected rows gives E = T * A[selected rows, panel], and Rp =
+// T * A[selected rows, c0:n] is in reduced form with an identity on the
+// panel's pivot columns.  Selected rows are swapped to the front of the
+// candidate list.
 
 constexpr int XW = 2 * PB + 1;  // padded slot row: panel part, transform part
-
-// wave 0: the smallest first-nonzero column over the lanes (slots) and the
-// lowest lane holding it; every lane stores the same two values
-__device__ __forceinline__ void pivot_search(int fc, int32_t &sh_js, int32_t &sh_is) {
-    // fc <= PB has 7 bits: the minimum is found bit by bit from the top, each
-    // ballot keeping the lanes that can still hold it
-    uint64_t live = ~0ull;
-    int mn = 0;
-#pragma unroll
-    for (int b = 6; b >= 0; --b) {
-        const uint64_t z = live & (uint64_t)__ballot(((fc >> b) & 1) == 0);
-        if (z) live = z;
-        else mn |= 1 << b;
-    }
-    sh_js = mn;
-    sh_is = __ffsll((unsigned long long)live) - 1;
-}
 
 // a x + b y mod p for p < 2^16, residues a, x, y and b <= p: both products
 // are below 2^32 (24-bit multiplies), the quotient of their sum by p is
@@ -124,27 +119,260 @@ __device__ __forceinline__ uint32_t mad2_mod16(uint32_t a, uint32_t x, uint32_t 
     const uint32_t r = s + t - __umul24(q, p);
     return min(min(r, r + p), r - p);
 }
-constexpr int SEL_T = 1024;     // threads of the select workgroup
-constexpr int SEL_RG = SEL_T / (2 * PB);  // row groups: thread = (row group, column)
-constexpr int SEL_RS = PB / SEL_RG;       // slots per thread
+
+// a x + b y mod p for residues a, x, y and b <= p (two products below 2^62)
+template <int P16>
+__device__ __forceinline__ uint32_t sel_mad(uint32_t a, uint32_t x, uint32_t b, uint32_t y,
+                                            const Mod &md, float pinv) {
+    if (P16 == 1) return mad2_mod16(a, x, b, y, md.p, pinv);
+    return md.reduce((uint64_t)a * x + (uint64_t)b * y);
+}
+
+constexpr int SEL_T = 512;            // threads of the select workgroup
+constexpr int SEL_W = SEL_T / 64;     // its waves
+constexpr int SW = 16;                // columns of a sub-panel of the pivot chain
+
+struct SelShared {
+    uint32_t X[PB][XW];
+    uint32_t C[PB][PB + 1];           // combination coefficients of each slot
+    uint32_t D[PB];                   // ... and its own scale
+    int32_t qs[PB];                   // the slots the coefficients refer to
+    uint32_t einv[PB];
+    int32_t ecol[PB], eslot[PB], erow[PB], epos[PB], srow[PB], spos[PB];
+    // the tile's reduction by E: coefficients and slots (kept apart from the
+    // chain's, the transform part is reduced after a chain)
+    uint32_t CE[PB][PB + 1];
+    int32_t qse[PB];
+    int32_t nzw[SEL_W];               // waves with a nonzero on the sub-panel
+    // a chain step's pivot slot, pivot value and column entries, in two
+    // buffers by step parity: a step's owner writes one while the other waves
+    // may still read the last step's
+    uint32_t f[2][PB];
+    int32_t sis[2];
+    uint32_t sav[2];
+    // the move of the selected rows to the front of the candidate list
+    int32_t fr[PB], fpos[PB], mvj[PB], mvv[PB];
+};
+
+// X[s][col] <- D[s] X[s][col] + sum_{q < nq} Cm[s][q] X[qm[q]][col] for every
+// slot s, over the panel columns in the mask pcm and, with tr, the
+// transform part (D[s] = 1 with unitd).  Lane = slot; a wave takes every
+// SEL_W-th column: a column's entries of every slot are read by one wave
+// before it writes them, and no other wave touches that column.  The terms
+// go in chunks of 16 with no branch inside a chunk (Cm is zero past nq up to
+// the chunk's end), so a chunk's 16 reads are in flight together.  KIND 0
+// (p < 2^16): products below 2^32 from 24-bit multiplies, summed in 64
+// bits; KIND 1 (p < 2^17): products below 2^34, 65 of them fit in 64 bits;
+// KIND 2: sums carry into hi.
+constexpr int QC = 16;  // terms per chunk
+
+template <int NQ, int KIND>
+__device__ __forceinline__ void sel_apply(SelShared &sh, const uint32_t (*Cm)[PB + 1],
+                                          const int32_t *qm, bool unitd, int nq, uint64_t pcm,
+                                          bool tr, const Mod &md, uint32_t r64) {
+    static_assert(NQ % QC == 0, "whole chunks");
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    nq = __builtin_amdgcn_readfirstlane(nq);
+    // one chunk's coefficients and row offsets (word offset of the row of
+    // slot qm[q]) in registers: loaded once for a single chunk, per chunk
+    // and column pair otherwise
+    uint32_t cq[QC];
+    int32_t qo[QC];
+    auto chunk = [&](int k) {
+#pragma unroll
+        for (int j = 0; j < QC; ++j) {
+            cq[j] = Cm[lane][k * QC + j];
+            qo[j] = qm[k * QC + j] * XW;
+        }
+    };
+    if (NQ == QC) chunk(0);
+    const uint32_t dv = unitd ? 1u : sh.D[lane];
+    const uint32_t *xf = &sh.X[0][0];
+    // two columns at once (cb = -1: none), so that their reads overlap
+    auto two = [&](int ca, int cb) {
+        const int cs[2] = {ca, cb < 0 ? ca : cb};  // indexed by unrolled loops only
+        uint64_t lo[2];
+        uint32_t hi[2] = {0, 0};
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            // __umul24 returns an int: the low 32 bits go through uint32_t,
+            // not a sign extension
+            lo[u] = KIND == 0 ? (uint64_t)(uint32_t)__umul24(dv, sh.X[lane][cs[u]])
+                              : (uint64_t)dv * sh.X[lane][cs[u]];
+        }
+#pragma unroll
+        for (int k = 0; k < NQ / QC; ++k) {
+            if (k * QC < nq) {
+                if (NQ != QC) chunk(k);
+                uint32_t pv[2][QC];
+#pragma unroll
+                for (int j = 0; j < QC; ++j)
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) pv[u][j] = xf[qo[j] + cs[u]];
+#pragma unroll
+                for (int j = 0; j < QC; ++j) {
+                    const uint32_t c = cq[j];
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        if (KIND == 0) {
+                            lo[u] += (uint64_t)(uint32_t)__umul24(c, pv[u][j]);
+                        } else if (KIND == 1) {
+                            lo[u] += (uint64_t)c * pv[u][j];
+                        } else {
+                            const uint64_t o = lo[u];
+                            lo[u] += (uint64_t)c * pv[u][j];
+                            hi[u] += lo[u] < o;
+                        }
+                    }
+                }
+            }
+        }
+        uint32_t r[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            r[u] = md.reduce(lo[u]);
+            if (KIND == 2 && hi[u]) {
+                r[u] += md.mul(hi[u], r64);
+                if (r[u] >= md.p) r[u] -= md.p;
+            }
+        }
+        sh.X[lane][ca] = r[0];
+        if (cb >= 0) sh.X[lane][cb] = r[1];
+    };
+    // panel columns: this wave's are the set bits wv, wv + SEL_W, ... of pcm
+    uint64_t m = pcm;
+    for (int i = 0; i < wv && m; ++i) m &= m - 1;
+    while (m) {
+        const int ca = __builtin_ctzll(m);
+        for (int i = 0; i < SEL_W && m; ++i) m &= m - 1;
+        int cb = -1;
+        if (m) {
+            cb = __builtin_ctzll(m);
+            for (int i = 0; i < SEL_W && m; ++i) m &= m - 1;
+        }
+        two(ca, cb);
+    }
+    if (tr)
+        for (int col = PB + wv; col < 2 * PB; col += 2 * SEL_W) two(col, col + SEL_W);
+}
+
+// The pivot chain over the sub-panel [cb, cb + SW) of the tile rows um,
+// run by every wave (lane = slot): column j of the sub-panel and its
+// combination coefficient belong to wave j % SEL_W, the row scale D is kept
+// by every wave.  Step c: the owner of column c finds the pivot (the lowest
+// unselected tile row nonzero there) and posts it with every slot's entry
+// of column c; after one barrier every wave eliminates its own columns.
+// Records the pivots as E rows kk, kk + 1, ... and leaves the slots'
+// combinations in C (column c of the sub-panel: coefficient of the row that
+// pivots there, zero if none), D and qs.  Returns the number of pivots (the
+// same in every thread); um loses the selected slots.  X is only read: the
+// apply that follows writes every column.
+constexpr int NCW = SW / SEL_W;  // sub-panel columns per wave
+static_assert(SW % SEL_W == 0, "sub-panel columns split evenly over the waves");
+
+template <int P16>
+__device__ __forceinline__ int sel_chain(SelShared &sh, int cb, int bw, uint64_t &um, int kk,
+                                         int32_t c0, uint32_t &pm, const Mod &md, float pinv) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t p = md.p;
+    uint32_t v[NCW], cc[NCW];
+    uint32_t dd = 1u;
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < NCW; ++i) {
+        const int j = wv + SEL_W * i;
+        v[i] = cb + j < bw ? sh.X[lane][cb + j] : 0u;
+        cc[i] = 0u;
+        any |= v[i] != 0u;
+    }
+    pm = 0;  // sub-panel columns with a pivot (uniform)
+    // no unselected tile row is nonzero on the sub-panel: no pivot in it
+    const uint64_t nzb = um & (uint64_t)__ballot(any);
+    if (lane == 0) sh.nzw[wv] = nzb != 0;
+    __syncthreads();
+    int anyz = 0;
+#pragma unroll
+    for (int w = 0; w < SEL_W; ++w) anyz |= sh.nzw[w];
+    if (!__builtin_amdgcn_readfirstlane(anyz)) return 0;
+    int np = 0;
+#pragma unroll
+    for (int c = 0; c < SW; ++c) {
+        const int own = c % SEL_W, ic = c / SEL_W, buf = c & 1;
+        if (wv == own) {
+            // columns past the panel hold zeros: no pivot there
+            const uint64_t ball = um & (uint64_t)__ballot(v[ic] != 0u);
+            const int is = ball ? __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)ball) - 1) : -1;
+            sh.f[buf][lane] = v[ic];
+            if (lane == 0) {
+                sh.sis[buf] = is;
+                sh.sav[buf] = is >= 0 ? (uint32_t)__builtin_amdgcn_readlane((int)v[ic], is) : 0u;
+            }
+        }
+        __syncthreads();
+        const int is = __builtin_amdgcn_readfirstlane(sh.sis[buf]);
+        if (is < 0) continue;
+        const uint32_t av = (uint32_t)__builtin_amdgcn_readfirstlane((int)sh.sav[buf]);
+        const uint32_t f = sh.f[buf][lane];
+        const uint32_t dp = (uint32_t)__builtin_amdgcn_readlane((int)dd, is);
+        const bool me = lane == is;
+        const uint32_t nf = p - f;
+#pragma unroll
+        for (int i = 0; i < NCW; ++i) {
+            const int j = wv + SEL_W * i;  // this wave's column (uniform)
+            if (j > c) {
+                const uint32_t pv = (uint32_t)__builtin_amdgcn_readlane((int)v[i], is);
+                const uint32_t nv = sel_mad<P16>(av, v[i], nf, pv, md, pinv);
+                v[i] = me ? v[i] : nv;
+            } else if (j < c) {
+                if ((pm >> j) & 1u) {
+                    const uint32_t pc = (uint32_t)__builtin_amdgcn_readlane((int)cc[i], is);
+                    const uint32_t nc = sel_mad<P16>(av, cc[i], nf, pc, md, pinv);
+                    cc[i] = me ? cc[i] : nc;
+                }
+            } else {
+                const uint32_t ncc = sel_mad<P16>(dp, nf, 0u, 0u, md, pinv);
+                cc[i] = me ? 0u : ncc;
+            }
+        }
+        const uint32_t ndd = sel_mad<P16>(av, dd, 0u, 0u, md, pinv);
+        dd = me ? dd : ndd;
+        if (wv == own && lane == 0) {
+            sh.qs[c] = is;
+            sh.ecol[kk + np] = c0 + cb + c;
+            sh.eslot[kk + np] = is;
+        }
+        pm |= 1u << c;
+        um &= ~(1ull << is);
+        ++np;
+    }
+#pragma unroll
+    for (int i = 0; i < NCW; ++i) {
+        const int j = wv + SEL_W * i;
+        sh.C[lane][j] = cc[i];
+        if (lane == 0 && !((pm >> j) & 1u)) sh.qs[j] = 0;  // no pivot: coefficient zero
+    }
+    if (wv == 0) sh.D[lane] = dd;
+    return np;
+}
 
 // P16: 1 for p < 2^16, 0 for larger p, 2 to decide at run time
 template <int P16>
-__device__ __forceinline__ void select_body(const uint32_t *a, int64_t ld, int32_t n, int32_t c0,
-                                            int32_t *cand, int32_t *rowpiv, int32_t *colpiv,
-                                            Panel *pan, ListState *st, Mod md) {
-    __shared__ uint32_t X[PB][XW];
-    __shared__ uint32_t G[PB][PB + 1];
-    __shared__ uint32_t einv[PB];
-    __shared__ int32_t ecol[PB], eslot[PB], erow[PB], epos[PB], srow[PB], spos[PB];
-    __shared__ int32_t sh_js, sh_is;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    // this thread's column and row group (the same in the whole wave)
-    const int col = tid & (2 * PB - 1), rg = __builtin_amdgcn_readfirstlane(tid >> 7);
+__device__ __forceinline__ void select_body(SelShared &sh, const uint32_t *a, int64_t ld,
+                                            int32_t n, int32_t c0, int32_t *cand,
+                                            int32_t *rowpiv, int32_t *colpiv, Panel *pan,
+                                            ListState *st, Mod md) {
+    if constexpr (P16 == 2) {
+        if (md.p < (1u << 16)) select_body<1>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md);
+        else select_body<0>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md);
+        return;
+    } else {
+    const int tid = threadIdx.x, wv = tid >> 6;
     const uint32_t p = md.p;
-    const bool small = p < (1u << 17);  // 64 products of residues fit in 64 bits
-    const bool p16 = P16 == 2 ? p < (1u << 16) : P16 == 1;  // mad2_mod16 applies
+    const bool small = p < (1u << 17);  // 65 products of residues fit in 64 bits
     const float pinv = 1.0f / (float)p;
+    uint32_t r64 = md.reduce(~0ull) + 1u;  // 2^64 mod p
+    if (r64 == p) r64 = 0;
     const int32_t done = __builtin_amdgcn_readfirstlane(st->done);
     if (done) {
         if (tid == 0) pan->k = 0;
@@ -157,7 +385,8 @@ __device__ __forceinline__ void select_body(const uint32_t *a, int64_t ld, int32
     int nzero = 0;
     int32_t pos = head;
     uint64_t pmask = 0;  // slots holding E rows (same value in every thread)
-    for (int s = rg; s < PB; s += SEL_RG) X[s][col] = 0u;
+    uint64_t epm = 0;    // panel columns of E's pivots (relative to c0)
+    for (int i = tid; i < PB * XW; i += SEL_T) (&sh.X[0][0])[i] = 0u;
     while (pos < count && kk < bw) {
         const int k0 = kk;  // pivots found before this tile
         const uint64_t freem = ~pmask;
@@ -175,164 +404,140 @@ __device__ __forceinline__ void select_body(const uint32_t *a, int64_t ld, int32
         }
         if (tid < PB && ((tmask >> tid) & 1ull)) {
             const int rk = __popcll(freem & ((1ull << tid) - 1ull));
-            srow[tid] = cand[pos + rk];
-            spos[tid] = pos + rk;
+            sh.srow[tid] = cand[pos + rk];
+            sh.spos[tid] = pos + rk;
         }
         __syncthreads();
-        // 1. load the tile: panel entries, unit transform
-        for (int s = rg; s < PB; s += SEL_RG) {
-            if (!((tmask >> s) & 1ull)) continue;
-            uint32_t v;
-            if (col < PB) v = col < bw ? a[(int64_t)srow[s] * ld + c0 + col] : 0u;
-            else v = (col - PB == s) ? 1u : 0u;
-            X[s][col] = v;
-        }
-        __syncthreads();
-        // 2. reduce the tile by E (normalised)
-        if (kk > 0) {
-            for (int idx = tid; idx < PB * PB; idx += SEL_T) {
-                const int s = idx >> 6, t = idx & 63;
-                G[s][t] = (t < kk && ((tmask >> s) & 1ull)) ? X[s][ecol[t] - c0] : 0u;
-            }
-            __syncthreads();
-            for (int s = rg; s < PB; s += SEL_RG) {
-                if (!((tmask >> s) & 1ull)) continue;
-                uint64_t acc = 0;
-                if (small) {
-                    for (int t = 0; t < kk; ++t) acc += (uint64_t)G[s][t] * X[eslot[t]][col];
-                } else {
-                    for (int t = 0; t < kk; ++t) acc += (uint64_t)md.mul(G[s][t], X[eslot[t]][col]);
-                }
-                X[s][col] = sub_mod(X[s][col], md.reduce(acc), p);
-            }
-            __syncthreads();
-        }
-        // 3. pivots of the tile, leftmost column first.  Wave 0 keeps, per
-        // lane s, the first nonzero panel column of tile slot s (PB: none, or
-        // not an unselected tile row); columns left of the last pivot are
-        // zero in every unselected row, so only rows hit by an elimination
-        // rescan.
-        int fc = PB;
-        if (wv == 0) {
-            if ((tmask >> lane) & 1ull) {
-                for (int c = 0; c < bw; ++c)
-                    if (X[lane][c] != 0u) { fc = c; break; }
-            }
-            pivot_search(fc, sh_js, sh_is);
-        }
-        __syncthreads();
-        bool exhausted = false;
-        for (;;) {
-            const int js = __builtin_amdgcn_readfirstlane(sh_js);
-            if (js >= bw) {
-                exhausted = true;
-                break;
-            }
-            const int is = __builtin_amdgcn_readfirstlane(sh_is);
-            const uint32_t av = X[is][js];
-            const uint32_t pv = X[is][col];
-            // x <- av x - x[js] * pivot row in every slot but the pivot's,
-            // whatever it holds (the scaling by av is harmless: E rows are
-            // normalised at the end of the tile, tile rows are either
-            // selected and normalised or end up zero, free slots are
-            // reloaded).  Column js comes out zero; it is written in the
-            // next phase, so every thread reads its x[js] before that.
-            uint32_t fv[SEL_RS], xv[SEL_RS];
+        // 1. load the tile: panel entries (a thread's loads all in flight at
+        // once, a row's 64 columns by one wave), unit transform
+        {
+            constexpr int NL = PB * PB / SEL_T;
+            const int col = tid & (PB - 1), s0 = wv;
+            uint32_t vals[NL];
 #pragma unroll
-            for (int i = 0; i < SEL_RS; ++i) {
-                const int r = rg + SEL_RG * i;
-                fv[i] = X[r][js];
-                xv[i] = X[r][col];
+            for (int i = 0; i < NL; ++i) {
+                const int s = s0 + i * SEL_W;
+                vals[i] = (((tmask >> s) & 1ull) && col < bw)
+                              ? a[(int64_t)sh.srow[s] * ld + c0 + col] : 0u;
             }
-            if (col != js) {
 #pragma unroll
-                for (int i = 0; i < SEL_RS; ++i) {
-                    const int r = rg + SEL_RG * i;
-                    if (r == is) continue;
-                    const uint32_t nf = p - fv[i];
-                    // two products below 2^62 each: the sum fits in 64 bits
-                    X[r][col] = p16 ? mad2_mod16(av, xv[i], nf, pv, p, pinv)
-                                    : md.reduce((uint64_t)av * xv[i] + (uint64_t)nf * pv);
+            for (int i = 0; i < NL; ++i) {
+                const int s = s0 + i * SEL_W;
+                if ((tmask >> s) & 1ull) {
+                    sh.X[s][col] = vals[i];
+                    sh.X[s][PB + col] = col == s ? 1u : 0u;
                 }
             }
-            __syncthreads();
-            // the pivot row joins E; column js is cleared in the other rows;
-            // wave 0 finds the next pivot (rows hit by this elimination
-            // rescan from js + 1, which this phase does not write)
-            if (tid < PB && tid != is) X[tid][js] = 0u;
-            if (tid == 0) {
-                ecol[kk] = c0 + js;
-                eslot[kk] = is;
-            }
-            pmask |= 1ull << is;
-            tmask &= ~(1ull << is);
-            ++kk;
-            if (wv == 0) {
-                if (lane == is) fc = PB;
-                else if (fc == js) {
-                    fc = PB;
-                    for (int c = js + 1; c < bw; ++c)
-                        if (X[lane][c] != 0u) { fc = c; break; }
-                }
-                pivot_search(fc, sh_js, sh_is);
-            }
-            __syncthreads();
-            if (kk == bw) break;
         }
+        __syncthreads();
+        // 2. reduce the tile by E (normalised): row s -= sum_t X[s][ecol t] E_t.
+        // Columns of E's pivots become zero; the other panel columns now,
+        // the transform part once a chain of this tile has found a pivot
+        // (a tile without pivots never needs it).
+        const int ke = kk;
+        bool trpend = false;
+        const uint64_t bwm = bw == PB ? ~0ull : ((1ull << bw) - 1ull);
+        if (ke > 0) {
+            for (int i = tid; i < PB * PB; i += SEL_T) {
+                const int s = i >> 6, t = i & 63;
+                const uint32_t g = (t < ke && ((tmask >> s) & 1ull)) ? sh.X[s][sh.ecol[t] - c0] : 0u;
+                sh.CE[s][t] = g ? p - g : 0u;
+            }
+            if (tid < PB) sh.qse[tid] = tid < ke ? sh.eslot[tid] : 0;
+            __syncthreads();
+            if (P16 == 1) sel_apply<PB, 0>(sh, sh.CE, sh.qse, true, ke, bwm & ~epm, false, md, r64);
+            else if (small) sel_apply<PB, 1>(sh, sh.CE, sh.qse, true, ke, bwm & ~epm, false, md, r64);
+            else sel_apply<PB, 2>(sh, sh.CE, sh.qse, true, ke, bwm & ~epm, false, md, r64);
+            for (int i = tid; i < PB * PB; i += SEL_T) {
+                const int s = i >> 6, t = i & 63;
+                if (t < ke && ((tmask >> s) & 1ull)) sh.X[s][sh.ecol[t] - c0] = 0u;
+            }
+            __syncthreads();
+            trpend = true;
+        }
+        // 3. the pivots of the tile, sub-panel by sub-panel
+        uint64_t um = tmask;
+        for (int cb = 0; cb < bw && kk < bw; cb += SW) {
+            uint32_t pm;
+            const int np = sel_chain<P16>(sh, cb, bw, um, kk, c0, pm, md, pinv);
+            __syncthreads();
+            if (np > 0) {
+                if (trpend) {
+                    if (P16 == 1) sel_apply<PB, 0>(sh, sh.CE, sh.qse, true, ke, 0, true, md, r64);
+                    else if (small) sel_apply<PB, 1>(sh, sh.CE, sh.qse, true, ke, 0, true, md, r64);
+                    else sel_apply<PB, 2>(sh, sh.CE, sh.qse, true, ke, 0, true, md, r64);
+                    __syncthreads();
+                    trpend = false;
+                }
+                kk += np;
+                epm |= (uint64_t)pm << cb;
+                if (P16 == 1) sel_apply<SW, 0>(sh, sh.C, sh.qs, false, SW, bwm, true, md, r64);
+                else if (small) sel_apply<SW, 1>(sh, sh.C, sh.qs, false, SW, bwm, true, md, r64);
+                else sel_apply<SW, 2>(sh, sh.C, sh.qs, false, SW, bwm, true, md, r64);
+                __syncthreads();
+            }
+        }
+        pmask |= tmask & ~um;
         // 4. normalise E by its pivot values (inverses in parallel)
         if (kk > k0) {
-            if (tid < kk) einv[tid] = inv_mod(X[eslot[tid]][ecol[tid] - c0], p);
+            if (tid < kk) sh.einv[tid] = inv_mod(sh.X[sh.eslot[tid]][sh.ecol[tid] - c0], p);
             __syncthreads();
-            for (int t = rg; t < kk; t += SEL_RG) {
-                const int s = eslot[t];
-                X[s][col] = md.mul(X[s][col], einv[t]);
+            for (int i = tid; i < kk * 2 * PB; i += SEL_T) {
+                const int t = i >> 7, col = i & (2 * PB - 1);
+                const int s = sh.eslot[t];
+                sh.X[s][col] = md.mul(sh.X[s][col], sh.einv[t]);
             }
             __syncthreads();
         }
-        if (exhausted) {
-            // rows of this tile that were not selected are zero on the panel
-            nzero += __popcll(tmask);
-        }
+        // the sweep went over every column: the rows of this tile that were
+        // not selected are zero on the panel
+        if (kk < bw) nzero += __popcll(um);
         pos += nc;
     }
     __syncthreads();
     if (tid < kk) {
-        const int s = eslot[tid];
-        const int32_t r = srow[s];
-        erow[tid] = r;
-        epos[tid] = spos[s];
-        rowpiv[r] = ecol[tid];
-        colpiv[ecol[tid]] = r;
+        const int s = sh.eslot[tid];
+        const int32_t r = sh.srow[s];
+        sh.erow[tid] = r;
+        sh.epos[tid] = sh.spos[s];
+        rowpiv[r] = sh.ecol[tid];
+        colpiv[sh.ecol[tid]] = r;
         pan->erow[tid] = r;
-        pan->ecol[tid] = ecol[tid];
+        pan->ecol[tid] = sh.ecol[tid];
     }
     for (int i = tid; i < PB * PB; i += SEL_T) {
         const int t = i >> 6, sl = i & 63;
-        pan->T[i] = (t < kk && sl < kk) ? X[eslot[t]][PB + eslot[sl]] : 0u;
+        pan->T[i] = (t < kk && sl < kk) ? sh.X[sh.eslot[t]][PB + sh.eslot[sl]] : 0u;
+    }
+    // move the selected rows to cand[head, head + kk): the j-th selected row
+    // further back takes the j-th unselected row of the front, all at once
+    if (tid < PB) sh.fr[tid] = 0;
+    __syncthreads();
+    const int lane = tid & 63;
+    const int dpos = tid < kk ? sh.epos[tid] - head : PB;
+    if (dpos < kk) sh.fr[dpos] = 1;
+    __syncthreads();
+    if (wv == 0) {
+        const uint64_t lt = (1ull << lane) - 1ull;
+        const bool freef = lane < kk && !sh.fr[lane];
+        const uint64_t bf = (uint64_t)__ballot(freef);
+        if (freef) sh.fpos[__popcll(bf & lt)] = lane;
+        const bool back = lane < kk && dpos >= kk;
+        const uint64_t bb = (uint64_t)__ballot(back);
+        sh.mvj[lane] = __popcll(bb & lt);
     }
     __syncthreads();
+    const bool back = tid < kk && dpos >= kk;
+    if (back) sh.mvv[tid] = cand[head + sh.fpos[sh.mvj[tid]]];
+    __syncthreads();  // every read of the front is done before it is written
+    if (back) cand[head + dpos] = sh.mvv[tid];
+    if (tid < kk) cand[head + tid] = sh.erow[tid];
     if (tid == 0) {
-        // move the selected rows to cand[head, head + kk): an unselected row
-        // from the front takes the place of each selected row further back
-        uint64_t front = 0;
-        for (int t = 0; t < kk; ++t) {
-            const int d = epos[t] - head;
-            if (d < kk) front |= 1ull << d;
-        }
-        int u = 0;
-        for (int t = 0; t < kk; ++t) {
-            const int d = epos[t] - head;
-            if (d >= kk) {
-                while ((front >> u) & 1ull) ++u;
-                cand[epos[t]] = cand[head + u];
-                ++u;
-            }
-        }
-        for (int t = 0; t < kk; ++t) cand[head + t] = erow[t];
         pan->k = kk;
         st->head = head + kk;
         st->nzero += nzero;
         if (head + kk >= count) st->done = 1;
+    }
     }
 }
 
@@ -341,7 +546,8 @@ __global__ __launch_bounds__(SEL_T) void k_panel_select(const uint32_t *a, int64
                                                         int32_t c0, int32_t *cand,
                                                         int32_t *rowpiv, int32_t *colpiv,
                                                         Panel *pan, ListState *st, Mod md) {
-    select_body<P16>(a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md);
+    __shared__ SelShared sh;
+    select_body<P16>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md);
 }
 
 // ---------------------------------------------------------------- prune
@@ -1263,9 +1469,10 @@ __global__ __launch_bounds__(256) void k_batch_dense_init(const DenseDesc *D) {
 }
 
 __global__ __launch_bounds__(SEL_T) void k_batch_select(const DenseDesc *D, int32_t c0) {
+    __shared__ SelShared sh;
     const DenseDesc &d = D[blockIdx.x];
     if (c0 >= d.n) return;
-    select_body<2>(d.a, d.ld, d.n, c0, d.cand, d.rowpiv, d.colpiv, d.pan, d.st, d.md);
+    select_body<2>(sh, d.a, d.ld, d.n, c0, d.cand, d.rowpiv, d.colpiv, d.pan, d.st, d.md);
 }
 
 __global__ __launch_bounds__(256) void k_batch_rp(const DenseDesc *D, const int32_t *pre,
