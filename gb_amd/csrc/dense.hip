@@ -152,6 +152,11 @@ struct SelShared {
     uint32_t sav[2];
     // the move of the selected rows to the front of the candidate list
     int32_t fr[PB], fpos[PB], mvj[PB], mvv[PB];
+    // signed base-256 digit planes of a sub-panel's combination (slot x
+    // term) and of the pivot rows it takes in (column x term), for the
+    // products on the matrix cores
+    int8_t Cd[4][PB][SW];
+    int8_t Pd[4][2 * PB][SW];
 };
 
 // X[s][col] <- D[s] X[s][col] + sum_{q < nq} Cm[s][q] X[qm[q]][col] for every
@@ -255,6 +260,101 @@ __device__ __forceinline__ void sel_apply(SelShared &sh, const uint32_t (*Cm)[PB
     }
     if (tr)
         for (int col = PB + wv; col < 2 * PB; col += 2 * SEL_W) two(col, col + SEL_W);
+}
+
+// the four signed base-256 digits of x's representative (x itself when
+// x <= hi = 127 (1 + 256 + ...), else x - p), digit i in byte i
+__device__ __forceinline__ uint32_t digits4(uint32_t x, uint32_t p, uint32_t hi) {
+    int32_t v = x <= hi ? (int32_t)x : (int32_t)x - (int32_t)p;
+    uint32_t w = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int8_t d = (int8_t)(v & 0xFF);
+        w |= (uint32_t)(uint8_t)d << (8 * k);
+        v = (v - d) >> 8;
+    }
+    return w;
+}
+
+// The sub-panel apply on the int8 matrix cores: X <- diag(D) X + C P with
+// P = X[qs[0..SW)] (SW x 2 PB), every column of X (the panel columns past
+// bw are zero in every slot).  C and P go into ND signed digits each; the
+// products of digit sum t = i + j are one v_mfma_i32_16x16x64_i8 with the
+// pairs (i, j) in its four 16-term K groups, below 4 * 16 * 128^2 = 2^20 in
+// size.  Wave w: slots 16 (w % 4) .., columns 64 (w / 4) .. (four 16 x 16
+// tiles).  Every output has one writer, which reads its old value only.
+template <int ND>
+__device__ __forceinline__ void sel_apply_mfma(SelShared &sh, const Mod &md, const Digits &dg) {
+    static_assert(SEL_W == 8, "four row blocks times two column halves");
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t p = md.p;
+    {
+        // P's digits: thread = (column, four terms); C's: (slot, four terms)
+        const int col = tid >> 2, k = tid & 3;
+        uint32_t w[ND] = {};
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t d4 = digits4(sh.X[sh.qs[4 * k + b]][col], p, dg.hi);
+#pragma unroll
+            for (int i = 0; i < ND; ++i) w[i] |= ((d4 >> (8 * i)) & 0xFFu) << (8 * b);
+        }
+#pragma unroll
+        for (int i = 0; i < ND; ++i) *(uint32_t *)&sh.Pd[i][col][4 * k] = w[i];
+        if (tid < PB * 4) {
+            const int s = tid >> 2;
+            uint32_t wc[ND] = {};
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t d4 = digits4(sh.C[s][4 * k + b], p, dg.hi);
+#pragma unroll
+                for (int i = 0; i < ND; ++i) wc[i] |= ((d4 >> (8 * i)) & 0xFFu) << (8 * b);
+            }
+#pragma unroll
+            for (int i = 0; i < ND; ++i) *(uint32_t *)&sh.Cd[i][s][4 * k] = wc[i];
+        }
+    }
+    __syncthreads();
+    const int rb = wv & 3, lr = lane & 15, g = lane >> 4;
+    // this lane's K group g of digit sum t: the pair (i, t - i), i = lowest + g
+    v4i A[2 * ND - 1];
+    int jt[2 * ND - 1];
+#pragma unroll
+    for (int t = 0; t < 2 * ND - 1; ++t) {
+        const int ilo = t - (ND - 1) > 0 ? t - (ND - 1) : 0, ihi = t < ND - 1 ? t : ND - 1;
+        const int i = ilo + g;
+        const bool ok = i <= ihi;
+        jt[t] = ok ? t - i : -1;
+        A[t] = ok ? *(const v4i *)&sh.Cd[ok ? i : 0][rb * 16 + lr][0] : v4i{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int cbi = 0; cbi < 4; ++cbi) {
+        const int cb16 = (wv >> 2) * 4 + cbi;
+        v4i acc[2 * ND - 1];
+#pragma unroll
+        for (int t = 0; t < 2 * ND - 1; ++t) {
+            const v4i B = jt[t] >= 0 ? *(const v4i *)&sh.Pd[jt[t] >= 0 ? jt[t] : 0][cb16 * 16 + lr][0]
+                                     : v4i{0, 0, 0, 0};
+            acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[t], B, v4i{0, 0, 0, 0}, 0, 0, 0);
+        }
+        // result register e of lane l: slot 16 rb + 4 (l / 16) + e, column l % 16
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int s = rb * 16 + 4 * g + e, col = cb16 * 16 + lr;
+            const uint32_t dx = md.mul(sh.D[s], sh.X[s][col]);
+            uint32_t r;
+            if constexpr (ND == 2) {
+                const int32_t ip = (int32_t)p;
+                int32_t v = (int32_t)dx + digits_balanced2(acc, e, ip, dg.pinvf);
+                v += v < 0 ? ip : 0;
+                v -= v >= ip ? ip : 0;
+                r = (uint32_t)v;
+            } else {
+                r = dx + digits_combine<ND>(acc, e, md, dg);
+                r = r >= p ? r - p : r;
+            }
+            sh.X[s][col] = r;
+        }
+    }
 }
 
 // The pivot chain over the sub-panel [cb, cb + SW) of the tile rows um,
@@ -361,10 +461,10 @@ template <int P16>
 __device__ __forceinline__ void select_body(SelShared &sh, const uint32_t *a, int64_t ld,
                                             int32_t n, int32_t c0, int32_t *cand,
                                             int32_t *rowpiv, int32_t *colpiv, Panel *pan,
-                                            ListState *st, Mod md) {
+                                            ListState *st, Mod md, const Digits &dg) {
     if constexpr (P16 == 2) {
-        if (md.p < (1u << 16)) select_body<1>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md);
-        else select_body<0>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md);
+        if (md.p < (1u << 16)) select_body<1>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg);
+        else select_body<0>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg);
         return;
     } else {
     const int tid = threadIdx.x, wv = tid >> 6;
@@ -471,9 +571,18 @@ __device__ __forceinline__ void select_body(SelShared &sh, const uint32_t *a, in
                 }
                 kk += np;
                 epm |= (uint64_t)pm << cb;
-                if (P16 == 1) sel_apply<SW, 0>(sh, sh.C, sh.qs, false, SW, bwm, true, md, r64);
-                else if (small) sel_apply<SW, 1>(sh, sh.C, sh.qs, false, SW, bwm, true, md, r64);
-                else sel_apply<SW, 2>(sh, sh.C, sh.qs, false, SW, bwm, true, md, r64);
+                // on the matrix cores in the MFMA form (digits given), on
+                // the integer ALU in the plain form
+                switch (dg.nd) {
+                    case 1: sel_apply_mfma<1>(sh, md, dg); break;
+                    case 2: sel_apply_mfma<2>(sh, md, dg); break;
+                    case 3: sel_apply_mfma<3>(sh, md, dg); break;
+                    case 4: sel_apply_mfma<4>(sh, md, dg); break;
+                    default:
+                        if (P16 == 1) sel_apply<SW, 0>(sh, sh.C, sh.qs, false, SW, bwm, true, md, r64);
+                        else if (small) sel_apply<SW, 1>(sh, sh.C, sh.qs, false, SW, bwm, true, md, r64);
+                        else sel_apply<SW, 2>(sh, sh.C, sh.qs, false, SW, bwm, true, md, r64);
+                }
                 __syncthreads();
             }
         }
@@ -545,9 +654,10 @@ template <int P16>
 __global__ __launch_bounds__(SEL_T) void k_panel_select(const uint32_t *a, int64_t ld, int32_t n,
                                                         int32_t c0, int32_t *cand,
                                                         int32_t *rowpiv, int32_t *colpiv,
-                                                        Panel *pan, ListState *st, Mod md) {
+                                                        Panel *pan, ListState *st, Mod md,
+                                                        Digits dg) {
     __shared__ SelShared sh;
-    select_body<P16>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md);
+    select_body<P16>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg);
 }
 
 // ---------------------------------------------------------------- prune
@@ -699,65 +809,101 @@ __device__ __forceinline__ void rp_body(uint32_t *a, int64_t ld, int32_t n, int3
 }
 
 // The same product over the columns [jlo, jhi) (jlo >= c0) in narrower
-// blocks: a block covers RPC columns and every row t (a thread: one column,
-// four rows), so the blocks that read a column of the selected rows are the
-// ones that write it, and a launch over a few columns still has PB / 4
-// threads per column.
-constexpr int RPC = 16;
-
+// blocks: a block covers RC columns and every row t (a thread: one column,
+// RC / 4 rows), so the blocks that read a column of the selected rows are
+// the ones that write it, and a launch over a few columns still spreads
+// over many blocks.  KIND 0 (p < 2^16): products below 2^32 from 24-bit
+// multiplies, summed in 64 bits; otherwise 64-bit products with a carry.
+template <int RC, int KIND>
 __global__ __launch_bounds__(256) void k_panel_rp(uint32_t *a, int64_t ld, int32_t c0, int32_t jlo,
                                                   int32_t jhi, const Panel *pan, uint32_t *rp,
                                                   int64_t ldr, Mod md, uint32_t r64, Digits dg) {
+    constexpr int NR = RC / 4;  // rows per thread
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
     __shared__ uint32_t T[PB][PB + 1];
-    __shared__ uint32_t V[PB][RPC + 1];
+    __shared__ uint32_t V[PB][RC + 1];
     const int tid = threadIdx.x;
-    const int32_t jb = jlo + (int32_t)blockIdx.x * RPC;
+    const int32_t jb = jlo + (int32_t)blockIdx.x * RC;
     for (int i = tid; i < PB * PB; i += 256) T[i >> 6][i & 63] = pan->T[i];
-    for (int i = tid; i < PB * RPC; i += 256) {
-        const int s_ = i / RPC, c = i % RPC;
+    for (int i = tid; i < PB * RC; i += 256) {
+        const int s_ = i / RC, c = i % RC;
         V[s_][c] = (s_ < kk && jb + c < jhi) ? a[(int64_t)pan->erow[s_] * ld + jb + c] : 0u;
     }
     __syncthreads();
-    const int jj = tid % RPC, tg = tid / RPC;  // column, rows 4 tg .. 4 tg + 3
+    const int jj = tid % RC, tg = tid / RC;  // column, rows NR tg .. NR tg + NR - 1
     const int32_t j = jb + jj;
-    uint32_t res[4];
+    uint32_t res[NR];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int t = 4 * tg + q;
+    for (int q = 0; q < NR; ++q) {
+        const int t = NR * tg + q;
         uint64_t lo = 0;
         uint32_t hi = 0;
         for (int s_ = 0; s_ < kk; ++s_) {
-            const uint64_t pr = (uint64_t)T[t][s_] * (uint64_t)V[s_][jj];
-            const uint64_t o = lo;
-            lo += pr;
-            hi += lo < o;
+            if (KIND == 0) {
+                // __umul24 returns an int: the low 32 bits go through uint32_t
+                lo += (uint64_t)(uint32_t)__umul24(T[t][s_], V[s_][jj]);
+            } else {
+                const uint64_t pr = (uint64_t)T[t][s_] * (uint64_t)V[s_][jj];
+                const uint64_t o = lo;
+                lo += pr;
+                hi += lo < o;
+            }
         }
         uint32_t r = md.reduce(lo);
-        if (hi) r = (uint32_t)((r + (uint64_t)md.mul(hi, r64)) % md.p);
+        if (KIND != 0 && hi) r = (uint32_t)((r + (uint64_t)md.mul(hi, r64)) % md.p);
         res[q] = r;
     }
     __syncthreads();  // all reads of the selected rows are done before any write
     if (j >= jhi) return;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int t = 4 * tg + q;
+    for (int q = 0; q < NR; ++q) {
+        const int t = NR * tg + q;
         if (t < kk) {
             rp[(int64_t)t * ldr + (j - c0)] = res[q];
             a[(int64_t)pan->erow[t] * ld + j] = res[q];
         }
     }
-    // digits of Rp^T: column j - c0, rows 4 tg .. 4 tg + 3 (zero beyond k),
-    // one 4-byte store per digit
+    // digits of Rp^T: column j - c0, rows NR tg .. NR tg + NR - 1 (zero
+    // beyond k): one 4-byte store per digit for 4 rows, else byte stores
     for (int i = 0; i < dg.nd; ++i) {
-        uint32_t v = 0;
+        int8_t *dst = dg.R + i * dg.rplane + (int64_t)(j - c0) * 64 + NR * tg;
+        if constexpr (NR % 4 == 0) {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const uint32_t r = (4 * tg + b) < kk ? res[b] : 0u;
-            v |= digit_word(r, md.p, dg.hi, i) << (8 * b);
+            for (int b4 = 0; b4 < NR; b4 += 4) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const uint32_t r = (NR * tg + b4 + b) < kk ? res[b4 + b] : 0u;
+                    v |= digit_word(r, md.p, dg.hi, i) << (8 * b);
+                }
+                *(uint32_t *)(dst + b4) = v;
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < NR; ++b) {
+                const uint32_t r = (NR * tg + b) < kk ? res[b] : 0u;
+                dst[b] = (int8_t)(uint8_t)digit_word(r, md.p, dg.hi, i);
+            }
         }
-        *(uint32_t *)(dg.R + i * dg.rplane + (int64_t)(j - c0) * 64 + 4 * tg) = v;
+    }
+}
+
+// Rp over [jlo, jhi): RC = 4 (a block per 4 columns) for the few columns
+// the next select waits for, RC = 16 for the rest
+void launch_rp(hipStream_t s, bool narrow, uint32_t *a, int64_t ld, int32_t c0, int32_t jlo,
+               int32_t jhi, const Panel *pan, uint32_t *rp, int64_t ldr, const Mod &md,
+               uint32_t r64, const Digits &dg) {
+    if (jhi <= jlo) return;
+    const bool k0 = md.p < (1u << 16);
+    const int rc = narrow ? 4 : 16;
+    int64_t g = (jhi - jlo + rc - 1) / rc;
+    if (narrow) {
+        if (k0) k_panel_rp<4, 0><<<(unsigned)g, 256, 0, s>>>(a, ld, c0, jlo, jhi, pan, rp, ldr, md, r64, dg);
+        else k_panel_rp<4, 1><<<(unsigned)g, 256, 0, s>>>(a, ld, c0, jlo, jhi, pan, rp, ldr, md, r64, dg);
+    } else {
+        if (k0) k_panel_rp<16, 0><<<(unsigned)g, 256, 0, s>>>(a, ld, c0, jlo, jhi, pan, rp, ldr, md, r64, dg);
+        else k_panel_rp<16, 1><<<(unsigned)g, 256, 0, s>>>(a, ld, c0, jlo, jhi, pan, rp, ldr, md, r64, dg);
     }
 }
 
@@ -876,13 +1022,17 @@ template <int ND, int KC, int RS = 2, int CS = 2, typename T = uint32_t>
 __device__ __forceinline__ void update_i8_body(T *a, int64_t ld, int32_t m, int32_t cb,
                                                int32_t ce, int32_t c0, const Panel *pan,
                                                const int32_t *active, Mod md, const Digits &dg,
-                                               int32_t bx, int32_t by) {
+                                               int32_t bx, int32_t by, int32_t gb = INT_MAX,
+                                               int32_t gsz = 0) {
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // the block's tile is 32 RS x 32 CS, each of its four waves 16 RS x 16 CS
+    // the block's tile is 32 RS x 32 CS, each of its four waves 16 RS x 16 CS;
+    // block tiles from column gb on skip the gsz columns of a gap
     const int32_t r0 = by * (32 * RS) + (wv >> 1) * (16 * RS);
-    const int32_t q0 = cb + bx * (32 * CS) + (wv & 1) * (16 * CS);
+    int32_t qb = cb + bx * (32 * CS);
+    if (qb >= gb) qb += gsz;
+    const int32_t q0 = qb + (wv & 1) * (16 * CS);
     // rows of this wave with a nonzero multiplier (wave-uniform exit)
     const bool act = lane < 16 * RS && r0 + lane < m && active[r0 + lane] != 0;
     if (!__any(act) || q0 >= ce) return;
@@ -967,9 +1117,10 @@ template <int ND, int KC = 1, int RS = 2, int CS = 2, typename T = uint32_t>
 __global__ __launch_bounds__(256) void k_update_mfma(T *a, int64_t ld, int32_t m,
                                                      int32_t cb, int32_t ce, int32_t c0,
                                                      const Panel *pan, const int32_t *active,
-                                                     Mod md, Digits dg) {
+                                                     Mod md, Digits dg, int32_t gb = INT_MAX,
+                                                     int32_t gsz = 0) {
     update_i8_body<ND, KC, RS, CS, T>(a, ld, m, cb, ce, c0, pan, active, md, dg, blockIdx.x,
-                                   blockIdx.y);
+                                   blockIdx.y, gb, gsz);
 }
 
 // K = 256 (four chunks from chunk 0) over wide column ranges.  The block's
@@ -1298,15 +1449,18 @@ int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws) {
     k_dense_init<<<grid_of(m > n ? m : n, 256), 256, 0, s>>>(m, n, rowpiv, colpiv, cand, pan, st);
     const uint32_t r64 = (uint32_t)((((unsigned __int128)1) << 64) % d->p);
     // A[r][cb, ce) -= L[r] * Rp for the rows of the current panel's update
-    auto update = [&](hipStream_t s, const Panel *pp, int32_t c0, int32_t cb, int32_t ce) {
+    // (the MFMA form may skip a gap [gb, gb + gsz) inside [cb, ce), gb - cb
+    // a multiple of 64: one launch for the columns on both sides of it)
+    auto update = [&](hipStream_t s, const Panel *pp, int32_t c0, int32_t cb, int32_t ce,
+                      int32_t gb = INT_MAX, int32_t gsz = 0) {
         if (cb >= ce) return;
         if (form == GBGPU_FORM_MFMA) {
-            dim3 g((unsigned)((ce - cb + 63) / 64), (unsigned)((m + 63) / 64));
+            dim3 g((unsigned)((ce - cb - gsz + 63) / 64), (unsigned)((m + 63) / 64));
             switch (dg.nd) {
-                case 1: k_update_mfma<1><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg); break;
-                case 2: k_update_mfma<2><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg); break;
-                case 3: k_update_mfma<3><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg); break;
-                default: k_update_mfma<4><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg); break;
+                case 1: k_update_mfma<1><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz); break;
+                case 2: k_update_mfma<2><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz); break;
+                case 3: k_update_mfma<3><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz); break;
+                default: k_update_mfma<4><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz); break;
             }
         } else {
             dim3 g((unsigned)((ce - cb + 255) / 256), (unsigned)((m + RT - 1) / RT));
@@ -1326,9 +1480,9 @@ int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws) {
     hipStream_t x = ctx->side;
     auto select = [&](hipStream_t sx, int32_t cs, Panel *pp) {
         if (d->p < (1u << 16))
-            k_panel_select<1><<<1, SEL_T, 0, sx>>>(d->a, d->ld, n, cs, cand, rowpiv, colpiv, pp, st, md);
+            k_panel_select<1><<<1, SEL_T, 0, sx>>>(d->a, d->ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg);
         else
-            k_panel_select<0><<<1, SEL_T, 0, sx>>>(d->a, d->ld, n, cs, cand, rowpiv, colpiv, pp, st, md);
+            k_panel_select<0><<<1, SEL_T, 0, sx>>>(d->a, d->ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg);
     };
     if (m > 0 && n > 0)
         select(s, 0, pan);
@@ -1341,8 +1495,7 @@ int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws) {
         const int32_t c2 = (n - c1) < PB ? n : c1 + PB;
         // Rp over the columns the next select needs first; the rest after
         // that select is on its way
-        k_panel_rp<<<grid_of(c2 - c0, RPC), 256, 0, s>>>(d->a, d->ld, c0, c0, c2, cur, rp, ldr, md,
-                                                          r64, dg);
+        launch_rp(s, true, d->a, d->ld, c0, c0, c2, cur, rp, ldr, md, r64, dg);
         k_panel_gather<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, m, c0, bw, rowpiv, cur, L, active,
                                                       d->p, dg);
         if (c1 < n) {
@@ -1351,11 +1504,13 @@ int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws) {
                 hipStreamWaitEvent(x, ctx->sync_ev[0], 0) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
-            if (c2 < n)
-                k_panel_rp<<<grid_of(n - c2, RPC), 256, 0, x>>>(d->a, d->ld, c0, c2, n, cur, rp, ldr,
-                                                                 md, r64, dg);
-            update(x, cur, c0, c0, c1);
-            update(x, cur, c0, c2, n);
+            if (c2 < n) launch_rp(x, false, d->a, d->ld, c0, c2, n, cur, rp, ldr, md, r64, dg);
+            if (form == GBGPU_FORM_MFMA && c2 < n && c1 - c0 == PB) {
+                update(x, cur, c0, c0, n, c1, c2 - c1);  // [c0, c1) and [c2, n)
+            } else {
+                update(x, cur, c0, c0, c1);
+                update(x, cur, c0, c2, n);
+            }
             if (hipEventRecord(ctx->sync_ev[1], x) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
@@ -1472,7 +1627,7 @@ __global__ __launch_bounds__(SEL_T) void k_batch_select(const DenseDesc *D, int3
     __shared__ SelShared sh;
     const DenseDesc &d = D[blockIdx.x];
     if (c0 >= d.n) return;
-    select_body<2>(sh, d.a, d.ld, d.n, c0, d.cand, d.rowpiv, d.colpiv, d.pan, d.st, d.md);
+    select_body<2>(sh, d.a, d.ld, d.n, c0, d.cand, d.rowpiv, d.colpiv, d.pan, d.st, d.md, d.dg);
 }
 
 __global__ __launch_bounds__(256) void k_batch_rp(const DenseDesc *D, const int32_t *pre,
