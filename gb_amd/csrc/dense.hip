@@ -49,6 +49,7 @@ struct ListState {
     int32_t head;
     int32_t count;
     int32_t nzero;  // candidates found zero on their panel since the last prune
+    int32_t pcount; // blocks of a prune that have written their flags
 };
 
 __device__ uint32_t inv_mod(uint32_t a, uint32_t p) {
@@ -456,15 +457,16 @@ __device__ __forceinline__ int sel_chain(SelShared &sh, int cb, int bw, uint64_t
     return np;
 }
 
-// P16: 1 for p < 2^16, 0 for larger p, 2 to decide at run time
-template <int P16>
-__device__ __forceinline__ void select_body(SelShared &sh, const uint32_t *a, int64_t ld,
+// P16: 1 for p < 2^16, 0 for larger p, 2 to decide at run time; A's
+// entries are T (uint16_t for p <= 65521 in the MFMA form, else uint32_t)
+template <int P16, typename T>
+__device__ __forceinline__ void select_body(SelShared &sh, const T *a, int64_t ld,
                                             int32_t n, int32_t c0, int32_t *cand,
                                             int32_t *rowpiv, int32_t *colpiv, Panel *pan,
                                             ListState *st, Mod md, const Digits &dg) {
     if constexpr (P16 == 2) {
-        if (md.p < (1u << 16)) select_body<1>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg);
-        else select_body<0>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg);
+        if (md.p < (1u << 16)) select_body<1, T>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg);
+        else select_body<0, T>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg);
         return;
     } else {
     const int tid = threadIdx.x, wv = tid >> 6;
@@ -650,14 +652,14 @@ __device__ __forceinline__ void select_body(SelShared &sh, const uint32_t *a, in
     }
 }
 
-template <int P16>
-__global__ __launch_bounds__(SEL_T) void k_panel_select(const uint32_t *a, int64_t ld, int32_t n,
+template <int P16, typename T>
+__global__ __launch_bounds__(SEL_T) void k_panel_select(const T *a, int64_t ld, int32_t n,
                                                         int32_t c0, int32_t *cand,
                                                         int32_t *rowpiv, int32_t *colpiv,
                                                         Panel *pan, ListState *st, Mod md,
                                                         Digits dg) {
     __shared__ SelShared sh;
-    select_body<P16>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg);
+    select_body<P16, T>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg);
 }
 
 // ---------------------------------------------------------------- prune
@@ -666,7 +668,8 @@ __global__ __launch_bounds__(SEL_T) void k_panel_select(const uint32_t *a, int64
 // zero is zero for good (its columns left of c1 are zero already): when the
 // selects since the last prune met at least a tile's worth of rows that
 // were zero on their panel, such rows are dropped from the candidate list.
-__device__ __forceinline__ void prune_flags_body(const uint32_t *a, int64_t ld, int32_t n,
+template <typename T>
+__device__ __forceinline__ void prune_flags_body(const T *a, int64_t ld, int32_t n,
                                                  int32_t c1, const int32_t *cand,
                                                  const ListState *st, int32_t *alive,
                                                  int32_t first, int32_t step) {
@@ -677,7 +680,7 @@ __device__ __forceinline__ void prune_flags_body(const uint32_t *a, int64_t ld, 
     const int32_t count = __builtin_amdgcn_readfirstlane(st->count);
     const int lane = threadIdx.x & 63;
     for (int32_t w = first; head + w < count; w += step) {
-        const uint32_t *row = a + (int64_t)cand[head + w] * ld;
+        const T *row = a + (int64_t)cand[head + w] * ld;
         int nz = 0;  // uniform over the wave
         for (int32_t cb = c1; cb < n; cb += 64) {
             const int32_t c = cb + lane;
@@ -691,7 +694,7 @@ __device__ __forceinline__ void prune_flags_body(const uint32_t *a, int64_t ld, 
 }
 
 __device__ __forceinline__ void prune_compact_body(int32_t *cand, ListState *st,
-                                                   const int32_t *alive) {
+                                                   const int32_t *alive, bool coherent = false) {
     __shared__ int32_t wsum[4];
     const int32_t nzero = __builtin_amdgcn_readfirstlane(st->nzero);
     const int32_t done = __builtin_amdgcn_readfirstlane(st->done);
@@ -703,7 +706,10 @@ __device__ __forceinline__ void prune_compact_body(int32_t *cand, ListState *st,
     // stable in-place compaction: writes never pass the reads of later chunks
     for (int32_t base = head; base < count; base += 256) {
         const int32_t i = base + tid;
-        const int f = i < count ? alive[i - head] : 0;
+        const int f = i < count ? (coherent ? __hip_atomic_load(&alive[i - head], __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT)
+                                            : alive[i - head])
+                                : 0;
         const int32_t v = i < count ? cand[i] : 0;
         const int incl = wave_incl_scan(f);
         wsum[wv] = __shfl(incl, 63, 64);  // every lane stores the wave total
@@ -725,7 +731,8 @@ __device__ __forceinline__ void prune_compact_body(int32_t *cand, ListState *st,
     }
 }
 
-__global__ void k_prune_flags(const uint32_t *a, int64_t ld, int32_t n, int32_t c1,
+template <typename T>
+__global__ void k_prune_flags(const T *a, int64_t ld, int32_t n, int32_t c1,
                               const int32_t *cand, const ListState *st, int32_t *alive) {
     prune_flags_body(a, ld, n, c1, cand, st, alive,
                      blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6),
@@ -735,6 +742,27 @@ __global__ void k_prune_flags(const uint32_t *a, int64_t ld, int32_t n, int32_t 
 __global__ __launch_bounds__(256) void k_prune_compact(int32_t *cand, ListState *st,
                                                        const int32_t *alive) {
     prune_compact_body(cand, st, alive);
+}
+
+// both steps in one launch: every block writes its rows' flags, and the last
+// block to finish compacts the list (the flags read through to L2)
+template <typename T>
+__global__ __launch_bounds__(256) void k_prune(const T *a, int64_t ld, int32_t n, int32_t c1,
+                                               int32_t *cand, ListState *st, int32_t *alive) {
+    __shared__ int32_t last;
+    const int32_t nzero = __builtin_amdgcn_readfirstlane(st->nzero);
+    const int32_t done = __builtin_amdgcn_readfirstlane(st->done);
+    if (nzero < PB || done) return;  // the same in every block: st changes below, after all checked
+    prune_flags_body(a, ld, n, c1, cand, st, alive,
+                     blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), gridDim.x * (blockDim.x >> 6));
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&st->pcount, 1) == (int32_t)gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    prune_compact_body(cand, st, alive, true);
+    if (threadIdx.x == 0) st->pcount = 0;
 }
 
 // ---------------------------------------------------------------- rp
@@ -814,18 +842,18 @@ __device__ __forceinline__ void rp_body(uint32_t *a, int64_t ld, int32_t n, int3
 // the ones that write it, and a launch over a few columns still spreads
 // over many blocks.  KIND 0 (p < 2^16): products below 2^32 from 24-bit
 // multiplies, summed in 64 bits; otherwise 64-bit products with a carry.
-template <int RC, int KIND>
-__global__ __launch_bounds__(256) void k_panel_rp(uint32_t *a, int64_t ld, int32_t c0, int32_t jlo,
+template <int RC, int KIND, typename T>
+__global__ __launch_bounds__(256) void k_panel_rp(T *a, int64_t ld, int32_t c0, int32_t jlo,
                                                   int32_t jhi, const Panel *pan, uint32_t *rp,
                                                   int64_t ldr, Mod md, uint32_t r64, Digits dg) {
     constexpr int NR = RC / 4;  // rows per thread
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
-    __shared__ uint32_t T[PB][PB + 1];
+    __shared__ uint32_t Ts[PB][PB + 1];
     __shared__ uint32_t V[PB][RC + 1];
     const int tid = threadIdx.x;
     const int32_t jb = jlo + (int32_t)blockIdx.x * RC;
-    for (int i = tid; i < PB * PB; i += 256) T[i >> 6][i & 63] = pan->T[i];
+    for (int i = tid; i < PB * PB; i += 256) Ts[i >> 6][i & 63] = pan->T[i];
     for (int i = tid; i < PB * RC; i += 256) {
         const int s_ = i / RC, c = i % RC;
         V[s_][c] = (s_ < kk && jb + c < jhi) ? a[(int64_t)pan->erow[s_] * ld + jb + c] : 0u;
@@ -842,9 +870,9 @@ __global__ __launch_bounds__(256) void k_panel_rp(uint32_t *a, int64_t ld, int32
         for (int s_ = 0; s_ < kk; ++s_) {
             if (KIND == 0) {
                 // __umul24 returns an int: the low 32 bits go through uint32_t
-                lo += (uint64_t)(uint32_t)__umul24(T[t][s_], V[s_][jj]);
+                lo += (uint64_t)(uint32_t)__umul24(Ts[t][s_], V[s_][jj]);
             } else {
-                const uint64_t pr = (uint64_t)T[t][s_] * (uint64_t)V[s_][jj];
+                const uint64_t pr = (uint64_t)Ts[t][s_] * (uint64_t)V[s_][jj];
                 const uint64_t o = lo;
                 lo += pr;
                 hi += lo < o;
@@ -861,7 +889,7 @@ __global__ __launch_bounds__(256) void k_panel_rp(uint32_t *a, int64_t ld, int32
         const int t = NR * tg + q;
         if (t < kk) {
             rp[(int64_t)t * ldr + (j - c0)] = res[q];
-            a[(int64_t)pan->erow[t] * ld + j] = res[q];
+            a[(int64_t)pan->erow[t] * ld + j] = (T)res[q];
         }
     }
     // digits of Rp^T: column j - c0, rows NR tg .. NR tg + NR - 1 (zero
@@ -891,7 +919,8 @@ __global__ __launch_bounds__(256) void k_panel_rp(uint32_t *a, int64_t ld, int32
 
 // Rp over [jlo, jhi): RC = 4 (a block per 4 columns) for the few columns
 // the next select waits for, RC = 16 for the rest
-void launch_rp(hipStream_t s, bool narrow, uint32_t *a, int64_t ld, int32_t c0, int32_t jlo,
+template <typename T>
+void launch_rp(hipStream_t s, bool narrow, T *a, int64_t ld, int32_t c0, int32_t jlo,
                int32_t jhi, const Panel *pan, uint32_t *rp, int64_t ldr, const Mod &md,
                uint32_t r64, const Digits &dg) {
     if (jhi <= jlo) return;
@@ -899,11 +928,11 @@ void launch_rp(hipStream_t s, bool narrow, uint32_t *a, int64_t ld, int32_t c0, 
     const int rc = narrow ? 4 : 16;
     int64_t g = (jhi - jlo + rc - 1) / rc;
     if (narrow) {
-        if (k0) k_panel_rp<4, 0><<<(unsigned)g, 256, 0, s>>>(a, ld, c0, jlo, jhi, pan, rp, ldr, md, r64, dg);
-        else k_panel_rp<4, 1><<<(unsigned)g, 256, 0, s>>>(a, ld, c0, jlo, jhi, pan, rp, ldr, md, r64, dg);
+        if (k0) k_panel_rp<4, 0, T><<<(unsigned)g, 256, 0, s>>>(a, ld, c0, jlo, jhi, pan, rp, ldr, md, r64, dg);
+        else k_panel_rp<4, 1, T><<<(unsigned)g, 256, 0, s>>>(a, ld, c0, jlo, jhi, pan, rp, ldr, md, r64, dg);
     } else {
-        if (k0) k_panel_rp<16, 0><<<(unsigned)g, 256, 0, s>>>(a, ld, c0, jlo, jhi, pan, rp, ldr, md, r64, dg);
-        else k_panel_rp<16, 1><<<(unsigned)g, 256, 0, s>>>(a, ld, c0, jlo, jhi, pan, rp, ldr, md, r64, dg);
+        if (k0) k_panel_rp<16, 0, T><<<(unsigned)g, 256, 0, s>>>(a, ld, c0, jlo, jhi, pan, rp, ldr, md, r64, dg);
+        else k_panel_rp<16, 1, T><<<(unsigned)g, 256, 0, s>>>(a, ld, c0, jlo, jhi, pan, rp, ldr, md, r64, dg);
     }
 }
 
@@ -911,7 +940,8 @@ void launch_rp(hipStream_t s, bool narrow, uint32_t *a, int64_t ld, int32_t c0, 
 
 // L[r][t] = A[r][ecol[t]] for rows that are not this panel's pivots;
 // active[r] = 1 if any of them is nonzero
-__device__ __forceinline__ void gather_body(const uint32_t *a, int64_t ld, int32_t m, int32_t c0,
+template <typename T>
+__device__ __forceinline__ void gather_body(const T *a, int64_t ld, int32_t m, int32_t c0,
                                             int32_t bw, const int32_t *rowpiv, const Panel *pan,
                                             uint32_t *L, int32_t *active, int32_t first,
                                             int32_t step, uint32_t p, const Digits &dg) {
@@ -930,7 +960,8 @@ __device__ __forceinline__ void gather_body(const uint32_t *a, int64_t ld, int32
     }
 }
 
-__global__ void k_panel_gather(const uint32_t *a, int64_t ld, int32_t m, int32_t c0, int32_t bw,
+template <typename T>
+__global__ void k_panel_gather(const T *a, int64_t ld, int32_t m, int32_t c0, int32_t bw,
                                const int32_t *rowpiv, const Panel *pan, uint32_t *L,
                                int32_t *active, uint32_t p, Digits dg) {
     gather_body(a, ld, m, c0, bw, rowpiv, pan, L, active,
@@ -1133,18 +1164,24 @@ __global__ __launch_bounds__(256) void k_update_mfma(T *a, int64_t ld, int32_t m
 constexpr int WT_NT = 16;   // column tiles per block
 constexpr int WT_CS = 2;    // 16-column products per tile
 
-template <int ND, int RS, typename T = uint32_t>
+template <int ND, int RS, typename T = uint32_t, int KC = 4>
 __global__ __launch_bounds__(256) void k_update_wide(T *a, int64_t ld, int32_t m,
                                                      int32_t cb, int32_t ce, int32_t c0,
                                                      const Panel *pan, const int32_t *active,
-                                                     Mod md, Digits dg) {
+                                                     Mod md, Digits dg, int32_t gb = INT_MAX,
+                                                     int32_t gsz = 0) {
     constexpr int CS = WT_CS, TW = 16 * CS;
-    constexpr int TB = ND * 4 * TW * 64;   // bytes of Rp digits per tile
-    constexpr int PPT = TB / (16 * 256);   // 16-byte pieces per thread
+    constexpr int TB = ND * KC * TW * 64;  // bytes of Rp digits per tile
+    constexpr int NP = TB / 16;            // its 16-byte pieces
+    constexpr int PPT = (NP + 255) / 256;  // per thread (the last round partial)
     __shared__ __align__(16) int8_t rl[2][TB];
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
+    // tiles walk the columns [cb, ce) without the gap [gb, gb + gsz) (gb - cb
+    // a multiple of TW): virtual column v is column v, or v + gsz from gb on
     const int32_t qb = cb + (int32_t)blockIdx.x * (WT_NT * TW);
-    if (kk == 0 || qb >= ce) return;  // uniform over the block
+    const int32_t vce = ce - gsz;
+    auto real = [&](int32_t v) { return v >= gb ? v + gsz : v; };
+    if (kk == 0 || qb >= vce) return;  // uniform over the block
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int32_t r0 = ((int32_t)blockIdx.y * 4 + wv) * (16 * RS);
     const bool wact = __any(lane < 16 * RS && r0 + lane < m && active[r0 + lane] != 0);
@@ -1155,26 +1192,29 @@ __global__ __launch_bounds__(256) void k_update_wide(T *a, int64_t ld, int32_t m
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
             const int idx = u * 256 + tid;
-            const int ik = idx / (TW * 4), c = idx % (TW * 4);
-            const int i = ik >> 2, kc = ik & 3;
-            int32_t col = q0 + (c >> 2);
-            col = col < ce ? col : ce - 1;
-            st[u] = *(const v4i *)(dg.R + i * dg.rplane + kc * dg.rchunk + (int64_t)(col - c0) * 64 +
-                                   16 * (c & 3));
+            if (NP % 256 == 0 || idx < NP) {
+                const int ik = idx / (TW * 4), c = idx % (TW * 4);
+                const int i = ik / KC, kc = ik % KC;
+                int32_t col = q0 + (c >> 2);
+                col = col < ce ? col : ce - 1;
+                st[u] = *(const v4i *)(dg.R + i * dg.rplane + kc * dg.rchunk +
+                                       (int64_t)(col - c0) * 64 + 16 * (c & 3));
+            }
         }
     };
     auto rstore = [&](int buf, const v4i (&st)[PPT]) {
 #pragma unroll
-        for (int u = 0; u < PPT; ++u) *(v4i *)(rl[buf] + (u * 256 + tid) * 16) = st[u];
+        for (int u = 0; u < PPT; ++u)
+            if (NP % 256 == 0 || u * 256 + tid < NP) *(v4i *)(rl[buf] + (u * 256 + tid) * 16) = st[u];
     };
-    v4i Lr[4][ND][RS];
+    v4i Lr[KC][ND][RS];
     if (wact) {
 #pragma unroll
         for (int h = 0; h < RS; ++h) {
             const int32_t row = r0 + 16 * h + lr;
             const int8_t *lp = dg.L + (int64_t)(row < m ? row : m - 1) * 64 + 16 * lk;
 #pragma unroll
-            for (int kc = 0; kc < 4; ++kc)
+            for (int kc = 0; kc < KC; ++kc)
 #pragma unroll
                 for (int i = 0; i < ND; ++i)
                     Lr[kc][i][h] = *(const v4i *)(lp + i * dg.lplane + kc * dg.lchunk);
@@ -1206,23 +1246,23 @@ __global__ __launch_bounds__(256) void k_update_wide(T *a, int64_t ld, int32_t m
                     }
                 }
     };
-    int nt = (ce - qb + TW - 1) / TW;
+    int nt = (vce - qb + TW - 1) / TW;
     nt = nt < WT_NT ? nt : WT_NT;
     {
         v4i st[PPT];
-        rload(qb, st);
+        rload(real(qb), st);
         rstore(0, st);
     }
-    if (wact) loadx(qb, xv);
+    if (wact) loadx(real(qb), xv);
     __syncthreads();
 #pragma unroll 1
     for (int t = 0; t < nt; ++t) {
-        const int32_t q0 = qb + t * TW;
+        const int32_t q0 = real(qb + t * TW), qn = real(qb + (t + 1) * TW);
         const bool more = t + 1 < nt;
         v4i st[PPT];
         if (more) {
-            rload(q0 + TW, st);
-            if (wact) loadx(q0 + TW, xn);
+            rload(qn, st);
+            if (wact) loadx(qn, xn);
         }
         if (wact) {
             const int8_t *lb = rl[t & 1] + lr * 64 + 16 * lk;
@@ -1234,13 +1274,13 @@ __global__ __launch_bounds__(256) void k_update_wide(T *a, int64_t ld, int32_t m
 #pragma unroll
                     for (int q = 0; q < 2 * ND - 1; ++q) acc[rs][cs][q] = v4i{0, 0, 0, 0};
 #pragma unroll
-            for (int kc = 0; kc < 4; ++kc) {
+            for (int kc = 0; kc < KC; ++kc) {
                 v4i B[ND][CS];
 #pragma unroll
                 for (int i = 0; i < ND; ++i)
 #pragma unroll
                     for (int h = 0; h < CS; ++h)
-                        B[i][h] = *(const v4i *)(lb + ((i * 4 + kc) * TW + 16 * h) * 64);
+                        B[i][h] = *(const v4i *)(lb + ((i * KC + kc) * TW + 16 * h) * 64);
 #pragma unroll
                 for (int i = 0; i < ND; ++i)
 #pragma unroll
@@ -1357,6 +1397,7 @@ __global__ void k_dense_init(int32_t m, int32_t n, int32_t *rowpiv, int32_t *col
         st->head = 0;
         st->count = m;
         st->nzero = 0;
+        st->pcount = 0;
     }
 }
 
@@ -1428,7 +1469,9 @@ int dense_rref(gbgpu_dense *d, int form) {
     return GBGPU_OK;
 }
 
-int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws) {
+// the panel loop on A's entries as T (a = A, ld its row stride in T)
+template <typename T>
+static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs &ws) {
     gbgpu_ctx *ctx = d->ctx;
     hipStream_t s = ctx->stream;
     const int32_t m = d->m, n = d->n;
@@ -1457,15 +1500,19 @@ int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws) {
         if (form == GBGPU_FORM_MFMA) {
             dim3 g((unsigned)((ce - cb - gsz + 63) / 64), (unsigned)((m + 63) / 64));
             switch (dg.nd) {
-                case 1: k_update_mfma<1><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz); break;
-                case 2: k_update_mfma<2><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz); break;
-                case 3: k_update_mfma<3><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz); break;
-                default: k_update_mfma<4><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz); break;
+                case 1: k_update_mfma<1, 1, 2, 2, T><<<g, 256, 0, s>>>(a, ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz); break;
+                case 2: k_update_mfma<2, 1, 2, 2, T><<<g, 256, 0, s>>>(a, ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz); break;
+                default:
+                    if constexpr (sizeof(T) == 4) {
+                        if (dg.nd == 3) k_update_mfma<3><<<g, 256, 0, s>>>(a, ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz);
+                        else k_update_mfma<4><<<g, 256, 0, s>>>(a, ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz);
+                    }
+                    break;
             }
-        } else {
+        } else if constexpr (sizeof(T) == 4) {
             dim3 g((unsigned)((ce - cb + 255) / 256), (unsigned)((m + RT - 1) / RT));
-            if (big) k_update_plain<true><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, rp, ldr, L, active, md, r64);
-            else k_update_plain<false><<<g, 256, 0, s>>>(d->a, d->ld, m, cb, ce, c0, pp, rp, ldr, L, active, md, r64);
+            if (big) k_update_plain<true><<<g, 256, 0, s>>>(a, ld, m, cb, ce, c0, pp, rp, ldr, L, active, md, r64);
+            else k_update_plain<false><<<g, 256, 0, s>>>(a, ld, m, cb, ce, c0, pp, rp, ldr, L, active, md, r64);
         }
     };
     if (hipEventRecord(ctx->ev[2], s) != hipSuccess) { return check_hip(ctx, hipGetLastError(), "event"); }
@@ -1480,9 +1527,9 @@ int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws) {
     hipStream_t x = ctx->side;
     auto select = [&](hipStream_t sx, int32_t cs, Panel *pp) {
         if (d->p < (1u << 16))
-            k_panel_select<1><<<1, SEL_T, 0, sx>>>(d->a, d->ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg);
+            k_panel_select<1, T><<<1, SEL_T, 0, sx>>>(a, ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg);
         else
-            k_panel_select<0><<<1, SEL_T, 0, sx>>>(d->a, d->ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg);
+            k_panel_select<0, T><<<1, SEL_T, 0, sx>>>(a, ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg);
     };
     if (m > 0 && n > 0)
         select(s, 0, pan);
@@ -1495,18 +1542,34 @@ int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws) {
         const int32_t c2 = (n - c1) < PB ? n : c1 + PB;
         // Rp over the columns the next select needs first; the rest after
         // that select is on its way
-        launch_rp(s, true, d->a, d->ld, c0, c0, c2, cur, rp, ldr, md, r64, dg);
-        k_panel_gather<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, m, c0, bw, rowpiv, cur, L, active,
-                                                      d->p, dg);
+        launch_rp<T>(s, true, a, ld, c0, c0, c2, cur, rp, ldr, md, r64, dg);
+        k_panel_gather<T><<<grid_of(m, 4), 256, 0, s>>>(a, ld, m, c0, bw, rowpiv, cur, L, active,
+                                                         d->p, dg);
         if (c1 < n) {
             update(s, cur, c0, c1, c2);
             if (hipEventRecord(ctx->sync_ev[0], s) != hipSuccess ||
                 hipStreamWaitEvent(x, ctx->sync_ev[0], 0) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
-            if (c2 < n) launch_rp(x, false, d->a, d->ld, c0, c2, n, cur, rp, ldr, md, r64, dg);
-            if (form == GBGPU_FORM_MFMA && c2 < n && c1 - c0 == PB) {
+            if (c2 < n) launch_rp<T>(x, false, a, ld, c0, c2, n, cur, rp, ldr, md, r64, dg);
+            if (form == GBGPU_FORM_MFMA && c2 < n && c1 - c0 == PB && dg.nd <= 2) {
                 update(x, cur, c0, c0, n, c1, c2 - c1);  // [c0, c1) and [c2, n)
+            } else if (form == GBGPU_FORM_MFMA && c2 < n && c1 - c0 == PB) {
+                // [c0, c1) and [c2, n) for three and four digits: blocks that
+                // walk 512 columns with the Rp digits staged in LDS (measured
+                // faster than 64 x 64 tiles there, slower for one or two)
+                const int rows = dg.nd <= 2 ? 128 : 64;
+                dim3 g((unsigned)((n - c0 - (c2 - c1) + 511) / 512), (unsigned)((m + rows - 1) / rows));
+                switch (dg.nd) {
+                    case 1: k_update_wide<1, 2, T, 1><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1); break;
+                    case 2: k_update_wide<2, 2, T, 1><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1); break;
+                    default:
+                        if constexpr (sizeof(T) == 4) {
+                            if (dg.nd == 3) k_update_wide<3, 1, T, 1><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1);
+                            else k_update_wide<4, 1, T, 1><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1);
+                        }
+                        break;
+                }
             } else {
                 update(x, cur, c0, c0, c1);
                 update(x, cur, c0, c2, n);
@@ -1520,8 +1583,7 @@ int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws) {
             if (hipStreamWaitEvent(s, ctx->sync_ev[1], 0) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
-            k_prune_flags<<<grid_of(m, 4), 256, 0, s>>>(d->a, d->ld, n, c1, cand, st, active);
-            k_prune_compact<<<1, 256, 0, s>>>(cand, st, active);
+            k_prune<T><<<grid_of(m, 4), 256, 0, s>>>(a, ld, n, c1, cand, st, active);
         } else {
             update(s, cur, c0, c0, n);
         }
@@ -1538,6 +1600,71 @@ int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws) {
         }
     }
     (void)hipEventRecord(ctx->ev[3], s);
+    return GBGPU_OK;
+}
+
+// In-place change of an m x n block's entries between uint32_t and uint16_t
+// (residues of p <= 65536): row r's 16-bit entries take the first half of
+// its own bytes (row stride 2 ld in 16-bit units).  One wave per row, in
+// groups of 512 entries: a group's 16-bit image covers the 32-bit entries of
+// half as many groups (to 16 bits) or its 32-bit image twice as many (back),
+// so going up (down) the row, every entry a group overwrites has been read
+// already, by this wave.
+__global__ void k_to_u16(uint32_t *a, int64_t ld, int32_t m, int32_t n) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= m) return;
+    uint32_t *row = a + w * ld;
+    uint16_t *out = (uint16_t *)row;
+    for (int32_t g = 0; g < n; g += 512) {
+        uint32_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int32_t j = g + 64 * k + lane;
+            v[k] = j < n ? row[j] : 0u;
+        }
+        __builtin_amdgcn_s_waitcnt(0);  // the group's reads are in before its writes
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int32_t j = g + 64 * k + lane;
+            if (j < n) out[j] = (uint16_t)v[k];
+        }
+    }
+}
+
+__global__ void k_to_u32(uint32_t *a, int64_t ld, int32_t m, int32_t n) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= m) return;
+    uint32_t *row = a + w * ld;
+    const uint16_t *in = (const uint16_t *)row;
+    for (int32_t g = ((n - 1) / 512) * 512; g >= 0; g -= 512) {
+        uint32_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int32_t j = g + 64 * k + lane;
+            v[k] = j < n ? (uint32_t)in[j] : 0u;
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int32_t j = g + 64 * k + lane;
+            if (j < n) row[j] = v[k];
+        }
+    }
+}
+
+int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws) {
+    // the MFMA form keeps residues of p <= 65521 in 16 bits while it runs,
+    // which halves the bytes every update moves
+    const bool h = form == GBGPU_FORM_MFMA && d->p <= 65521u && d->m > 0 && d->n > 0;
+    if (!h) return panel_loop<uint32_t>(d, d->a, d->ld, form, ws);
+    gbgpu_ctx *ctx = d->ctx;
+    k_to_u16<<<grid_of(d->m, 4), 256, 0, ctx->stream>>>(d->a, d->ld, d->m, d->n);
+    const int rc = panel_loop<uint16_t>(d, (uint16_t *)d->a, 2 * d->ld, form, ws);
+    k_to_u32<<<grid_of(d->m, 4), 256, 0, ctx->stream>>>(d->a, d->ld, d->m, d->n);
+    if (rc) return rc;
+    GBGPU_HIP(ctx, hipGetLastError());
     return GBGPU_OK;
 }
 
