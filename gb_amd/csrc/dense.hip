@@ -131,7 +131,7 @@ __device__ __forceinline__ uint32_t sel_mad(uint32_t a, uint32_t x, uint32_t b, 
 
 constexpr int SEL_T = 512;            // threads of the select workgroup
 constexpr int SEL_W = SEL_T / 64;     // its waves
-constexpr int SW = 16;                // columns of a sub-panel of the pivot chain
+constexpr int SW = 32;                // columns of a sub-panel of the pivot chain
 
 struct SelShared {
     uint32_t X[PB][XW];
@@ -280,18 +280,22 @@ __device__ __forceinline__ uint32_t digits4(uint32_t x, uint32_t p, uint32_t hi)
 // The sub-panel apply on the int8 matrix cores: X <- diag(D) X + C P with
 // P = X[qs[0..SW)] (SW x 2 PB), every column of X (the panel columns past
 // bw are zero in every slot).  C and P go into ND signed digits each; the
-// products of digit sum t = i + j are one v_mfma_i32_16x16x64_i8 with the
-// pairs (i, j) in its four 16-term K groups, below 4 * 16 * 128^2 = 2^20 in
-// size.  Wave w: slots 16 (w % 4) .., columns 64 (w / 4) .. (four 16 x 16
-// tiles).  Every output has one writer, which reads its old value only.
+// products of digit sum t = i + j are v_mfma_i32_16x16x64_i8 with the pairs
+// (i, j) in its four 16-term K groups, a pair taking SW / 16 of them (so
+// 64 / SW pairs per product), below 4 * SW * 128^2 <= 2^21 in size.  Wave
+// w: slots 16 (w % 4) .., columns 64 (w / 4) .. (four 16 x 16 tiles).
+// Every output has one writer, which reads its old value only.
 template <int ND>
 __device__ __forceinline__ void sel_apply_mfma(SelShared &sh, const Mod &md, const Digits &dg) {
     static_assert(SEL_W == 8, "four row blocks times two column halves");
+    constexpr int GPP = SW / 16;                 // K groups per pair
+    constexpr int PPM = 4 / GPP;                 // pairs per product
+    constexpr int NQD = SW / 4;                  // four-term pieces per row
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t p = md.p;
-    {
-        // P's digits: thread = (column, four terms); C's: (slot, four terms)
-        const int col = tid >> 2, k = tid & 3;
+    // P's digits: thread = (column, four terms); C's: (slot, four terms)
+    for (int u = tid; u < 2 * PB * NQD; u += SEL_T) {
+        const int col = u / NQD, k = u % NQD;
         uint32_t w[ND] = {};
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
@@ -301,41 +305,42 @@ __device__ __forceinline__ void sel_apply_mfma(SelShared &sh, const Mod &md, con
         }
 #pragma unroll
         for (int i = 0; i < ND; ++i) *(uint32_t *)&sh.Pd[i][col][4 * k] = w[i];
-        if (tid < PB * 4) {
-            const int s = tid >> 2;
-            uint32_t wc[ND] = {};
+    }
+    for (int u = tid; u < PB * NQD; u += SEL_T) {
+        const int s = u / NQD, k = u % NQD;
+        uint32_t wc[ND] = {};
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const uint32_t d4 = digits4(sh.C[s][4 * k + b], p, dg.hi);
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t d4 = digits4(sh.C[s][4 * k + b], p, dg.hi);
 #pragma unroll
-                for (int i = 0; i < ND; ++i) wc[i] |= ((d4 >> (8 * i)) & 0xFFu) << (8 * b);
-            }
-#pragma unroll
-            for (int i = 0; i < ND; ++i) *(uint32_t *)&sh.Cd[i][s][4 * k] = wc[i];
+            for (int i = 0; i < ND; ++i) wc[i] |= ((d4 >> (8 * i)) & 0xFFu) << (8 * b);
         }
+#pragma unroll
+        for (int i = 0; i < ND; ++i) *(uint32_t *)&sh.Cd[i][s][4 * k] = wc[i];
     }
     __syncthreads();
     const int rb = wv & 3, lr = lane & 15, g = lane >> 4;
-    // this lane's K group g of digit sum t: the pair (i, t - i), i = lowest + g
-    v4i A[2 * ND - 1];
-    int jt[2 * ND - 1];
-#pragma unroll
-    for (int t = 0; t < 2 * ND - 1; ++t) {
-        const int ilo = t - (ND - 1) > 0 ? t - (ND - 1) : 0, ihi = t < ND - 1 ? t : ND - 1;
-        const int i = ilo + g;
-        const bool ok = i <= ihi;
-        jt[t] = ok ? t - i : -1;
-        A[t] = ok ? *(const v4i *)&sh.Cd[ok ? i : 0][rb * 16 + lr][0] : v4i{0, 0, 0, 0};
-    }
+    // product m of digit sum t: this lane's K group g holds terms
+    // 16 (g % GPP) .. of the pair (i, t - i), i = lowest + PPM m + g / GPP
+    constexpr int NM = (ND + PPM - 1) / PPM;     // products per digit sum, at most
+    const int ho = 16 * (g % GPP);
 #pragma unroll
     for (int cbi = 0; cbi < 4; ++cbi) {
         const int cb16 = (wv >> 2) * 4 + cbi;
         v4i acc[2 * ND - 1];
 #pragma unroll
         for (int t = 0; t < 2 * ND - 1; ++t) {
-            const v4i B = jt[t] >= 0 ? *(const v4i *)&sh.Pd[jt[t] >= 0 ? jt[t] : 0][cb16 * 16 + lr][0]
-                                     : v4i{0, 0, 0, 0};
-            acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[t], B, v4i{0, 0, 0, 0}, 0, 0, 0);
+            const int ilo = t - (ND - 1) > 0 ? t - (ND - 1) : 0, ihi = t < ND - 1 ? t : ND - 1;
+            acc[t] = v4i{0, 0, 0, 0};
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                if (ilo + PPM * m > ihi) continue;  // no pair left for this product
+                const int i = ilo + PPM * m + g / GPP;
+                const bool ok = i <= ihi;
+                const v4i A = ok ? *(const v4i *)&sh.Cd[ok ? i : 0][rb * 16 + lr][ho] : v4i{0, 0, 0, 0};
+                const v4i B = ok ? *(const v4i *)&sh.Pd[ok ? t - i : 0][cb16 * 16 + lr][ho] : v4i{0, 0, 0, 0};
+                acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B, acc[t], 0, 0, 0);
+            }
         }
         // result register e of lane l: slot 16 rb + 4 (l / 16) + e, column l % 16
 #pragma unroll
@@ -362,20 +367,69 @@ __device__ __forceinline__ void sel_apply_mfma(SelShared &sh, const Mod &md, con
 // run by every wave (lane = slot): column j of the sub-panel and its
 // combination coefficient belong to wave j % SEL_W, the row scale D is kept
 // by every wave.  Step c: the owner of column c finds the pivot (the lowest
-// unselected tile row nonzero there) and posts it with every slot's entry
-// of column c; after one barrier every wave eliminates its own columns.
-// Records the pivots as E rows kk, kk + 1, ... and leaves the slots'
-// combinations in C (column c of the sub-panel: coefficient of the row that
-// pivots there, zero if none), D and qs.  Returns the number of pivots (the
-// same in every thread); um loses the selected slots.  X is only read: the
-// apply that follows writes every column.
+// unselected tile row nonzero there: the columns left of c are zero in
+// every unselected row) and posts it with every slot's entry of column c;
+// after one barrier every wave eliminates its own columns, every other slot
+// x becoming a x - x[c] * pivot row.  Records the pivots as E rows kk, kk +
+// 1, ... and leaves the slots' combinations in C (column c of the
+// sub-panel: coefficient of the row that pivots there, zero if none), D
+// and qs.  Returns the number of pivots (the same in every thread); um
+// loses the selected slots.  X is only read: the apply that follows writes
+// every column.  (Blocks of consecutive columns per wave, one barrier per
+// block, were measured slower: the next block's owner has to take the
+// block's steps before its own.)
 constexpr int NCW = SW / SEL_W;  // sub-panel columns per wave
 static_assert(SW % SEL_W == 0, "sub-panel columns split evenly over the waves");
+
+// one step of the pivot chain on a wave's own columns wv, wv + SEL_W, ...:
+// column c, pivot slot is, pivot value av, this slot's entry f of column c.  Every other slot x becomes av x - f * pivot row (its entries
+// right of c) and its combination coefficients follow; the pivot's slot
+// keeps its own.
+template <int P16>
+__device__ __forceinline__ void chain_step(uint32_t (&v)[NCW], uint32_t (&cc)[NCW], uint32_t &dd,
+                                           uint32_t &pm, uint64_t &um, int &np, int c, int is,
+                                           uint32_t av, uint32_t f, int wv, int lane, const Mod &md,
+                                           float pinv) {
+    const uint32_t p = md.p;
+    const uint32_t dp = (uint32_t)__builtin_amdgcn_readlane((int)dd, is);
+    const bool me = lane == is;
+    const uint32_t nf = p - f;
+#pragma unroll
+    for (int i = 0; i < NCW; ++i) {
+        const int j = wv + SEL_W * i;  // this wave's column (uniform)
+        // new values first, one unconditional write of each at the end (a
+        // store per branch would be merged through a pointer and put the
+        // arrays in scratch memory)
+        uint32_t nv = v[i], nc = cc[i];
+        if (j > c) {
+            const uint32_t pv = (uint32_t)__builtin_amdgcn_readlane((int)v[i], is);
+            const uint32_t t = sel_mad<P16>(av, v[i], nf, pv, md, pinv);
+            nv = me ? nv : t;
+        } else if (j < c) {
+            if ((pm >> j) & 1u) {
+                const uint32_t pc = (uint32_t)__builtin_amdgcn_readlane((int)cc[i], is);
+                const uint32_t t = sel_mad<P16>(av, cc[i], nf, pc, md, pinv);
+                nc = me ? nc : t;
+            }
+        } else {
+            const uint32_t t = sel_mad<P16>(dp, nf, 0u, 0u, md, pinv);
+            nc = me ? 0u : t;
+        }
+        v[i] = nv;
+        cc[i] = nc;
+    }
+    const uint32_t ndd = sel_mad<P16>(av, dd, 0u, 0u, md, pinv);
+    dd = me ? dd : ndd;
+    pm |= 1u << c;
+    um &= ~(1ull << is);
+    ++np;
+}
 
 template <int P16>
 __device__ __forceinline__ int sel_chain(SelShared &sh, int cb, int bw, uint64_t &um, int kk,
                                          int32_t c0, uint32_t &pm, const Mod &md, float pinv) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t p = md.p;
     uint32_t v[NCW], cc[NCW];
     uint32_t dd = 1u;
@@ -408,44 +462,18 @@ __device__ __forceinline__ int sel_chain(SelShared &sh, int cb, int bw, uint64_t
             if (lane == 0) {
                 sh.sis[buf] = is;
                 sh.sav[buf] = is >= 0 ? (uint32_t)__builtin_amdgcn_readlane((int)v[ic], is) : 0u;
+                if (is >= 0) {
+                    sh.qs[c] = is;
+                    sh.ecol[kk + np] = c0 + cb + c;
+                    sh.eslot[kk + np] = is;
+                }
             }
         }
         __syncthreads();
         const int is = __builtin_amdgcn_readfirstlane(sh.sis[buf]);
         if (is < 0) continue;
         const uint32_t av = (uint32_t)__builtin_amdgcn_readfirstlane((int)sh.sav[buf]);
-        const uint32_t f = sh.f[buf][lane];
-        const uint32_t dp = (uint32_t)__builtin_amdgcn_readlane((int)dd, is);
-        const bool me = lane == is;
-        const uint32_t nf = p - f;
-#pragma unroll
-        for (int i = 0; i < NCW; ++i) {
-            const int j = wv + SEL_W * i;  // this wave's column (uniform)
-            if (j > c) {
-                const uint32_t pv = (uint32_t)__builtin_amdgcn_readlane((int)v[i], is);
-                const uint32_t nv = sel_mad<P16>(av, v[i], nf, pv, md, pinv);
-                v[i] = me ? v[i] : nv;
-            } else if (j < c) {
-                if ((pm >> j) & 1u) {
-                    const uint32_t pc = (uint32_t)__builtin_amdgcn_readlane((int)cc[i], is);
-                    const uint32_t nc = sel_mad<P16>(av, cc[i], nf, pc, md, pinv);
-                    cc[i] = me ? cc[i] : nc;
-                }
-            } else {
-                const uint32_t ncc = sel_mad<P16>(dp, nf, 0u, 0u, md, pinv);
-                cc[i] = me ? 0u : ncc;
-            }
-        }
-        const uint32_t ndd = sel_mad<P16>(av, dd, 0u, 0u, md, pinv);
-        dd = me ? dd : ndd;
-        if (wv == own && lane == 0) {
-            sh.qs[c] = is;
-            sh.ecol[kk + np] = c0 + cb + c;
-            sh.eslot[kk + np] = is;
-        }
-        pm |= 1u << c;
-        um &= ~(1ull << is);
-        ++np;
+        chain_step<P16>(v, cc, dd, pm, um, np, c, is, av, sh.f[buf][lane], wv, lane, md, pinv);
     }
 #pragma unroll
     for (int i = 0; i < NCW; ++i) {
