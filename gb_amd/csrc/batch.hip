@@ -256,7 +256,7 @@ int layout_dense(gbgpu_batch *b, int form) {
     b->nsteps = (max_n + 63) / 64;
     b->htab.assign((size_t)b->nsteps * 4 * (nmat + 1), 0);
     for (int32_t k = 0; k < b->nsteps; ++k)
-        batch_dense_counts(b->hd.data(), nmat, k, form, b->htab.data() + (size_t)k * 4 * (nmat + 1));
+        batch_dense_counts(b->hd.data(), nmat, k, form, b->nd, b->htab.data() + (size_t)k * 4 * (nmat + 1));
     if ((rc = dalloc(ctx, &b->d_dense, (size_t)nmat)) ||
         (rc = dalloc(ctx, &b->d_tab, b->htab.size() ? b->htab.size() : 1)))
         return rc;

@@ -1192,12 +1192,12 @@ __global__ __launch_bounds__(256) void k_update_mfma(T *a, int64_t ld, int32_t m
 constexpr int WT_NT = 16;   // column tiles per block
 constexpr int WT_CS = 2;    // 16-column products per tile
 
-template <int ND, int RS, typename T = uint32_t, int KC = 4>
-__global__ __launch_bounds__(256) void k_update_wide(T *a, int64_t ld, int32_t m,
-                                                     int32_t cb, int32_t ce, int32_t c0,
-                                                     const Panel *pan, const int32_t *active,
-                                                     Mod md, Digits dg, int32_t gb = INT_MAX,
-                                                     int32_t gsz = 0) {
+template <int ND, int RS, typename T, int KC>
+__device__ __forceinline__ void update_wide_body(T *a, int64_t ld, int32_t m, int32_t cb,
+                                                 int32_t ce, int32_t c0, const Panel *pan,
+                                                 const int32_t *active, const Mod &md,
+                                                 const Digits &dg, int32_t gb, int32_t gsz,
+                                                 int32_t bx, int32_t by) {
     constexpr int CS = WT_CS, TW = 16 * CS;
     constexpr int TB = ND * KC * TW * 64;  // bytes of Rp digits per tile
     constexpr int NP = TB / 16;            // its 16-byte pieces
@@ -1206,12 +1206,12 @@ __global__ __launch_bounds__(256) void k_update_wide(T *a, int64_t ld, int32_t m
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     // tiles walk the columns [cb, ce) without the gap [gb, gb + gsz) (gb - cb
     // a multiple of TW): virtual column v is column v, or v + gsz from gb on
-    const int32_t qb = cb + (int32_t)blockIdx.x * (WT_NT * TW);
+    const int32_t qb = cb + bx * (WT_NT * TW);
     const int32_t vce = ce - gsz;
     auto real = [&](int32_t v) { return v >= gb ? v + gsz : v; };
     if (kk == 0 || qb >= vce) return;  // uniform over the block
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int32_t r0 = ((int32_t)blockIdx.y * 4 + wv) * (16 * RS);
+    const int32_t r0 = (by * 4 + wv) * (16 * RS);
     const bool wact = __any(lane < 16 * RS && r0 + lane < m && active[r0 + lane] != 0);
     const int lr = lane & 15, lk = lane >> 4;
     // piece u 256 + tid of a tile: digit i, chunk kc, column c / 4, bytes
@@ -1353,6 +1353,16 @@ __global__ __launch_bounds__(256) void k_update_wide(T *a, int64_t ld, int32_t m
         // with the buffer the tile after it overwrites
         __syncthreads();
     }
+}
+
+template <int ND, int RS, typename T = uint32_t, int KC = 4>
+__global__ __launch_bounds__(256) void k_update_wide(T *a, int64_t ld, int32_t m,
+                                                     int32_t cb, int32_t ce, int32_t c0,
+                                                     const Panel *pan, const int32_t *active,
+                                                     Mod md, Digits dg, int32_t gb = INT_MAX,
+                                                     int32_t gsz = 0) {
+    update_wide_body<ND, RS, T, KC>(a, ld, m, cb, ce, c0, pan, active, md, dg, gb, gsz,
+                                    (int32_t)blockIdx.x, (int32_t)blockIdx.y);
 }
 
 template <typename T>
@@ -1811,6 +1821,19 @@ __global__ __launch_bounds__(256) void k_batch_update_mfma(const DenseDesc *D, c
     update_i8_body<ND, 1, 2, 2>(d.a, d.ld, d.m, c0, d.n, c0, d.pan, d.active, d.md, d.dg, l % ct, l / ct);
 }
 
+// three and four digits: blocks that walk 512 columns of 64 rows with the Rp
+// digits staged in LDS (K = 64), as the single block's side stream does
+template <int ND>
+__global__ __launch_bounds__(256) void k_batch_update_wide(const DenseDesc *D, const int32_t *pre,
+                                                           int32_t nmat, int32_t c0) {
+    const int i = tab_find(pre, nmat, blockIdx.x);
+    const DenseDesc &d = D[i];
+    const int32_t l = blockIdx.x - pre[i];
+    const int32_t ct = (d.n - c0 + 511) / 512;
+    update_wide_body<ND, 1, uint32_t, 1>(d.a, d.ld, d.m, c0, d.n, c0, d.pan, d.active, d.md, d.dg,
+                                         INT_MAX, 0, l % ct, l / ct);
+}
+
 template <bool BIG>
 __global__ __launch_bounds__(256) void k_batch_update_plain(const DenseDesc *D, const int32_t *pre,
                                                             int32_t nmat, int32_t c0) {
@@ -1892,8 +1915,8 @@ int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int
     const void *upd = form == GBGPU_FORM_MFMA
                           ? (nd == 1 ? (const void *)k_batch_update_mfma<1>
                              : nd == 2 ? (const void *)k_batch_update_mfma<2>
-                             : nd == 3 ? (const void *)k_batch_update_mfma<3>
-                                       : (const void *)k_batch_update_mfma<4>)
+                             : nd == 3 ? (const void *)k_batch_update_wide<3>
+                                       : (const void *)k_batch_update_wide<4>)
                           : (big ? (const void *)k_batch_update_plain<true>
                                  : (const void *)k_batch_update_plain<false>);
     if ((rc = check_launch(ctx, (const void *)k_batch_select, nmat, SEL_T, "batch select")) ||
@@ -1912,8 +1935,8 @@ int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int
             switch (nd) {
                 case 1: k_batch_update_mfma<1><<<nup, 256, 0, s>>>(D, pu, nmat, c0); break;
                 case 2: k_batch_update_mfma<2><<<nup, 256, 0, s>>>(D, pu, nmat, c0); break;
-                case 3: k_batch_update_mfma<3><<<nup, 256, 0, s>>>(D, pu, nmat, c0); break;
-                default: k_batch_update_mfma<4><<<nup, 256, 0, s>>>(D, pu, nmat, c0); break;
+                case 3: k_batch_update_wide<3><<<nup, 256, 0, s>>>(D, pu, nmat, c0); break;
+                default: k_batch_update_wide<4><<<nup, 256, 0, s>>>(D, pu, nmat, c0); break;
             }
         } else {
             if (big) k_batch_update_plain<true><<<nup, 256, 0, s>>>(D, pu, nmat, c0);
@@ -1929,7 +1952,8 @@ int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int
 }
 
 // block counts of one batch step (prefix sums over the matrices)
-void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int form, int32_t *out) {
+void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int form, int nd,
+                        int32_t *out) {
     const int32_t c0 = step * PB;
     int32_t *prp = out, *pga = out + (nmat + 1), *pup = out + 2 * (nmat + 1),
             *ppr = out + 3 * (nmat + 1);
@@ -1941,7 +1965,8 @@ void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int for
         const int32_t cap = rows4 < 2048 ? rows4 : 2048;
         int32_t nup = 0;
         if (on) {
-            if (form == GBGPU_FORM_MFMA) nup = ((n - c0 + 63) / 64) * ((m + 63) / 64);
+            if (form == GBGPU_FORM_MFMA && nd >= 3) nup = ((n - c0 + 511) / 512) * ((m + 63) / 64);
+            else if (form == GBGPU_FORM_MFMA) nup = ((n - c0 + 63) / 64) * ((m + 63) / 64);
             else nup = ((n - c0 + 255) / 256) * ((m + RT - 1) / RT);
         }
         prp[i + 1] = prp[i] + (on ? (n - c0 + 63) / 64 : 0);

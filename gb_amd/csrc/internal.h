@@ -293,8 +293,10 @@ struct DenseDesc {
 };
 int batch_dense_init(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat);
 // block counts of step `step` for the four grids (rp, gather, update,
-// prune), prefix sums over the matrices: out[4 * (nmat + 1)]
-void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int form, int32_t *out);
+// prune), prefix sums over the matrices: out[4 * (nmat + 1)]; nd: the
+// update's digit count (three and four take 512-column strips)
+void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int form, int nd,
+                        int32_t *out);
 // nd: digits of the MFMA form's update (the largest of the batch)
 int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int32_t *tab,
                      const int32_t *htab, int32_t step, int form, bool big, int nd);
