@@ -165,7 +165,9 @@ struct Digits {
     uint64_t off = 0;      // multiple of p above 2^57: makes a combined sum positive
     uint64_t off2 = 0;     // multiple of p above 2^41 (the high part for nd = 4)
     uint32_t w32 = 0;      // 2^32 mod p
+    uint32_t w32s = 0;     // floor(w32 2^32 / p): Shoup's quotient for products by w32
     float pinvf = 0;       // 1 / p in single precision (digits_balanced2, nd = 2)
+    double pinvd = 0;      // 1 / p in double precision (mod_f64, nd >= 3)
 };
 
 // nd = 2 stops at 65521, the largest prime below 2^16, where
@@ -182,7 +184,9 @@ __host__ inline void digits_setup(Digits &g, uint32_t p, int nd) {
     g.w32 = (uint32_t)((1ull << 32) % p);
     g.off = ((1ull << 57) / p + 1) * p;
     g.off2 = ((1ull << 41) / p + 1) * p;
+    g.w32s = (uint32_t)(((uint64_t)g.w32 << 32) / p);
     g.pinvf = 1.0f / (float)p;
+    g.pinvd = 1.0 / (double)p;
 }
 
 __device__ __forceinline__ uint32_t digit_word(uint32_t x, uint32_t p, uint32_t hi, int i) {
@@ -198,28 +202,54 @@ __device__ __forceinline__ uint32_t digit_word(uint32_t x, uint32_t p, uint32_t 
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
+// x mod p for |x| < 2^52: x is exact in double precision, the quotient
+// floor(x / p) from the double product x (1/p) is off by at most one (the
+// product's relative error is below 2^-52, so its absolute error is below
+// one), and the remainder x - q p, exact in 64 bits, is brought into [0, p)
+// by one correction either way.  Full-rate double operations and one 64-bit
+// product instead of a 64-bit Barrett reduction.
+__device__ __forceinline__ uint32_t mod_f64(int64_t x, uint32_t p, double pinv) {
+    const double q = __builtin_floor((double)x * pinv);
+    int64_t r = x - (int64_t)q * (int64_t)p;
+    r += r < 0 ? (int64_t)p : 0;
+    r -= r >= (int64_t)p ? (int64_t)p : 0;
+    return (uint32_t)r;
+}
+
 // sum_s Cs 256^s mod p for result element e of the accumulators.  Over up
-// to 256 pivot slots |Cs| <= min(s + 1, 2 nd - 1 - s) 256 128^2 < 2^24, so
-// for nd <= 3 the shifted sum (C4 enters as C4 2^32) is below 2^55 in size
-// and dg.off (a multiple of p above 2^57) makes it positive; for nd = 4 the
-// terms s >= 4 form hi = C4 + 2^8 C5 + 2^16 C6 (below 2^39), reduced with
-// dg.off2 and brought in as (hi mod p) (2^32 mod p).  One reduction at the
-// end.
+// to 256 pivot slots |Cs| <= min(s + 1, 2 nd - 1 - s) 256 128^2 < 2^24.
+//   nd <= 2: the shifted sum is below 2^41, one reduction (dg.off, a
+//     multiple of p above 2^57, makes it positive);
+//   nd = 3: hi = C3 + 2^8 C4 (below 2^33) is reduced first, then
+//     C0 + 2^8 C1 + 2^16 C2 + (hi mod p) 2^24 (below 2^49, p < 2^24);
+//   nd = 4: hi = C4 + 2^8 C5 + 2^16 C6 (below 2^41) is reduced first and
+//     multiplied by 2^32 mod p with Shoup's quotient (a result below 2p),
+//     then C0 + 2^8 C1 + 2^16 C2 + 2^24 C3 (below 2^49) plus that.
+// The reductions of nd >= 3 run in double precision (mod_f64).
 template <int ND>
 __device__ __forceinline__ uint32_t digits_combine(const v4i *acc, int e, const Mod &md,
                                                    const Digits &dg) {
-    int64_t lo = 0;
+    if constexpr (ND <= 2) {
+        int64_t lo = 0;
 #pragma unroll
-    for (int q = 0; q < (2 * ND - 1 < 4 ? 2 * ND - 1 : 4); ++q) lo += (int64_t)acc[q][e] << (8 * q);
-    if constexpr (ND == 3) lo += (int64_t)acc[4][e] << 32;
-    if constexpr (ND == 4) {
-        int64_t hi = 0;
-#pragma unroll
-        for (int q = 4; q < 2 * ND - 1; ++q) hi += (int64_t)acc[q][e] << (8 * (q - 4));
-        const uint32_t hr = md.reduce((uint64_t)(hi + (int64_t)dg.off2));
-        lo += (int64_t)md.mul(hr, dg.w32);
+        for (int q = 0; q < 2 * ND - 1; ++q) lo += (int64_t)acc[q][e] << (8 * q);
+        return md.reduce((uint64_t)(lo + (int64_t)dg.off));
+    } else if constexpr (ND == 3) {
+        const int64_t hi = (int64_t)acc[3][e] + ((int64_t)acc[4][e] << 8);
+        const uint32_t hr = mod_f64(hi, md.p, dg.pinvd);
+        const int64_t lo = (int64_t)acc[0][e] + ((int64_t)acc[1][e] << 8) +
+                           ((int64_t)acc[2][e] << 16) + ((int64_t)hr << 24);
+        return mod_f64(lo, md.p, dg.pinvd);
+    } else {
+        const int64_t hi = (int64_t)acc[4][e] + ((int64_t)acc[5][e] << 8) + ((int64_t)acc[6][e] << 16);
+        const uint32_t hr = mod_f64(hi, md.p, dg.pinvd);
+        // hr (2^32 mod p) mod p, below 2p, by Shoup's precomputed quotient
+        const uint32_t q = __umulhi(hr, dg.w32s);
+        const uint32_t t = hr * dg.w32 - q * md.p;
+        const int64_t lo = (int64_t)acc[0][e] + ((int64_t)acc[1][e] << 8) +
+                           ((int64_t)acc[2][e] << 16) + ((int64_t)acc[3][e] << 24) + (int64_t)t;
+        return mod_f64(lo, md.p, dg.pinvd);
     }
-    return md.reduce((uint64_t)(lo + (int64_t)dg.off));
 }
 
 // C0 + 2^8 C1 + 2^16 C2 (nd = 2, up to 256 pivot slots: |C0|, |C2| <= 2^22,
