@@ -156,12 +156,39 @@ int gbgpu_init(int device, gbgpu_ctx **out) {
             return GBGPU_EDEVICE;
         }
     }
+    // ordering between the context's two streams on one device needs no
+    // system-scope fence; the lag events order nothing (the host reads only
+    // the mapped flag after them, which the device writes uncached)
     for (auto &e : ctx->sync_ev) {
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess) {
             gbgpu_finalize(ctx);
             return GBGPU_EDEVICE;
         }
     }
+    for (auto &e : ctx->lag_ev) {
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess) {
+            gbgpu_finalize(ctx);
+            return GBGPU_EDEVICE;
+        }
+    }
+    void *hflag = nullptr;
+    if (hipHostMalloc(&hflag, 256, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+        (void)hipGetLastError();
+        gbgpu_finalize(ctx);
+        return GBGPU_EDEVICE;
+    }
+    ctx->hdone = (volatile int32_t *)hflag;
+    *ctx->hdone = 0;
+    ctx->hscan = (volatile int64_t *)((char *)hflag + 64);
+    *ctx->hscan = 0;
+    void *dflag = nullptr;
+    if (hipHostGetDevicePointer(&dflag, hflag, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        gbgpu_finalize(ctx);
+        return GBGPU_EDEVICE;
+    }
+    ctx->ddone = (int32_t *)dflag;
+    ctx->dscan = (int64_t *)((char *)dflag + 64);
     *out = ctx;
     return GBGPU_OK;
 }
@@ -173,6 +200,9 @@ void gbgpu_finalize(gbgpu_ctx *ctx) {
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->sync_ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : ctx->lag_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->hdone) (void)hipHostFree((void *)ctx->hdone);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

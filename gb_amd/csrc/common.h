@@ -72,7 +72,18 @@ struct gbgpu_ctx {
     double phase_ms[3] = {-1, -1, -1};
     hipEvent_t ev[6] = {};          // timing events of the phases
     hipStream_t side = nullptr;     // second stream of the dense look-ahead
-    hipEvent_t sync_ev[2] = {};     // ordering events between the two streams
+    hipEvent_t sync_ev[1] = {};     // orders the main stream's work before the side's
+    // the side stream's work before the main stream's next step, recorded
+    // in turn; the dense panel loop also keeps at most as many panels queued
+    // ahead of the device as there are lag events and learns that the
+    // echelon form is complete from a flag in mapped host memory, which the
+    // device sets: no stream is drained while panels remain
+    hipEvent_t lag_ev[4] = {};
+    volatile int32_t *hdone = nullptr;  // host view of the flag
+    int32_t *ddone = nullptr;           // device view of the same word
+    // a scan's total, written by the device into the same mapped block
+    volatile int64_t *hscan = nullptr;
+    int64_t *dscan = nullptr;
 };
 
 namespace gbgpu {

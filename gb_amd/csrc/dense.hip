@@ -145,12 +145,10 @@ struct SelShared {
     uint32_t CE[PB][PB + 1];
     int32_t qse[PB];
     int32_t nzw[SEL_W];               // waves with a nonzero on the sub-panel
-    // a chain step's pivot slot, pivot value and column entries, in two
-    // buffers by step parity: a step's owner writes one while the other waves
-    // may still read the last step's
-    uint32_t f[2][PB];
-    int32_t sis[2];
-    uint32_t sav[2];
+    // each chain step's column entries (one per slot), tagged with the
+    // chain and step in the high word: a wave polls its entries until every
+    // lane shows the step's tag instead of meeting the others at a barrier
+    uint64_t f[SW][PB];
     // the move of the selected rows to the front of the candidate list
     int32_t fr[PB], fpos[PB], mvj[PB], mvv[PB];
     // signed base-256 digit planes of a sub-panel's combination (slot x
@@ -368,16 +366,18 @@ __device__ __forceinline__ void sel_apply_mfma(SelShared &sh, const Mod &md, con
 // combination coefficient belong to wave j % SEL_W, the row scale D is kept
 // by every wave.  Step c: the owner of column c finds the pivot (the lowest
 // unselected tile row nonzero there: the columns left of c are zero in
-// every unselected row) and posts it with every slot's entry of column c;
-// after one barrier every wave eliminates its own columns, every other slot
-// x becoming a x - x[c] * pivot row.  Records the pivots as E rows kk, kk +
-// 1, ... and leaves the slots' combinations in C (column c of the
-// sub-panel: coefficient of the row that pivots there, zero if none), D
-// and qs.  Returns the number of pivots (the same in every thread); um
-// loses the selected slots.  X is only read: the apply that follows writes
-// every column.  (Blocks of consecutive columns per wave, one barrier per
-// block, were measured slower: the next block's owner has to take the
-// block's steps before its own.)
+// every unselected row) from every slot's entry of column c, which its
+// owner posts as the step's record; every wave, once each of its lanes
+// reads the record with the step's tag, finds the same pivot and
+// eliminates its own columns, every other slot x becoming a x - x[c] *
+// pivot row.  No barrier per step: the waves run apart, each as far as the
+// records posted so far take it, and the owner of step c + 1 posts it
+// before its other columns of step c.  Records the
+// pivots as E rows kk, kk + 1, ... and leaves the slots' combinations in C
+// (column c of the sub-panel: coefficient of the row that pivots there,
+// zero if none), D and qs.  Returns the number of pivots (the same in
+// every thread); um loses the selected slots.  X is only read: the apply
+// that follows writes every column.
 constexpr int NCW = SW / SEL_W;  // sub-panel columns per wave
 static_assert(SW % SEL_W == 0, "sub-panel columns split evenly over the waves");
 
@@ -427,10 +427,10 @@ __device__ __forceinline__ void chain_step(uint32_t (&v)[NCW], uint32_t (&cc)[NC
 
 template <int P16>
 __device__ __forceinline__ int sel_chain(SelShared &sh, int cb, int bw, uint64_t &um, int kk,
-                                         int32_t c0, uint32_t &pm, const Mod &md, float pinv) {
+                                         int32_t c0, uint32_t &pm, const Mod &md, float pinv,
+                                         uint32_t gen) {
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t p = md.p;
     uint32_t v[NCW], cc[NCW];
     uint32_t dd = 1u;
     bool any = false;
@@ -450,30 +450,56 @@ __device__ __forceinline__ int sel_chain(SelShared &sh, int cb, int bw, uint64_t
 #pragma unroll
     for (int w = 0; w < SEL_W; ++w) anyz |= sh.nzw[w];
     if (!__builtin_amdgcn_readfirstlane(anyz)) return 0;
+    // step cn's record: every slot's entry of column cn after the steps
+    // before it, each lane's one 64-bit word carrying the tag of this chain
+    // and step (never zero; the records start zeroed)
+    const uint32_t g64 = gen << 6;
+    auto post = [&](int cn, uint32_t val) {
+        __hip_atomic_store(&sh.f[cn][lane], ((uint64_t)(g64 + cn + 1) << 32) | val,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    if (wv == 0) post(0, v[0]);
     int np = 0;
 #pragma unroll
     for (int c = 0; c < SW; ++c) {
-        const int own = c % SEL_W, ic = c / SEL_W, buf = c & 1;
+        const int own = c % SEL_W, ic = c / SEL_W;
+        const int nown = (c + 1) % SEL_W, nic = (c + 1) / SEL_W;  // next step's owner
+        uint32_t fv;
         if (wv == own) {
-            // columns past the panel hold zeros: no pivot there
-            const uint64_t ball = um & (uint64_t)__ballot(v[ic] != 0u);
-            const int is = ball ? __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)ball) - 1) : -1;
-            sh.f[buf][lane] = v[ic];
-            if (lane == 0) {
-                sh.sis[buf] = is;
-                sh.sav[buf] = is >= 0 ? (uint32_t)__builtin_amdgcn_readlane((int)v[ic], is) : 0u;
-                if (is >= 0) {
-                    sh.qs[c] = is;
-                    sh.ecol[kk + np] = c0 + cb + c;
-                    sh.eslot[kk + np] = is;
-                }
+            fv = v[ic];  // column c after steps 0 .. c - 1, which this wave took
+        } else {
+            // step c's record, complete once every lane's word has its tag:
+            // each owner posts its step after taking the earlier ones, so
+            // the record comes; the bound only guards against a hang
+            uint64_t rec = 0;
+            for (int spin = 0; spin < (1 << 24); ++spin) {
+                rec = __hip_atomic_load(&sh.f[c][lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (__ballot((uint32_t)(rec >> 32) == g64 + c + 1) == ~0ull) break;
             }
+            fv = (uint32_t)rec;
         }
-        __syncthreads();
-        const int is = __builtin_amdgcn_readfirstlane(sh.sis[buf]);
-        if (is < 0) continue;
-        const uint32_t av = (uint32_t)__builtin_amdgcn_readfirstlane((int)sh.sav[buf]);
-        chain_step<P16>(v, cc, dd, pm, um, np, c, is, av, sh.f[buf][lane], wv, lane, md, pinv);
+        // the pivot: the lowest unselected slot nonzero on column c (the
+        // columns left of c are zero in every unselected slot)
+        const uint64_t ball = um & (uint64_t)__ballot(fv != 0u);
+        if (!ball) {
+            if (c + 1 < SW && wv == nown) post(c + 1, v[nic]);
+            continue;
+        }
+        const int is = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)ball) - 1);
+        const uint32_t av = (uint32_t)__builtin_amdgcn_readlane((int)fv, is);
+        if (c + 1 < SW && wv == nown) {
+            // the next step's column first, so that its record goes out
+            // before this wave's other columns of step c
+            const uint32_t pv = (uint32_t)__builtin_amdgcn_readlane((int)v[nic], is);
+            const uint32_t t = sel_mad<P16>(av, v[nic], md.p - fv, pv, md, pinv);
+            post(c + 1, lane == is ? v[nic] : t);
+        }
+        if (wv == own && lane == 0) {
+            sh.qs[c] = is;
+            sh.ecol[kk + np] = c0 + cb + c;
+            sh.eslot[kk + np] = is;
+        }
+        chain_step<P16>(v, cc, dd, pm, um, np, c, is, av, fv, wv, lane, md, pinv);
     }
 #pragma unroll
     for (int i = 0; i < NCW; ++i) {
@@ -491,10 +517,11 @@ template <int P16, typename T>
 __device__ __forceinline__ void select_body(SelShared &sh, const T *a, int64_t ld,
                                             int32_t n, int32_t c0, int32_t *cand,
                                             int32_t *rowpiv, int32_t *colpiv, Panel *pan,
-                                            ListState *st, Mod md, const Digits &dg) {
+                                            ListState *st, Mod md, const Digits &dg,
+                                            int32_t *hdone) {
     if constexpr (P16 == 2) {
-        if (md.p < (1u << 16)) select_body<1, T>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg);
-        else select_body<0, T>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg);
+        if (md.p < (1u << 16)) select_body<1, T>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg, hdone);
+        else select_body<0, T>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg, hdone);
         return;
     } else {
     const int tid = threadIdx.x, wv = tid >> 6;
@@ -505,7 +532,12 @@ __device__ __forceinline__ void select_body(SelShared &sh, const T *a, int64_t l
     if (r64 == p) r64 = 0;
     const int32_t done = __builtin_amdgcn_readfirstlane(st->done);
     if (done) {
-        if (tid == 0) pan->k = 0;
+        // nothing left: this panel and every later one are empty, which
+        // the host learns from hdone (mapped host memory, if given)
+        if (tid == 0) {
+            pan->k = 0;
+            if (hdone) *(volatile int32_t *)hdone = 1;
+        }
         return;
     }
     const int32_t head = __builtin_amdgcn_readfirstlane(st->head);
@@ -517,6 +549,8 @@ __device__ __forceinline__ void select_body(SelShared &sh, const T *a, int64_t l
     uint64_t pmask = 0;  // slots holding E rows (same value in every thread)
     uint64_t epm = 0;    // panel columns of E's pivots (relative to c0)
     for (int i = tid; i < PB * XW; i += SEL_T) (&sh.X[0][0])[i] = 0u;
+    for (int i = tid; i < SW * PB; i += SEL_T) (&sh.f[0][0])[i] = 0ull;
+    uint32_t gen = 0;  // chains run so far (tags their step records)
     while (pos < count && kk < bw) {
         const int k0 = kk;  // pivots found before this tile
         const uint64_t freem = ~pmask;
@@ -589,7 +623,7 @@ __device__ __forceinline__ void select_body(SelShared &sh, const T *a, int64_t l
         uint64_t um = tmask;
         for (int cb = 0; cb < bw && kk < bw; cb += SW) {
             uint32_t pm;
-            const int np = sel_chain<P16>(sh, cb, bw, um, kk, c0, pm, md, pinv);
+            const int np = sel_chain<P16>(sh, cb, bw, um, kk, c0, pm, md, pinv, gen++);
             __syncthreads();
             if (np > 0) {
                 if (trpend) {
@@ -685,9 +719,9 @@ __global__ __launch_bounds__(SEL_T) void k_panel_select(const T *a, int64_t ld, 
                                                         int32_t c0, int32_t *cand,
                                                         int32_t *rowpiv, int32_t *colpiv,
                                                         Panel *pan, ListState *st, Mod md,
-                                                        Digits dg) {
+                                                        Digits dg, int32_t *hdone) {
     __shared__ SelShared sh;
-    select_body<P16, T>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg);
+    select_body<P16, T>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg, hdone);
 }
 
 // ---------------------------------------------------------------- prune
@@ -1558,22 +1592,37 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
     // workgroup) runs while the rest of this panel's Rp and update fill the
     // other CUs from the side stream.  The chain select - prune - Rp of the
     // next panel's columns - gather - update of those columns - select stays
-    // on the main stream; the side stream's work is the shorter, so the main
-    // stream's wait for it finds it done.  Panel records alternate between
-    // two slots; the candidate list is touched by the select and the prune
-    // only, both on the main stream.
+    // on the main stream, which waits for the side stream before the prune
+    // (the side stream's work is the longer one in the first panels, where
+    // it covers most of the columns).  Panel records alternate
+    // between two slots; the candidate list is touched by the select and the
+    // prune only, both on the main stream.
     hipStream_t x = ctx->side;
     auto select = [&](hipStream_t sx, int32_t cs, Panel *pp) {
         if (d->p < (1u << 16))
-            k_panel_select<1, T><<<1, SEL_T, 0, sx>>>(a, ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg);
+            k_panel_select<1, T><<<1, SEL_T, 0, sx>>>(a, ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg, ctx->ddone);
         else
-            k_panel_select<0, T><<<1, SEL_T, 0, sx>>>(a, ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg);
+            k_panel_select<0, T><<<1, SEL_T, 0, sx>>>(a, ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg, ctx->ddone);
     };
+    // a select that finds no candidate left sets the flag; every entry point
+    // has synchronised its stream before it returned, so no earlier loop's
+    // select writes it after this
+    *ctx->hdone = 0;
     if (m > 0 && n > 0)
         select(s, 0, pan);
-    struct { int32_t done, k; } hs = {0, 0};
+    constexpr int LAG = (int)(sizeof(ctx->lag_ev) / sizeof(ctx->lag_ev[0]));
     int it = 0;
     for (int32_t c0 = 0; c0 < n && m > 0; c0 += PB, ++it) {
+        if (it >= LAG) {
+            // wait for the side stream's work of the panel LAG back (the
+            // device still has the ones after it queued; every panel but the
+            // last records it); stop once a select found nothing left: the
+            // panels from there on are empty
+            if (hipEventSynchronize(ctx->lag_ev[it % LAG]) != hipSuccess) {
+                return check_hip(ctx, hipGetLastError(), "dense panel lag");
+            }
+            if (*ctx->hdone) break;
+        }
         Panel *cur = pan + (it & 1), *nxt = pan + ((it + 1) & 1);
         const int bw = (n - c0) < PB ? (n - c0) : PB;
         const int32_t c1 = c0 + bw;                       // next panel
@@ -1585,8 +1634,14 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
                                                          d->p, dg);
         if (c1 < n) {
             update(s, cur, c0, c1, c2);
-            if (hipEventRecord(ctx->sync_ev[0], s) != hipSuccess ||
-                hipStreamWaitEvent(x, ctx->sync_ev[0], 0) != hipSuccess) {
+            if (hipEventRecord(ctx->sync_ev[0], s) != hipSuccess) {
+                return check_hip(ctx, hipGetLastError(), "dense look-ahead");
+            }
+            // the select reads the columns [c1, c2) of candidate rows, which
+            // the side stream's work does not touch; queued first, it takes
+            // its CU before the side stream's blocks fill the device
+            select(s, c1, nxt);
+            if (hipStreamWaitEvent(x, ctx->sync_ev[0], 0) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
             if (c2 < n) launch_rp<T>(x, false, a, ld, c0, c2, n, cur, rp, ldr, md, r64, dg);
@@ -1612,13 +1667,12 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
                 update(x, cur, c0, c0, c1);
                 update(x, cur, c0, c2, n);
             }
-            if (hipEventRecord(ctx->sync_ev[1], x) != hipSuccess) {
+            // the event that orders the side stream's work before the prune
+            // is also the one the host waits on LAG panels later
+            if (hipEventRecord(ctx->lag_ev[it % LAG], x) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
-            // the select reads the columns [c1, c2) of candidate rows, which
-            // the side stream's work does not touch
-            select(s, c1, nxt);
-            if (hipStreamWaitEvent(s, ctx->sync_ev[1], 0) != hipSuccess) {
+            if (hipStreamWaitEvent(s, ctx->lag_ev[it % LAG], 0) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
             k_prune<T><<<grid_of(m, 4), 256, 0, s>>>(a, ld, n, c1, cand, st, active);
@@ -1627,15 +1681,6 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { return check_hip(ctx, e, "dense panel launch"); }
-        if ((it + 1) % 8 == 0) {
-            // stop once no candidate is left and the selected next panel is empty
-            if (hipMemcpyAsync(&hs.done, &st->done, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-                hipMemcpyAsync(&hs.k, &nxt->k, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-                hipStreamSynchronize(s) != hipSuccess) {
-                return check_hip(ctx, hipGetLastError(), "dense panel sync");
-            }
-            if (hs.done && hs.k == 0) break;
-        }
     }
     (void)hipEventRecord(ctx->ev[3], s);
     return GBGPU_OK;
@@ -1792,7 +1837,8 @@ __global__ __launch_bounds__(SEL_T) void k_batch_select(const DenseDesc *D, int3
     __shared__ SelShared sh;
     const DenseDesc &d = D[blockIdx.x];
     if (c0 >= d.n) return;
-    select_body<2>(sh, d.a, d.ld, d.n, c0, d.cand, d.rowpiv, d.colpiv, d.pan, d.st, d.md, d.dg);
+    select_body<2>(sh, d.a, d.ld, d.n, c0, d.cand, d.rowpiv, d.colpiv, d.pan, d.st, d.md, d.dg,
+                   nullptr);
 }
 
 __global__ __launch_bounds__(256) void k_batch_rp(const DenseDesc *D, const int32_t *pre,

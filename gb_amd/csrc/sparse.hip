@@ -257,24 +257,24 @@ __global__ __launch_bounds__(64) void k_gbucket_fill(int32_t ncl, int32_t ngroup
                                                      int32_t wshift, int32_t acc_shift,
                                                      const int64_t *seg, const uint2 *bucket,
                                                      const int64_t *gseg, uint2 *gbucket) {
+    // one wave per (group, window): block (g, w)
     const int lane = threadIdx.x;
     const int32_t g = blockIdx.x;
+    const int w = blockIdx.y;
     const int32_t c = g * 64 + lane;
     const uint32_t tag = ((uint32_t)(g & 3) * 64u + (uint32_t)lane) << 24;
-    for (int w = 0; w < nwin; ++w) {
-        int64_t sb = 0;
-        int len = 0;
-        if (c < ncl) {
-            sb = seg[(int64_t)w * ncl + c];
-            len = (int)(seg[(int64_t)(w + 1) * ncl + c] - sb);
-        }
-        const int incl = wave_incl_scan(len);
-        const int64_t at = gseg[(int64_t)w * ngroups + g] + (incl - len);
-        const uint32_t w0 = (uint32_t)w << wshift;
-        for (int k = 0; k < len; ++k) {
-            const uint2 e = bucket[sb + k];
-            gbucket[at + k] = make_uint2(tag | ((e.x - w0) << acc_shift), e.y);
-        }
+    int64_t sb = 0;
+    int len = 0;
+    if (c < ncl) {
+        sb = seg[(int64_t)w * ncl + c];
+        len = (int)(seg[(int64_t)(w + 1) * ncl + c] - sb);
+    }
+    const int incl = wave_incl_scan(len);
+    const int64_t at = gseg[(int64_t)w * ngroups + g] + (incl - len);
+    const uint32_t w0 = (uint32_t)w << wshift;
+    for (int k = 0; k < len; ++k) {
+        const uint2 e = bucket[sb + k];
+        gbucket[at + k] = make_uint2(tag | ((e.x - w0) << acc_shift), e.y);
     }
 }
 
@@ -293,14 +293,14 @@ int prep_group_buckets(gbgpu_mat *m, int acc_shift) {
     GBGPU_HIP(ctx, hipMemsetAsync(gcnt, 0, ((size_t)nseg + 1) * sizeof(int64_t), s));
     if (ngroups > 0) k_gbucket_count<<<ngroups, 64, 0, s>>>(m->ncl, ngroups, m->nwin, m->seg, gcnt);
     int64_t total = 0;
-    if ((rc = exclusive_scan_i64(ctx, gcnt, m->gseg, nseg + 1, &total))) { dfree(gcnt); return rc; }
-    if ((rc = dalloc(ctx, &m->gbucket, (size_t)total))) { dfree(gcnt); return rc; }
+    rc = exclusive_scan_i64(ctx, gcnt, m->gseg, nseg + 1, &total);
+    dfree(gcnt);  // the scan has waited for the stream
+    if (rc) return rc;
+    if ((rc = dalloc(ctx, &m->gbucket, (size_t)total))) return rc;
     if (ngroups > 0)
-        k_gbucket_fill<<<ngroups, 64, 0, s>>>(m->ncl, ngroups, m->nwin, m->wshift, acc_shift, m->seg,
-                                              m->bucket, m->gseg, m->gbucket);
+        k_gbucket_fill<<<dim3((unsigned)ngroups, (unsigned)m->nwin), 64, 0, s>>>(
+            m->ncl, ngroups, m->nwin, m->wshift, acc_shift, m->seg, m->bucket, m->gseg, m->gbucket);
     GBGPU_HIP(ctx, hipGetLastError());
-    GBGPU_HIP(ctx, hipStreamSynchronize(s));
-    dfree(gcnt);
     return GBGPU_OK;
 }
 
@@ -519,6 +519,7 @@ __global__ __launch_bounds__(64) void k_sparse_plain(SpArgs a) {
 
 // one wave per 64-column group: T from the pivots' in-group entries, then
 // T^-1 row by row (lane i owns row i: Y[i][j] = -sum_{i<=k<j} Y[i][k] T[k][j])
+template <bool BIG>
 __global__ __launch_bounds__(64) void k_group_inverse(int32_t ncl, int32_t wshift,
                                                       const uint32_t *pbits, const int64_t *seg,
                                                       const uint2 *bucket, Mod md, uint32_t *tinv) {
@@ -544,9 +545,12 @@ __global__ __launch_bounds__(64) void k_group_inverse(int32_t ncl, int32_t wshif
         }
     }
     __syncthreads();
+    // row lane of the inverse, left to right; p < 2^17: the products (below
+    // 2^34, at most 63 of them) are summed unreduced
     for (int j = lane + 1; j < 64; ++j) {
         uint64_t acc = 0;
-        for (int k = lane; k < j; ++k) acc += (uint64_t)md.mul(Y[lane][k], T[k][j]);
+        for (int k = lane; k < j; ++k)
+            acc += BIG ? (uint64_t)md.mul(Y[lane][k], T[k][j]) : (uint64_t)Y[lane][k] * T[k][j];
         const uint32_t r = md.reduce(acc);
         Y[lane][j] = r ? md.p - r : 0u;
     }
@@ -916,6 +920,17 @@ void sparse_args(const gbgpu_mat *m, SpArgs *a) {
     a->md = Mod::make(m->p);
 }
 
+// the inverses of the groups' pivot blocks (they need the per-pivot layout
+// only) on stream sx
+static void launch_group_inverse(gbgpu_mat *m, uint32_t *tinv, hipStream_t sx) {
+    const int32_t ngroups = (m->ncl + 63) / 64;
+    const Mod md = Mod::make(m->p);
+    if (m->p >= (1u << 17))
+        k_group_inverse<true><<<ngroups, 64, 0, sx>>>(m->ncl, m->wshift, m->pbits, m->seg, m->bucket, md, tinv);
+    else
+        k_group_inverse<false><<<ngroups, 64, 0, sx>>>(m->ncl, m->wshift, m->pbits, m->seg, m->bucket, md, tinv);
+}
+
 int prep_blocked(gbgpu_mat *m, uint32_t **tinv) {
     gbgpu_ctx *ctx = m->ctx;
     int rc;
@@ -924,8 +939,7 @@ int prep_blocked(gbgpu_mat *m, uint32_t **tinv) {
     const int32_t ngroups = (m->ncl + 63) / 64;
     if (ngroups > 0) {
         if ((rc = dalloc(ctx, tinv, (size_t)ngroups * 4096))) return rc;
-        k_group_inverse<<<ngroups, 64, 0, ctx->stream>>>(m->ncl, m->wshift, m->pbits, m->seg,
-                                                          m->bucket, Mod::make(m->p), *tinv);
+        launch_group_inverse(m, *tinv, ctx->stream);
         GBGPU_HIP(ctx, hipGetLastError());
     }
     return GBGPU_OK;
@@ -998,12 +1012,29 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
     const bool dense = blocked && dense_route_pays(m);
     uint32_t *tinv = nullptr;
     const int32_t ngroups = (m->ncl + 63) / 64;
-    if (blocked && !dense && (rc = prep_group_buckets(m, acc32 ? 2 : 3))) return rc;
     if (blocked && ngroups > 0) {
         if ((rc = dalloc(ctx, &tinv, (size_t)ngroups * 4096))) return rc;
-        k_group_inverse<<<ngroups, 64, 0, s>>>(m->ncl, m->wshift, m->pbits, m->seg, m->bucket,
-                                               Mod::make(m->p), tinv);
-        GBGPU_HIP(ctx, hipGetLastError());
+        if (!dense) {
+            // the inverses run on the side stream while the (window, group)
+            // layout is built on the main one
+            GBGPU_HIP(ctx, hipEventRecord(ctx->sync_ev[0], s));
+            GBGPU_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->sync_ev[0], 0));
+            launch_group_inverse(m, tinv, ctx->side);
+            GBGPU_HIP(ctx, hipGetLastError());
+            GBGPU_HIP(ctx, hipEventRecord(ctx->lag_ev[0], ctx->side));
+        } else {
+            launch_group_inverse(m, tinv, s);
+            GBGPU_HIP(ctx, hipGetLastError());
+        }
+    }
+    if (blocked && !dense) {
+        rc = prep_group_buckets(m, acc32 ? 2 : 3);
+        if (rc) {
+            (void)hipStreamSynchronize(ctx->side);  // the inverses still write tinv
+            dfree(tinv);
+            return rc;
+        }
+        if (ngroups > 0) GBGPU_HIP(ctx, hipStreamWaitEvent(s, ctx->lag_ev[0], 0));
     }
     // waves in flight: LDS of one wave is W*8 bytes + staging, 160 KB per CU
     const int W = 1 << m->wshift;
@@ -1075,46 +1106,40 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
 
 // ----------------------------------------------------------- scans
 
-int exclusive_scan_i64(gbgpu_ctx *ctx, const int64_t *in, int64_t *out, int64_t n, int64_t *total) {
+// in[n - 1] + out[n - 1] (the sum of an exclusive scan's input) into the
+// context's mapped host word
+template <typename I>
+__global__ void k_scan_total(const I *in, const I *out, int64_t n, int64_t *dst) {
+    *(volatile int64_t *)dst = (int64_t)in[n - 1] + (int64_t)out[n - 1];
+}
+
+// out = exclusive prefix sums of in, *total = their sum: one wait for the
+// stream, the total read from mapped host memory
+template <typename I>
+static int exclusive_scan(gbgpu_ctx *ctx, const I *in, I *out, int64_t n, int64_t *total) {
     *total = 0;
     if (n <= 0) return GBGPU_OK;
     hipStream_t s = ctx->stream;
-    int64_t last_in = 0;
-    GBGPU_HIP(ctx, hipMemcpyAsync(&last_in, in + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    GBGPU_HIP(ctx, hipStreamSynchronize(s));
     size_t tmp = 0;
     GBGPU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, s));
     void *d_tmp = nullptr;
     int rc;
     if ((rc = dalloc(ctx, (char **)&d_tmp, tmp))) return rc;
     GBGPU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp, in, out, (int)n, s));
-    int64_t last_out = 0;
-    GBGPU_HIP(ctx, hipMemcpyAsync(&last_out, out + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    k_scan_total<I><<<1, 1, 0, s>>>(in, out, n, ctx->dscan);
+    GBGPU_HIP(ctx, hipGetLastError());
     GBGPU_HIP(ctx, hipStreamSynchronize(s));
     dfree(d_tmp);
-    *total = last_out + last_in;
+    *total = *ctx->hscan;
     return GBGPU_OK;
 }
 
+int exclusive_scan_i64(gbgpu_ctx *ctx, const int64_t *in, int64_t *out, int64_t n, int64_t *total) {
+    return exclusive_scan<int64_t>(ctx, in, out, n, total);
+}
+
 int exclusive_scan_i32(gbgpu_ctx *ctx, const int32_t *in, int32_t *out, int64_t n, int64_t *total) {
-    *total = 0;
-    if (n <= 0) return GBGPU_OK;
-    hipStream_t s = ctx->stream;
-    int32_t last_in = 0;
-    GBGPU_HIP(ctx, hipMemcpyAsync(&last_in, in + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    GBGPU_HIP(ctx, hipStreamSynchronize(s));
-    size_t tmp = 0;
-    GBGPU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, s));
-    void *d_tmp = nullptr;
-    int rc;
-    if ((rc = dalloc(ctx, (char **)&d_tmp, tmp))) return rc;
-    GBGPU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp, in, out, (int)n, s));
-    int32_t last_out = 0;
-    GBGPU_HIP(ctx, hipMemcpyAsync(&last_out, out + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    GBGPU_HIP(ctx, hipStreamSynchronize(s));
-    dfree(d_tmp);
-    *total = (int64_t)last_out + last_in;
-    return GBGPU_OK;
+    return exclusive_scan<int32_t>(ctx, in, out, n, total);
 }
 
 }  // namespace gbgpu
