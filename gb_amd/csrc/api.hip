@@ -196,6 +196,9 @@ int gbgpu_init(int device, gbgpu_ctx **out) {
 void gbgpu_finalize(gbgpu_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    flush_later(ctx);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->sync_ev)

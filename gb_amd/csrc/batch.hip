@@ -143,6 +143,7 @@ int layout_sparse(gbgpu_batch *b, int form, int wshift) {
     GBGPU_HIP(ctx, hipMemcpyAsync(b->d_rowoff, rowoff.data(), rowoff.size() * sizeof(int32_t),
                                   hipMemcpyHostToDevice, ctx->stream));
     GBGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    flush_later(ctx);
     b->wshift = wshift;
     return GBGPU_OK;
 }
@@ -266,6 +267,7 @@ int layout_dense(gbgpu_batch *b, int form) {
         GBGPU_HIP(ctx, hipMemcpyAsync(b->d_tab, b->htab.data(), b->htab.size() * sizeof(int32_t),
                                       hipMemcpyHostToDevice, ctx->stream));
     GBGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    flush_later(ctx);
     return GBGPU_OK;
 }
 

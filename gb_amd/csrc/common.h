@@ -15,6 +15,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/gbgpu.h"
 
@@ -84,6 +85,9 @@ struct gbgpu_ctx {
     // a scan's total, written by the device into the same mapped block
     volatile int64_t *hscan = nullptr;
     int64_t *dscan = nullptr;
+    // buffers freed while queued work of this context may still read them:
+    // they go back to the pool at the next wait for the stream
+    std::vector<void *> later;
 };
 
 namespace gbgpu {
@@ -120,11 +124,28 @@ void dfree(T *&p) {
     p = nullptr;
 }
 
-// exclusive prefix sum of n int64 values into out (may alias), returns the
-// total on the host; scratch from hipcub
+// free p once the context's stream has passed its last use: a buffer back
+// in the pool at once could go to another context's thread while this
+// stream's queued work still reads it
+template <class T>
+void dfree_later(gbgpu_ctx *ctx, T *&p) {
+    if (p) ctx->later.push_back((void *)p);
+    p = nullptr;
+}
+
+// after a wait for the stream: the deferred buffers are idle
+inline void flush_later(gbgpu_ctx *ctx) {
+    for (void *q : ctx->later) pool_free(q);
+    ctx->later.clear();
+}
+
+// exclusive prefix sum of n values into out (not aliasing in); with total
+// given, waits for the stream once and returns the sum of the values in
+// *total, and a device word dflag (if given) in *hflag; with total null it
+// waits for nothing.  Scratch from hipcub
 int exclusive_scan_i64(gbgpu_ctx *ctx, const int64_t *in, int64_t *out, int64_t n,
-                       int64_t *total);
+                       int64_t *total, const int32_t *dflag = nullptr, int32_t *hflag = nullptr);
 int exclusive_scan_i32(gbgpu_ctx *ctx, const int32_t *in, int32_t *out, int64_t n,
-                       int64_t *total);
+                       int64_t *total, const int32_t *dflag = nullptr, int32_t *hflag = nullptr);
 
 }  // namespace gbgpu

@@ -37,6 +37,16 @@
 
 namespace gbgpu {
 
+// in[n - 1] + out[n - 1] (the sum of an exclusive scan's input) and a flag
+// word into the context's mapped host block
+template <typename I>
+__global__ void k_scan_total(const I *in, const I *out, int64_t n, const int32_t *flag,
+                             int64_t *dst) {
+    volatile int64_t *d = dst;
+    d[0] = n > 0 ? (int64_t)in[n - 1] + (int64_t)out[n - 1] : 0;
+    d[1] = flag ? *flag : 0;
+}
+
 // ------------------------------------------------------------ split
 
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
@@ -57,10 +67,15 @@ __global__ void k_leads(int32_t nr, const int64_t *rp, const int32_t *ci, int32_
     }
 }
 
+// to-reduce flags; a row that breaks the contract (k_leads reports it) is
+// not read past its end nor its lead used as an index
 __global__ void k_nonpiv(int32_t nr, const int64_t *rp, const int32_t *ci, const int32_t *first,
-                         int32_t *flag) {
-    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += gridDim.x * blockDim.x)
-        flag[i] = first[ci[rp[i]]] != i;
+                         int32_t lim, int32_t *flag) {
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += gridDim.x * blockDim.x) {
+        const int64_t b = rp[i], e = rp[i + 1];
+        const int32_t lead = e > b ? ci[b] : -1;
+        flag[i] = (lead >= 0 && lead < lim) ? first[lead] != i : 0;
+    }
 }
 
 // the to-reduce list is filled from its end (upivs[nrl - j],
@@ -93,19 +108,18 @@ int prep_split(gbgpu_mat *m, int32_t lim) {
     k_fill_i32<<<grid_for(m->nc, 256), 256, 0, s>>>(m->first, m->nc, INT32_MAX);
     k_fill_i32<<<1, 64, 0, s>>>(err, 1, 0);
     k_leads<<<grid_for(m->nr, 256), 256, 0, s>>>(m->nr, m->rp, m->ci, lim, m->first, err);
+    k_nonpiv<<<grid_for(m->nr, 256), 256, 0, s>>>(m->nr, m->rp, m->ci, m->first, lim, flag);
+    // one wait: the count of rows to reduce and the contract flags
+    int64_t nrl = 0;
     int32_t herr = 0;
-    GBGPU_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    GBGPU_HIP(ctx, hipStreamSynchronize(s));
+    if ((rc = exclusive_scan_i32(ctx, flag, rank, m->nr, &nrl, err, &herr))) {
+        dfree(err); dfree(flag); dfree(rank); return rc;
+    }
     if (herr) {
         dfree(err); dfree(flag); dfree(rank);
         return set_error(ctx, GBGPU_ECONTRACT,
                          (herr & 1) ? "empty row in matrix"
                                     : "row lead outside the known-pivot columns");
-    }
-    k_nonpiv<<<grid_for(m->nr, 256), 256, 0, s>>>(m->nr, m->rp, m->ci, m->first, flag);
-    int64_t nrl = 0;
-    if ((rc = exclusive_scan_i32(ctx, flag, rank, m->nr, &nrl))) {
-        dfree(err); dfree(flag); dfree(rank); return rc;
     }
     m->nrl = (int32_t)nrl;
     if ((rc = dalloc(ctx, &m->upiv, (size_t)nrl))) {
@@ -113,8 +127,7 @@ int prep_split(gbgpu_mat *m, int32_t lim) {
     }
     k_upiv<<<grid_for(m->nr, 256), 256, 0, s>>>(m->nr, flag, rank, m->nrl, m->upiv);
     GBGPU_HIP(ctx, hipGetLastError());
-    GBGPU_HIP(ctx, hipStreamSynchronize(s));
-    dfree(err); dfree(flag); dfree(rank);
+    dfree_later(ctx, err); dfree_later(ctx, flag); dfree_later(ctx, rank);
     m->split_lim = lim;
     return GBGPU_OK;
 }
@@ -230,9 +243,8 @@ int prep_buckets(gbgpu_mat *m, int force_wshift) {
                                   pbase, m->seg, m->bucket);
     }
     GBGPU_HIP(ctx, hipGetLastError());
-    GBGPU_HIP(ctx, hipStreamSynchronize(s));
-    dfree(plen);
-    dfree(pbase);
+    dfree_later(ctx, plen);
+    dfree_later(ctx, pbase);
     return GBGPU_OK;
 }
 
@@ -292,11 +304,12 @@ int prep_group_buckets(gbgpu_mat *m, int acc_shift) {
     if ((rc = dalloc(ctx, &m->gseg, (size_t)nseg + 1))) { dfree(gcnt); return rc; }
     GBGPU_HIP(ctx, hipMemsetAsync(gcnt, 0, ((size_t)nseg + 1) * sizeof(int64_t), s));
     if (ngroups > 0) k_gbucket_count<<<ngroups, 64, 0, s>>>(m->ncl, ngroups, m->nwin, m->seg, gcnt);
-    int64_t total = 0;
-    rc = exclusive_scan_i64(ctx, gcnt, m->gseg, nseg + 1, &total);
-    dfree(gcnt);  // the scan has waited for the stream
+    // the (window, group) layout holds the per-pivot layout's entries in
+    // another order: as many (no wait for the scan's total)
+    rc = exclusive_scan_i64(ctx, gcnt, m->gseg, nseg + 1, nullptr);
+    dfree_later(ctx, gcnt);
     if (rc) return rc;
-    if ((rc = dalloc(ctx, &m->gbucket, (size_t)total))) return rc;
+    if ((rc = dalloc(ctx, &m->gbucket, (size_t)m->nbucket))) return rc;
     if (ngroups > 0)
         k_gbucket_fill<<<dim3((unsigned)ngroups, (unsigned)m->nwin), 64, 0, s>>>(
             m->ncl, ngroups, m->nwin, m->wshift, acc_shift, m->seg, m->bucket, m->gseg, m->gbucket);
@@ -1083,15 +1096,20 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
     // compact the nonzero rows, keeping the order (la_ff_32.c:1166-1171)
     int32_t *rank = nullptr;
     if ((rc = dalloc(ctx, &rank, (size_t)m->nrl))) { dfree(xs); dfree(queue); return rc; }
-    int64_t nd = 0;
-    if ((rc = exclusive_scan_i32(ctx, m->dnz, rank, m->nrl, &nd))) {
+    // (the list sized for every row to reduce, so that the count is read in
+    // the call's one final wait)
+    if ((rc = exclusive_scan_i32(ctx, m->dnz, rank, m->nrl, nullptr)) ||
+        (rc = dalloc(ctx, &m->dlist, (size_t)m->nrl))) {
         dfree(xs); dfree(queue); dfree(rank); return rc;
     }
-    m->nd = (int32_t)nd;
-    if ((rc = dalloc(ctx, &m->dlist, (size_t)nd))) { dfree(xs); dfree(queue); dfree(rank); return rc; }
-    if (m->nrl > 0) k_compact<<<grid_for(m->nrl, 256), 256, 0, s>>>(m->nrl, m->dnz, rank, m->dlist);
+    if (m->nrl > 0) {
+        k_compact<<<grid_for(m->nrl, 256), 256, 0, s>>>(m->nrl, m->dnz, rank, m->dlist);
+        k_scan_total<int32_t><<<1, 1, 0, s>>>(m->dnz, rank, m->nrl, nullptr, ctx->dscan);
+    }
     GBGPU_HIP(ctx, hipGetLastError());
     GBGPU_HIP(ctx, hipStreamSynchronize(s));
+    flush_later(ctx);
+    m->nd = m->nrl > 0 ? (int32_t)ctx->hscan[0] : 0;
     float ms = 0;
     if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->phase_ms[0] = ms;
     if (work) {
@@ -1106,40 +1124,41 @@ int run_sparse_pass(gbgpu_mat *m, uint64_t *work, int form) {
 
 // ----------------------------------------------------------- scans
 
-// in[n - 1] + out[n - 1] (the sum of an exclusive scan's input) into the
-// context's mapped host word
 template <typename I>
-__global__ void k_scan_total(const I *in, const I *out, int64_t n, int64_t *dst) {
-    *(volatile int64_t *)dst = (int64_t)in[n - 1] + (int64_t)out[n - 1];
-}
-
-// out = exclusive prefix sums of in, *total = their sum: one wait for the
-// stream, the total read from mapped host memory
-template <typename I>
-static int exclusive_scan(gbgpu_ctx *ctx, const I *in, I *out, int64_t n, int64_t *total) {
-    *total = 0;
-    if (n <= 0) return GBGPU_OK;
+static int exclusive_scan(gbgpu_ctx *ctx, const I *in, I *out, int64_t n, int64_t *total,
+                          const int32_t *dflag, int32_t *hflag) {
     hipStream_t s = ctx->stream;
-    size_t tmp = 0;
-    GBGPU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, s));
-    void *d_tmp = nullptr;
-    int rc;
-    if ((rc = dalloc(ctx, (char **)&d_tmp, tmp))) return rc;
-    GBGPU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp, in, out, (int)n, s));
-    k_scan_total<I><<<1, 1, 0, s>>>(in, out, n, ctx->dscan);
-    GBGPU_HIP(ctx, hipGetLastError());
-    GBGPU_HIP(ctx, hipStreamSynchronize(s));
-    dfree(d_tmp);
-    *total = *ctx->hscan;
+    if (total) *total = 0;
+    if (hflag) *hflag = 0;
+    if (n > 0) {
+        size_t tmp = 0;
+        GBGPU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, s));
+        void *d_tmp = nullptr;
+        int rc;
+        if ((rc = dalloc(ctx, (char **)&d_tmp, tmp))) return rc;
+        GBGPU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp, in, out, (int)n, s));
+        dfree_later(ctx, d_tmp);
+    }
+    if (!total) return GBGPU_OK;
+    if (n > 0 || dflag) {
+        k_scan_total<I><<<1, 1, 0, s>>>(n > 0 ? in : nullptr, n > 0 ? out : nullptr, n, dflag, ctx->dscan);
+        GBGPU_HIP(ctx, hipGetLastError());
+        GBGPU_HIP(ctx, hipStreamSynchronize(s));
+        flush_later(ctx);
+        if (n > 0) *total = ctx->hscan[0];
+        if (hflag) *hflag = (int32_t)ctx->hscan[1];
+    }
     return GBGPU_OK;
 }
 
-int exclusive_scan_i64(gbgpu_ctx *ctx, const int64_t *in, int64_t *out, int64_t n, int64_t *total) {
-    return exclusive_scan<int64_t>(ctx, in, out, n, total);
+int exclusive_scan_i64(gbgpu_ctx *ctx, const int64_t *in, int64_t *out, int64_t n,
+                       int64_t *total, const int32_t *dflag, int32_t *hflag) {
+    return exclusive_scan<int64_t>(ctx, in, out, n, total, dflag, hflag);
 }
 
-int exclusive_scan_i32(gbgpu_ctx *ctx, const int32_t *in, int32_t *out, int64_t n, int64_t *total) {
-    return exclusive_scan<int32_t>(ctx, in, out, n, total);
+int exclusive_scan_i32(gbgpu_ctx *ctx, const int32_t *in, int32_t *out, int64_t n,
+                       int64_t *total, const int32_t *dflag, int32_t *hflag) {
+    return exclusive_scan<int32_t>(ctx, in, out, n, total, dflag, hflag);
 }
 
 }  // namespace gbgpu
