@@ -19,10 +19,12 @@
 //     synchronisation inside.
 //
 // The launch parameters of the sparse pass (window width 2^10 .. 2^13
-// columns and waves per CU) are chosen by timing the candidates on the batch
-// itself the first time a batch of that shape runs in a form; the choice is
-// kept per process.
+// columns and waves per CU) and the MFMA form's dense update tile (64 x 64
+// tiles, or strips of 256 or 512 columns of 64 or 128 rows) are chosen by
+// timing the candidates on the batch itself the first time a batch of that
+// shape runs in a form; the choice is kept per process.
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -39,6 +41,7 @@ struct gbgpu_batch {
     int wshift = 0, waves_per_cu = 0;
     bool big = false;               // some prime needs reduced products
     int nd = 0;                     // digits of the MFMA form's dense update
+    int ucols = 64, urows = 64;     // its blocks' tile (columns x rows)
     // sparse pass
     gbgpu::BlockedArgs *d_args = nullptr;
     int32_t *d_rowoff = nullptr;
@@ -79,7 +82,11 @@ struct TuneKey {
 // batches with fewer rows to reduce than this are not tuned
 constexpr int64_t SMALL_BATCH_ROWS = 20000;
 std::mutex g_tune_mu;
-std::map<TuneKey, std::pair<int, int>> g_tuned;  // -> (wshift, waves per CU)
+struct Tuned {
+    int wshift, waves_per_cu;  // sparse pass
+    int ucols, urows;          // dense update tile
+};
+std::map<TuneKey, Tuned> g_tuned;
 
 __global__ void k_zero_i32(int32_t *p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
@@ -209,6 +216,25 @@ int tune(gbgpu_batch *b, int form, int *best_ws, int *best_wpc) {
     return GBGPU_OK;
 }
 
+// every step's block counts for the update tile b->ucols x b->urows
+int layout_counts(gbgpu_batch *b, int form) {
+    gbgpu_ctx *ctx = b->ctx;
+    const int32_t nmat = (int32_t)b->mats.size();
+    int rc;
+    b->htab.assign((size_t)b->nsteps * 4 * (nmat + 1), 0);
+    for (int32_t k = 0; k < b->nsteps; ++k)
+        batch_dense_counts(b->hd.data(), nmat, k, form, b->ucols, b->urows,
+                           b->htab.data() + (size_t)k * 4 * (nmat + 1));
+    dfree(b->d_tab);
+    if ((rc = dalloc(ctx, &b->d_tab, b->htab.size() ? b->htab.size() : 1))) return rc;
+    if (!b->htab.empty())
+        GBGPU_HIP(ctx, hipMemcpyAsync(b->d_tab, b->htab.data(), b->htab.size() * sizeof(int32_t),
+                                      hipMemcpyHostToDevice, ctx->stream));
+    GBGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    flush_later(ctx);
+    return GBGPU_OK;
+}
+
 int layout_dense(gbgpu_batch *b, int form) {
     gbgpu_ctx *ctx = b->ctx;
     int rc;
@@ -255,20 +281,10 @@ int layout_dense(gbgpu_batch *b, int form) {
         if (mr > 0) max_n = std::max(max_n, ncr);
     }
     b->nsteps = (max_n + 63) / 64;
-    b->htab.assign((size_t)b->nsteps * 4 * (nmat + 1), 0);
-    for (int32_t k = 0; k < b->nsteps; ++k)
-        batch_dense_counts(b->hd.data(), nmat, k, form, b->nd, b->htab.data() + (size_t)k * 4 * (nmat + 1));
-    if ((rc = dalloc(ctx, &b->d_dense, (size_t)nmat)) ||
-        (rc = dalloc(ctx, &b->d_tab, b->htab.size() ? b->htab.size() : 1)))
-        return rc;
+    if ((rc = dalloc(ctx, &b->d_dense, (size_t)nmat))) return rc;
     GBGPU_HIP(ctx, hipMemcpyAsync(b->d_dense, b->hd.data(), nmat * sizeof(DenseDesc),
                                   hipMemcpyHostToDevice, ctx->stream));
-    if (!b->htab.empty())
-        GBGPU_HIP(ctx, hipMemcpyAsync(b->d_tab, b->htab.data(), b->htab.size() * sizeof(int32_t),
-                                      hipMemcpyHostToDevice, ctx->stream));
-    GBGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    flush_later(ctx);
-    return GBGPU_OK;
+    return layout_counts(b, form);
 }
 
 // everything one run launches, in stream order
@@ -280,9 +296,49 @@ int enqueue_run(gbgpu_batch *b, int form) {
     if ((rc = batch_dense_init(ctx, b->d_dense, nmat))) return rc;
     for (int32_t k = 0; k < b->nsteps; ++k)
         if ((rc = batch_dense_step(ctx, b->d_dense, nmat, b->d_tab, b->htab.data(), k, form, b->big,
-                                   b->nd)))
+                                   b->nd, b->ucols, b->urows)))
             return rc;
     return GBGPU_OK;
+}
+
+// times the MFMA form's dense update tiles on the batch itself: each
+// candidate's run of the dense phase after a fresh sparse pass (the phase
+// works on D' in place), the second of two; keeps the fastest
+int tune_dense(gbgpu_batch *b, int form) {
+    gbgpu_ctx *ctx = b->ctx;
+    const int32_t nmat = (int32_t)b->mats.size();
+    static const int cand[5][2] = {{64, 64}, {256, 64}, {512, 64}, {256, 128}, {512, 128}};
+    float best = -1;
+    int bc = b->ucols, br = b->urows, rc;
+    for (const auto &c : cand) {
+        if (!batch_update_shape_ok(b->nd, c[0], c[1])) continue;
+        b->ucols = c[0];
+        b->urows = c[1];
+        if ((rc = layout_counts(b, form))) return rc;
+        float t = 0;
+        for (int rep = 0; rep < 2; ++rep) {
+            if ((rc = launch_sparse(b, form))) return rc;
+            GBGPU_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+            if ((rc = batch_dense_init(ctx, b->d_dense, nmat))) return rc;
+            for (int32_t k = 0; k < b->nsteps; ++k)
+                if ((rc = batch_dense_step(ctx, b->d_dense, nmat, b->d_tab, b->htab.data(), k, form,
+                                           b->big, b->nd, b->ucols, b->urows)))
+                    return rc;
+            GBGPU_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+            GBGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            float ms = 0;
+            GBGPU_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+            if (rep == 1) t = ms;
+        }
+        if (best < 0 || t < best) {
+            best = t;
+            bc = c[0];
+            br = c[1];
+        }
+    }
+    b->ucols = bc;
+    b->urows = br;
+    return layout_counts(b, form);
 }
 
 int prepare(gbgpu_batch *b, int form) {
@@ -310,31 +366,56 @@ int prepare(gbgpu_batch *b, int form) {
         key.nnz += m->nnz;
         key.max_nc = std::max(key.max_nc, m->nc);
     }
-    int ws = 0, wpc = 0;
+    Tuned tp{0, 0, 0, 0};
+    bool have = false;
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tuned.find(key);
         if (it != g_tuned.end()) {
-            ws = it->second.first;
-            wpc = it->second.second;
+            tp = it->second;
+            have = true;
         }
     }
-    if (!ws && key.rows < SMALL_BATCH_ROWS) {
+    const bool small = key.rows < SMALL_BATCH_ROWS;
+    if (!have && small) {
         // a small batch (the images of a multimodular round, say) runs in
         // less time than timing the candidates would take: the smallest
         // window that covers the widest matrix in 63, at full occupancy
-        ws = 10;
-        while ((((int64_t)key.max_nc + (1 << ws) - 1) >> ws) > 63 && ws < 13) ++ws;
-        wpc = max_waves(ws, form);
+        tp.wshift = 10;
+        while ((((int64_t)key.max_nc + (1 << tp.wshift) - 1) >> tp.wshift) > 63 && tp.wshift < 13)
+            ++tp.wshift;
+        tp.waves_per_cu = max_waves(tp.wshift, form);
     }
-    if (!ws) {
-        if ((rc = tune(b, form, &ws, &wpc))) return rc;
-        std::lock_guard<std::mutex> g(g_tune_mu);
-        g_tuned[key] = {ws, wpc};
+    if (!have && !small && (rc = tune(b, form, &tp.wshift, &tp.waves_per_cu))) return rc;
+    if ((rc = layout_sparse(b, form, tp.wshift))) return rc;
+    if ((rc = set_grid(b, tp.waves_per_cu))) return rc;
+    // the dense update tile: the one kept for this shape, else 512-column
+    // strips for three and four digits and 64 x 64 tiles otherwise, timed
+    // against the other tiles on a batch large enough
+    int nd = 0;
+    if (form == GBGPU_FORM_MFMA)
+        for (auto *m : b->mats) nd = std::max(nd, digits_for(m->p));
+    b->ucols = have ? tp.ucols : (nd >= 3 ? 512 : 64);
+    b->urows = have ? tp.urows : 64;
+    // GBGPU_BATCH_TILE=cols,rows: that tile, untimed (one of the candidates)
+    const char *tl = getenv("GBGPU_BATCH_TILE");
+    bool forced = false;
+    if (tl && tl[0] && form == GBGPU_FORM_MFMA) {
+        int c = 0, r = 0;
+        if (sscanf(tl, "%d,%d", &c, &r) != 2 || !batch_update_shape_ok(nd, c, r))
+            return set_error(ctx, GBGPU_EINVAL, std::string("GBGPU_BATCH_TILE: no update tile ") + tl);
+        b->ucols = c;
+        b->urows = r;
+        forced = true;
     }
-    if ((rc = layout_sparse(b, form, ws))) return rc;
-    if ((rc = set_grid(b, wpc))) return rc;
     if ((rc = layout_dense(b, form))) return rc;
+    if (!have && !small && !forced && form == GBGPU_FORM_MFMA && (rc = tune_dense(b, form))) return rc;
+    if (!have && !forced) {
+        tp.ucols = b->ucols;
+        tp.urows = b->urows;
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tuned[key] = tp;
+    }
     // GBGPU_BATCH_EAGER=1: the same launches straight on the stream, each
     // run (for tracing tools that follow stream launches one by one)
     const char *eg = getenv("GBGPU_BATCH_EAGER");
@@ -447,9 +528,9 @@ int gbgpu_batch_rows_download(gbgpu_ctx *ctx, gbgpu_batch *b, int32_t i, gbgpu_r
 
 int gbgpu_batch_info(gbgpu_batch *b, int32_t *out, int32_t nout) {
     if (!b || !out) return GBGPU_EINVAL;
-    const int32_t v[6] = {(int32_t)b->mats.size(), b->wshift, b->waves_per_cu, b->grid, b->replays,
-                          b->nodes};
-    for (int32_t i = 0; i < nout && i < 6; ++i) out[i] = v[i];
+    const int32_t v[8] = {(int32_t)b->mats.size(), b->wshift, b->waves_per_cu, b->grid, b->replays,
+                          b->nodes, b->ucols, b->urows};
+    for (int32_t i = 0; i < nout && i < 8; ++i) out[i] = v[i];
     return GBGPU_OK;
 }
 

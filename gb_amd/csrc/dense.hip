@@ -1226,7 +1226,7 @@ __global__ __launch_bounds__(256) void k_update_mfma(T *a, int64_t ld, int32_t m
 constexpr int WT_NT = 16;   // column tiles per block
 constexpr int WT_CS = 2;    // 16-column products per tile
 
-template <int ND, int RS, typename T, int KC>
+template <int ND, int RS, typename T, int KC, int NT = WT_NT>
 __device__ __forceinline__ void update_wide_body(T *a, int64_t ld, int32_t m, int32_t cb,
                                                  int32_t ce, int32_t c0, const Panel *pan,
                                                  const int32_t *active, const Mod &md,
@@ -1240,7 +1240,7 @@ __device__ __forceinline__ void update_wide_body(T *a, int64_t ld, int32_t m, in
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     // tiles walk the columns [cb, ce) without the gap [gb, gb + gsz) (gb - cb
     // a multiple of TW): virtual column v is column v, or v + gsz from gb on
-    const int32_t qb = cb + bx * (WT_NT * TW);
+    const int32_t qb = cb + bx * (NT * TW);
     const int32_t vce = ce - gsz;
     auto real = [&](int32_t v) { return v >= gb ? v + gsz : v; };
     if (kk == 0 || qb >= vce) return;  // uniform over the block
@@ -1309,7 +1309,7 @@ __device__ __forceinline__ void update_wide_body(T *a, int64_t ld, int32_t m, in
                 }
     };
     int nt = (vce - qb + TW - 1) / TW;
-    nt = nt < WT_NT ? nt : WT_NT;
+    nt = nt < NT ? nt : NT;
     {
         v4i st[PPT];
         rload(real(qb), st);
@@ -1867,17 +1867,42 @@ __global__ __launch_bounds__(256) void k_batch_update_mfma(const DenseDesc *D, c
     update_i8_body<ND, 1, 2, 2>(d.a, d.ld, d.m, c0, d.n, c0, d.pan, d.active, d.md, d.dg, l % ct, l / ct);
 }
 
-// three and four digits: blocks that walk 512 columns of 64 rows with the Rp
-// digits staged in LDS (K = 64), as the single block's side stream does
-template <int ND>
+// blocks that walk strips of 32 NT columns of 64 RS rows with the Rp digits
+// staged in LDS (K = 64), as the single block's side stream does
+template <int ND, int RS, int NT>
 __global__ __launch_bounds__(256) void k_batch_update_wide(const DenseDesc *D, const int32_t *pre,
                                                            int32_t nmat, int32_t c0) {
+    constexpr int W = NT * 16 * WT_CS;
     const int i = tab_find(pre, nmat, blockIdx.x);
     const DenseDesc &d = D[i];
     const int32_t l = blockIdx.x - pre[i];
-    const int32_t ct = (d.n - c0 + 511) / 512;
-    update_wide_body<ND, 1, uint32_t, 1>(d.a, d.ld, d.m, c0, d.n, c0, d.pan, d.active, d.md, d.dg,
-                                         INT_MAX, 0, l % ct, l / ct);
+    const int32_t ct = (d.n - c0 + W - 1) / W;
+    update_wide_body<ND, RS, uint32_t, 1, NT>(d.a, d.ld, d.m, c0, d.n, c0, d.pan, d.active, d.md,
+                                              d.dg, INT_MAX, 0, l % ct, l / ct);
+}
+
+// the MFMA form's batch update kernel for nd digits and tile shape (cols x
+// rows per block: 64 x 64 tiles, or strips of 256 or 512 columns of 64 or,
+// for nd <= 2, 128 rows); nullptr for a shape without a kernel
+static const void *batch_update_fn(int nd, int cols, int rows) {
+#define GBGPU_BU(ND)                                                                        \
+    if (cols == 64 && rows == 64) return (const void *)k_batch_update_mfma<ND>;             \
+    if (cols == 256 && rows == 64) return (const void *)k_batch_update_wide<ND, 1, 8>;      \
+    if (cols == 512 && rows == 64) return (const void *)k_batch_update_wide<ND, 1, 16>;     \
+    if (ND <= 2 && cols == 256 && rows == 128) return (const void *)k_batch_update_wide<(ND <= 2 ? ND : 2), 2, 8>;  \
+    if (ND <= 2 && cols == 512 && rows == 128) return (const void *)k_batch_update_wide<(ND <= 2 ? ND : 2), 2, 16>; \
+    return nullptr;
+    switch (nd) {
+        case 1: { GBGPU_BU(1) }
+        case 2: { GBGPU_BU(2) }
+        case 3: { GBGPU_BU(3) }
+        default: { GBGPU_BU(4) }
+    }
+#undef GBGPU_BU
+}
+
+bool batch_update_shape_ok(int nd, int cols, int rows) {
+    return batch_update_fn(nd, cols, rows) != nullptr;
 }
 
 template <bool BIG>
@@ -1950,7 +1975,8 @@ int check_launch(gbgpu_ctx *ctx, const void *fn, int64_t grid, int block, const 
 }
 
 int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int32_t *tab,
-                     const int32_t *htab, int32_t step, int form, bool big, int nd) {
+                     const int32_t *htab, int32_t step, int form, bool big, int nd, int ucols,
+                     int urows) {
     hipStream_t s = ctx->stream;
     const int32_t c0 = step * PB;
     const int32_t *pre = tab + (int64_t)step * 4 * (nmat + 1);
@@ -1959,12 +1985,10 @@ int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int
     const int32_t nup = hpre[2 * (nmat + 1) + nmat], npr = hpre[3 * (nmat + 1) + nmat];
     int rc;
     const void *upd = form == GBGPU_FORM_MFMA
-                          ? (nd == 1 ? (const void *)k_batch_update_mfma<1>
-                             : nd == 2 ? (const void *)k_batch_update_mfma<2>
-                             : nd == 3 ? (const void *)k_batch_update_wide<3>
-                                       : (const void *)k_batch_update_wide<4>)
+                          ? batch_update_fn(nd, ucols, urows)
                           : (big ? (const void *)k_batch_update_plain<true>
                                  : (const void *)k_batch_update_plain<false>);
+    if (!upd) return set_error(ctx, GBGPU_EINVAL, "no batch update kernel of that shape");
     if ((rc = check_launch(ctx, (const void *)k_batch_select, nmat, SEL_T, "batch select")) ||
         (nrp > 0 && (rc = check_launch(ctx, (const void *)k_batch_rp, nrp, 256, "batch rp"))) ||
         (nga > 0 && (rc = check_launch(ctx, (const void *)k_batch_gather, nga, 256, "batch gather"))) ||
@@ -1978,12 +2002,9 @@ int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int
     if (nup > 0) {
         const int32_t *pu = pre + 2 * (nmat + 1);
         if (form == GBGPU_FORM_MFMA) {
-            switch (nd) {
-                case 1: k_batch_update_mfma<1><<<nup, 256, 0, s>>>(D, pu, nmat, c0); break;
-                case 2: k_batch_update_mfma<2><<<nup, 256, 0, s>>>(D, pu, nmat, c0); break;
-                case 3: k_batch_update_wide<3><<<nup, 256, 0, s>>>(D, pu, nmat, c0); break;
-                default: k_batch_update_wide<4><<<nup, 256, 0, s>>>(D, pu, nmat, c0); break;
-            }
+            int32_t nm = nmat, cc = c0;
+            void *args[] = {(void *)&D, (void *)&pu, (void *)&nm, (void *)&cc};
+            GBGPU_HIP(ctx, hipLaunchKernel(upd, dim3((unsigned)nup), dim3(256), args, 0, s));
         } else {
             if (big) k_batch_update_plain<true><<<nup, 256, 0, s>>>(D, pu, nmat, c0);
             else k_batch_update_plain<false><<<nup, 256, 0, s>>>(D, pu, nmat, c0);
@@ -1998,8 +2019,8 @@ int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int
 }
 
 // block counts of one batch step (prefix sums over the matrices)
-void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int form, int nd,
-                        int32_t *out) {
+void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int form, int ucols,
+                        int urows, int32_t *out) {
     const int32_t c0 = step * PB;
     int32_t *prp = out, *pga = out + (nmat + 1), *pup = out + 2 * (nmat + 1),
             *ppr = out + 3 * (nmat + 1);
@@ -2011,8 +2032,7 @@ void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int for
         const int32_t cap = rows4 < 2048 ? rows4 : 2048;
         int32_t nup = 0;
         if (on) {
-            if (form == GBGPU_FORM_MFMA && nd >= 3) nup = ((n - c0 + 511) / 512) * ((m + 63) / 64);
-            else if (form == GBGPU_FORM_MFMA) nup = ((n - c0 + 63) / 64) * ((m + 63) / 64);
+            if (form == GBGPU_FORM_MFMA) nup = ((n - c0 + ucols - 1) / ucols) * ((m + urows - 1) / urows);
             else nup = ((n - c0 + 255) / 256) * ((m + RT - 1) / RT);
         }
         prp[i + 1] = prp[i] + (on ? (n - c0 + 63) / 64 : 0);

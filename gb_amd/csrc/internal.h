@@ -323,13 +323,17 @@ struct DenseDesc {
 };
 int batch_dense_init(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat);
 // block counts of step `step` for the four grids (rp, gather, update,
-// prune), prefix sums over the matrices: out[4 * (nmat + 1)]; nd: the
-// update's digit count (three and four take 512-column strips)
-void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int form, int nd,
-                        int32_t *out);
-// nd: digits of the MFMA form's update (the largest of the batch)
+// prune), prefix sums over the matrices: out[4 * (nmat + 1)]; the MFMA
+// form's update blocks cover ucols x urows of a block
+void batch_dense_counts(const DenseDesc *hd, int32_t nmat, int32_t step, int form, int ucols,
+                        int urows, int32_t *out);
+// nd: digits of the MFMA form's update (the largest of the batch); its
+// blocks cover ucols x urows (64 x 64 tiles, or strips of 256 or 512
+// columns of 64 rows, or of 128 rows for nd <= 2)
 int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int32_t *tab,
-                     const int32_t *htab, int32_t step, int form, bool big, int nd);
+                     const int32_t *htab, int32_t step, int form, bool big, int nd, int ucols,
+                     int urows);
+bool batch_update_shape_ok(int nd, int cols, int rows);
 size_t dense_panel_bytes();      // sizeof(Panel)
 // a launch's grid, block and static LDS against the device's limits
 int check_launch(gbgpu_ctx *ctx, const void *fn, int64_t grid, int block, const char *what);

@@ -500,10 +500,11 @@ class DeviceBatch:
         return _take_rows(r)
 
     def info(self) -> dict:
-        v = np.zeros(6, dtype=np.int32)
-        lib().gbgpu_batch_info(self.h, v.ctypes.data_as(C.POINTER(C.c_int32)), 6)
+        v = np.zeros(8, dtype=np.int32)
+        lib().gbgpu_batch_info(self.h, v.ctypes.data_as(C.POINTER(C.c_int32)), 8)
         return {"matrices": int(v[0]), "window_shift": int(v[1]), "waves_per_cu": int(v[2]),
-                "sparse_grid": int(v[3]), "replays": int(v[4]), "graph_nodes": int(v[5])}
+                "sparse_grid": int(v[3]), "replays": int(v[4]), "graph_nodes": int(v[5]),
+                "dense_tile": [int(v[6]), int(v[7])]}
 
 
 class DeviceDense:

@@ -239,7 +239,8 @@ int gbgpu_batch_la(gbgpu_ctx *ctx, gbgpu_batch *b, int form, int32_t *np);
 int gbgpu_batch_rows_download(gbgpu_ctx *ctx, gbgpu_batch *b, int32_t i,
                               gbgpu_rows_t *out);
 /* out[0..nout): matrices, window shift, waves per CU, sparse grid, replays,
- * kernel launches in the graph.  GBGPU_BATCH_EAGER=1 in the environment
+ * kernel launches in the graph, the MFMA form's dense update tile (columns,
+ * rows; 64 x 64 in the plain form).  GBGPU_BATCH_EAGER=1 in the environment
  * issues the same launches straight on the stream instead of through the
  * graph (for tools that trace stream launches). */
 int gbgpu_batch_info(gbgpu_batch *b, int32_t *out, int32_t nout);

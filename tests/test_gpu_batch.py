@@ -7,7 +7,7 @@ import pytest
 
 from gb_amd import synth
 from gb_amd.matrix import from_dense_rows, sort_rows
-from gb_amd.native import FORM_MFMA, FORM_PLAIN
+from gb_amd.native import FORM_MFMA, FORM_PLAIN, GpuError
 from oracle import oracle
 
 pytestmark = pytest.mark.gpu
@@ -134,3 +134,49 @@ def test_batch_eager_launches_equal_the_graph(gpu_lib, form, monkeypatch):
             assert e.rows(i).equals(exp[i])
     assert e.info()["graph_nodes"] == 0
     e.free()
+
+
+TILES = [(64, 64), (256, 64), (512, 64), (256, 128), (512, 128)]
+_TILE_CASES = {}
+
+
+def _tile_case(primes):
+    """Three matrices whose D' blocks span several update strips and row
+    blocks, and the oracle's results (computed once per prime set)."""
+    key = tuple(primes)
+    if key not in _TILE_CASES:
+        mats = [synth.f4_shaped(nr, int(nr * 1.5), int(nr * 0.6), p, seed=40 + k,
+                                piv_len=(3, 40), piv_mu=np.log(12), red_nnz=20)
+                for k, (nr, p) in enumerate(zip((1500, 2200, 3000), primes))]
+        _TILE_CASES[key] = (mats, [oracle.la_sparse_dense(m) for m in mats])
+    return _TILE_CASES[key]
+
+
+@pytest.mark.parametrize("tile", TILES, ids=[f"{c}x{r}" for c, r in TILES])
+@pytest.mark.parametrize("primes", [[2147483647, 16777213, 2147483587], [65521, 32003, 40009],
+                                    [251, 241, 239]],
+                         ids=["four_digits", "two_digits", "one_digit"])
+def test_batch_update_tiles(gpu_lib, tile, primes, monkeypatch):
+    """Every dense update tile the batch's tuner chooses from (forced with
+    GBGPU_BATCH_TILE) gives the oracle's rows in the MFMA form, on blocks
+    wide and tall enough for several strips and row blocks; tiles of 128
+    rows exist for at most two digits and are refused above."""
+    two_digits = max(primes) <= 65521
+    monkeypatch.setenv("GBGPU_BATCH_TILE", f"{tile[0]},{tile[1]}")
+    if tile[1] == 128 and not two_digits:
+        mats = [synth.f4_shaped(300, 450, 180, p, seed=7, piv_len=(3, 30), red_nnz=12)
+                for p in primes]
+        b = gpu_lib.batch(mats)
+        with pytest.raises(GpuError, match="GBGPU_BATCH_TILE"):
+            b.la(FORM_MFMA)
+        b.free()
+        return
+    mats, exp = _tile_case(primes)
+    b = gpu_lib.batch(mats)
+    for _ in range(2):
+        npv = b.la(FORM_MFMA)
+        for i in range(len(mats)):
+            assert npv[i] == exp[i].np
+            assert b.rows(i).equals(exp[i])
+    assert b.info()["dense_tile"] == list(tile)
+    b.free()
