@@ -261,6 +261,22 @@ __device__ __forceinline__ void sel_apply(SelShared &sh, const uint32_t (*Cm)[PB
         for (int col = PB + wv; col < 2 * PB; col += 2 * SEL_W) two(col, col + SEL_W);
 }
 
+// the first ND signed base-256 digits of x's representative (x itself when
+// x <= hi = 127 (1 + 256 + ...), else x - p), digit i in byte i, the bytes
+// above zero
+template <int ND>
+__device__ __forceinline__ uint32_t digits_n(uint32_t x, uint32_t p, uint32_t hi) {
+    int32_t v = x <= hi ? (int32_t)x : (int32_t)x - (int32_t)p;
+    uint32_t w = 0;
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+        const int32_t d = (int32_t)(int8_t)(v & 0xFF);
+        w |= (uint32_t)(d & 0xFF) << (8 * k);
+        if (k + 1 < ND) v = (v - d) >> 8;
+    }
+    return w;
+}
+
 // the four signed base-256 digits of x's representative (x itself when
 // x <= hi = 127 (1 + 256 + ...), else x - p), digit i in byte i
 __device__ __forceinline__ uint32_t digits4(uint32_t x, uint32_t p, uint32_t hi) {
@@ -297,7 +313,7 @@ __device__ __forceinline__ void sel_apply_mfma(SelShared &sh, const Mod &md, con
         uint32_t w[ND] = {};
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const uint32_t d4 = digits4(sh.X[sh.qs[4 * k + b]][col], p, dg.hi);
+            const uint32_t d4 = digits_n<ND>(sh.X[sh.qs[4 * k + b]][col], p, dg.hi);
 #pragma unroll
             for (int i = 0; i < ND; ++i) w[i] |= ((d4 >> (8 * i)) & 0xFFu) << (8 * b);
         }
@@ -309,7 +325,7 @@ __device__ __forceinline__ void sel_apply_mfma(SelShared &sh, const Mod &md, con
         uint32_t wc[ND] = {};
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const uint32_t d4 = digits4(sh.C[s][4 * k + b], p, dg.hi);
+            const uint32_t d4 = digits_n<ND>(sh.C[s][4 * k + b], p, dg.hi);
 #pragma unroll
             for (int i = 0; i < ND; ++i) wc[i] |= ((d4 >> (8 * i)) & 0xFFu) << (8 * b);
         }
@@ -1473,6 +1489,280 @@ __global__ void k_dense_init(int32_t m, int32_t n, int32_t *rowpiv, int32_t *col
     }
 }
 
+// the digit planes of T (Td[i][t][s], t, s < kk; zero elsewhere) and of
+// the pivot rows' entries of the columns [q0, min(q0 + 64, qe)), transposed
+// (Pd[i][j][s]); erow in LDS.  A thread builds four consecutive s of one row
+// t and of one column j at a time (a wave reads 64 consecutive columns of a
+// pivot row); every load is in flight before the first is used.
+template <int ND, typename T>
+__device__ __forceinline__ void tile_digits(int8_t (*Td)[PB][PB], int8_t (*Pd)[PB][PB],
+                                            const Panel *pan, const T *a, int64_t ld,
+                                            const int32_t *erow, int32_t q0, int32_t qe, int kk,
+                                            uint32_t p, uint32_t hi) {
+    const int tid = threadIdx.x;
+    constexpr int IT = PB * PB / 4 / 256;  // four-element pieces per thread
+    uint32_t tv[IT][4], pv[IT][4];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int u = tid + 256 * it;
+        // T holds zeros past kk in both indices
+#pragma unroll
+        for (int b = 0; b < 4; ++b) tv[it][b] = pan->T[(u >> 4) * PB + (u & 15) * 4 + b];
+        const int j = u & 63, s4 = (u >> 6) * 4;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            pv[it][b] = q0 + j < qe ? (uint32_t)a[(int64_t)erow[s4 + b] * ld + q0 + j] : 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int u = tid + 256 * it;
+        uint32_t wt[ND] = {}, wp[ND] = {};
+        const int j = u & 63, s4 = (u >> 6) * 4;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t dt = digits_n<ND>(tv[it][b], p, hi);
+            const uint32_t dp = digits_n<ND>(s4 + b < kk ? pv[it][b] : 0u, p, hi);
+#pragma unroll
+            for (int i = 0; i < ND; ++i) {
+                wt[i] |= ((dt >> (8 * i)) & 0xFFu) << (8 * b);
+                wp[i] |= ((dp >> (8 * i)) & 0xFFu) << (8 * b);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < ND; ++i) {
+            *(uint32_t *)&Td[i][u >> 4][(u & 15) * 4] = wt[i];
+            *(uint32_t *)&Pd[i][j][s4] = wp[i];
+        }
+    }
+}
+
+// ------------------------------------------------- next panel's share
+//
+// The main stream's part of a panel step in the MFMA form, one launch for
+// what the select of the next panel waits for.  Block b takes the rows
+// [64 b, 64 b + 64):
+//   1. Rp of the next panel's columns [c1, c2): T (the transform of the
+//      select) times the pivot rows' entries there, on the matrix cores in
+//      signed base-256 digits (every block computes it; block 0 also leaves
+//      it in rp, from where the side stream writes the pivot rows once every
+//      block has read them);
+//   2. the multipliers of its rows: L[r][t] = A[r][ecol t] (zero for this
+//      panel's pivot rows), their digit planes and the active flags, as
+//      gather_body;
+//   3. A[r][c1, c2) -= L[r] Rp for its active rows, on the matrix cores.
+// The digits go through LDS: T and the pivot rows' entries (operands of 1),
+// Rp^T and L (operands of 3).
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void k_panel_narrow(T *a, int64_t ld, int32_t m, int32_t c0,
+                                                      int32_t c1, int32_t c2,
+                                                      const int32_t *rowpiv, const Panel *pan,
+                                                      uint32_t *L, int32_t *active, uint32_t *rp,
+                                                      int64_t ldr, Mod md, Digits dg) {
+    const int kk = __builtin_amdgcn_readfirstlane(pan->k);
+    if (kk == 0) return;
+    __shared__ __align__(16) int8_t Td[ND][PB][PB];  // T: [digit][t][s]
+    __shared__ __align__(16) int8_t Pd[ND][PB][PB];  // pivot rows, transposed: [digit][j][s]
+    __shared__ __align__(16) int8_t Rd[ND][PB][PB];  // Rp, transposed: [digit][j][t]
+    __shared__ __align__(16) int8_t Ld[ND][PB][PB];  // multipliers: [digit][row][t]
+    __shared__ int32_t erow_s[PB], ecol_s[PB];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const uint32_t p = md.p;
+    const int32_t w2 = c2 - c1;  // columns of the next panel (1 .. 64)
+    const int32_t r0 = (int32_t)blockIdx.x * PB;
+    // loads that need nothing computed here go out first: the pivot
+    // columns of this wave's rows and this lane's entries of step 3 (rows
+    // 16 wv + 4 (lane / 16) + e, columns c1 + 16 cb + lane % 16)
+    int32_t rpv[16];
+#pragma unroll
+    for (int h = 0; h < 16; ++h) {
+        const int32_t r = r0 + 16 * wv + h;
+        rpv[h] = r < m ? rowpiv[r] : 0;
+    }
+    uint32_t xv[4][4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int32_t r = r0 + 16 * wv + 4 * lk + e;
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            const int32_t j = 16 * cb + lr;
+            xv[e][cb] = (r < m && j < w2) ? (uint32_t)a[(int64_t)r * ld + c1 + j] : 0u;
+        }
+    }
+    if (tid < PB) {
+        erow_s[tid] = tid < kk ? pan->erow[tid] : 0;
+        ecol_s[tid] = tid < kk ? pan->ecol[tid] : 0;
+    }
+    __syncthreads();
+    // 1a. digits of T (a thread: row t, four consecutive s) and of the pivot
+    // rows (a thread: column j, four consecutive s; a wave reads 64
+    // consecutive columns of one row); every load in flight before the
+    // first is used
+    tile_digits<ND, T>(Td, Pd, pan, a, ld, erow_s, c1, c1 + w2, kk, p, dg.hi);
+    // 2a. this wave's rows of the block (16), their multipliers' loads in
+    // flight together
+    uint32_t lv[16];
+#pragma unroll
+    for (int h = 0; h < 16; ++h) {
+        const int32_t r = r0 + 16 * wv + h;
+        const bool on = r < m && !(rpv[h] >= c0 && rpv[h] < c1);
+        lv[h] = (on && lane < kk) ? (uint32_t)a[(int64_t)r * ld + ecol_s[lane]] : 0u;
+    }
+    __syncthreads();
+    // 1b. Rp^T: wave wv takes rows t of 16 wv .. (the transform's), all 64
+    // columns j; result register e of lane l: t = 16 wv + 4 (l / 16) + e,
+    // j = 16 cb + l % 16
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        v4i acc[2 * ND - 1];
+#pragma unroll
+        for (int q = 0; q < 2 * ND - 1; ++q) acc[q] = v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+#pragma unroll
+            for (int jd = 0; jd < ND; ++jd)
+                acc[i + jd] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                    *(const v4i *)&Td[i][16 * wv + lr][16 * lk],
+                    *(const v4i *)&Pd[jd][16 * cb + lr][16 * lk], acc[i + jd], 0, 0, 0);
+        const int j = 16 * cb + lr;
+        uint32_t wd[ND] = {};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int t = 16 * wv + 4 * lk + e;
+            uint32_t r;
+            if constexpr (ND == 2) {
+                const int32_t v = digits_balanced2(acc, e, (int32_t)p, dg.pinvf);
+                r = (uint32_t)(v < 0 ? v + (int32_t)p : v);
+            } else {
+                r = digits_combine<ND>(acc, e, md, dg);
+            }
+            r = (t < kk && j < w2) ? r : 0u;
+            if (blockIdx.x == 0 && t < kk && j < w2) rp[(int64_t)t * ldr + (c1 + j - c0)] = r;
+            const uint32_t d4 = digits_n<ND>(r, p, dg.hi);
+#pragma unroll
+            for (int i = 0; i < ND; ++i) wd[i] |= ((d4 >> (8 * i)) & 0xFFu) << (8 * e);
+        }
+#pragma unroll
+        for (int i = 0; i < ND; ++i) *(uint32_t *)&Rd[i][j][16 * wv + 4 * lk] = wd[i];
+    }
+    // 2b. the multipliers (plain, digit planes, LDS) and the active flags
+    bool xa[4] = {false, false, false, false};
+#pragma unroll
+    for (int h = 0; h < 16; ++h) {
+        const int32_t r = r0 + 16 * wv + h;
+        const uint32_t v = lv[h];
+        const uint32_t d4 = digits_n<ND>(v, p, dg.hi);
+        const bool act = __any(v != 0u);
+        if (r < m) {
+            L[(int64_t)r * PB + lane] = v;
+#pragma unroll
+            for (int i = 0; i < ND; ++i)
+                dg.L[i * dg.lplane + (int64_t)r * PB + lane] = (int8_t)((d4 >> (8 * i)) & 0xFFu);
+            active[r] = act ? 1 : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < ND; ++i) Ld[i][16 * wv + h][lane] = (int8_t)((d4 >> (8 * i)) & 0xFFu);
+        // this lane's output rows of step 3 are 16 wv + 4 (lane / 16) + e
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (h == 4 * lk + e) xa[e] = act && r < m;
+    }
+    // 3. A[r][c1, c2) -= L[r] Rp for this wave's active rows
+    __syncthreads();
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        v4i acc[2 * ND - 1];
+#pragma unroll
+        for (int q = 0; q < 2 * ND - 1; ++q) acc[q] = v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+#pragma unroll
+            for (int jd = 0; jd < ND; ++jd)
+                acc[i + jd] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                    *(const v4i *)&Ld[i][16 * wv + lr][16 * lk],
+                    *(const v4i *)&Rd[jd][16 * cb + lr][16 * lk], acc[i + jd], 0, 0, 0);
+        const int32_t j = 16 * cb + lr;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int32_t r = r0 + 16 * wv + 4 * lk + e;
+            if (xa[e] && j < w2)
+                a[(int64_t)r * ld + c1 + j] = (T)digits_sub<ND>(xv[e][cb], acc, e, md, dg);
+        }
+    }
+}
+
+// Rp of the columns [c0, jhi) in the MFMA form, a block per 64 columns
+// [c0 + 64 b, c0 + 64 b + 64): T times the pivot rows' entries there, on
+// the matrix cores in signed base-256 digits; writes the pivot rows (in
+// reduced form) and the digit planes of Rp^T for the update.  The block at
+// column wb (the next panel's, whose Rp k_panel_narrow has left in rp)
+// writes the pivot rows there from rp instead: every block of that launch
+// has read them by now.
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void k_panel_rp_mfma(T *a, int64_t ld, int32_t c0, int32_t jhi,
+                                                       int32_t wb, const Panel *pan,
+                                                       const uint32_t *rp, int64_t ldr, Mod md,
+                                                       Digits dg) {
+    const int kk = __builtin_amdgcn_readfirstlane(pan->k);
+    if (kk == 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const uint32_t p = md.p;
+    const int32_t q0 = c0 + (int32_t)blockIdx.x * PB;
+    if (q0 == wb) {
+        for (int i = tid; i < kk * PB; i += 256) {
+            const int t = i >> 6, j = i & 63;
+            if (q0 + j < jhi) a[(int64_t)pan->erow[t] * ld + q0 + j] = (T)rp[(int64_t)t * ldr + (q0 + j - c0)];
+        }
+        return;
+    }
+    __shared__ __align__(16) int8_t Td[ND][PB][PB];  // T: [digit][t][s]
+    __shared__ __align__(16) int8_t Pd[ND][PB][PB];  // pivot rows, transposed: [digit][j][s]
+    __shared__ int32_t erow_s[PB];
+    if (tid < PB) erow_s[tid] = tid < kk ? pan->erow[tid] : 0;
+    __syncthreads();
+    tile_digits<ND, T>(Td, Pd, pan, a, ld, erow_s, q0, jhi, kk, p, dg.hi);
+    __syncthreads();  // every read of the pivot rows is done before they are written
+    // wave wv: rows t of 16 wv .., all 64 columns; register e of lane l:
+    // t = 16 wv + 4 (l / 16) + e, column 16 cb + l % 16
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        v4i acc[2 * ND - 1];
+#pragma unroll
+        for (int q = 0; q < 2 * ND - 1; ++q) acc[q] = v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+#pragma unroll
+            for (int jd = 0; jd < ND; ++jd)
+                acc[i + jd] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                    *(const v4i *)&Td[i][16 * wv + lr][16 * lk],
+                    *(const v4i *)&Pd[jd][16 * cb + lr][16 * lk], acc[i + jd], 0, 0, 0);
+        const int32_t col = q0 + 16 * cb + lr;
+        uint32_t wd[ND] = {};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int t = 16 * wv + 4 * lk + e;
+            uint32_t r;
+            if constexpr (ND == 2) {
+                const int32_t v = digits_balanced2(acc, e, (int32_t)p, dg.pinvf);
+                r = (uint32_t)(v < 0 ? v + (int32_t)p : v);
+            } else {
+                r = digits_combine<ND>(acc, e, md, dg);
+            }
+            r = (t < kk && col < jhi) ? r : 0u;
+            if (t < kk && col < jhi) a[(int64_t)erow_s[t] * ld + col] = (T)r;
+            const uint32_t d4 = digits_n<ND>(r, p, dg.hi);
+#pragma unroll
+            for (int i = 0; i < ND; ++i) wd[i] |= ((d4 >> (8 * i)) & 0xFFu) << (8 * e);
+        }
+        if (col < jhi) {
+#pragma unroll
+            for (int i = 0; i < ND; ++i)
+                *(uint32_t *)(dg.R + i * dg.rplane + (int64_t)(col - c0) * 64 + 16 * wv + 4 * lk) = wd[i];
+        }
+    }
+}
+
 static int grid_of(int64_t n, int bs) {
     int64_t g = (n + bs - 1) / bs;
     if (g < 1) g = 1;
@@ -1627,13 +1917,31 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
         const int bw = (n - c0) < PB ? (n - c0) : PB;
         const int32_t c1 = c0 + bw;                       // next panel
         const int32_t c2 = (n - c1) < PB ? n : c1 + PB;
-        // Rp over the columns the next select needs first; the rest after
-        // that select is on its way
-        launch_rp<T>(s, true, a, ld, c0, c0, c2, cur, rp, ldr, md, r64, dg);
-        k_panel_gather<T><<<grid_of(m, 4), 256, 0, s>>>(a, ld, m, c0, bw, rowpiv, cur, L, active,
-                                                         d->p, dg);
+        // the MFMA form: the next panel's columns (their Rp, the multipliers,
+        // their update) in one launch, the panel's own columns' Rp and the
+        // pivot rows' write-back on the side stream; the plain form: Rp over
+        // the columns the next select needs first, the rest after that
+        // select is on its way
+        const bool fused = form == GBGPU_FORM_MFMA && c1 < n;
+        if (fused) {
+            const unsigned g = (unsigned)((m + PB - 1) / PB);
+            switch (dg.nd) {
+                case 1: k_panel_narrow<1, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, L, active, rp, ldr, md, dg); break;
+                case 2: k_panel_narrow<2, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, L, active, rp, ldr, md, dg); break;
+                default:
+                    if constexpr (sizeof(T) == 4) {
+                        if (dg.nd == 3) k_panel_narrow<3, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, L, active, rp, ldr, md, dg);
+                        else k_panel_narrow<4, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, L, active, rp, ldr, md, dg);
+                    }
+                    break;
+            }
+        } else {
+            launch_rp<T>(s, true, a, ld, c0, c0, c2, cur, rp, ldr, md, r64, dg);
+            k_panel_gather<T><<<grid_of(m, 4), 256, 0, s>>>(a, ld, m, c0, bw, rowpiv, cur, L, active,
+                                                             d->p, dg);
+        }
         if (c1 < n) {
-            update(s, cur, c0, c1, c2);
+            if (!fused) update(s, cur, c0, c1, c2);
             if (hipEventRecord(ctx->sync_ev[0], s) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
@@ -1644,7 +1952,23 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
             if (hipStreamWaitEvent(x, ctx->sync_ev[0], 0) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
-            if (c2 < n) launch_rp<T>(x, false, a, ld, c0, c2, n, cur, rp, ldr, md, r64, dg);
+            if (fused) {
+                // Rp of [c0, c1) and [c2, n) on the matrix cores, the pivot
+                // rows of [c1, c2) from rp
+                const unsigned g = (unsigned)((n - c0 + PB - 1) / PB);
+                switch (dg.nd) {
+                    case 1: k_panel_rp_mfma<1, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg); break;
+                    case 2: k_panel_rp_mfma<2, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg); break;
+                    default:
+                        if constexpr (sizeof(T) == 4) {
+                            if (dg.nd == 3) k_panel_rp_mfma<3, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg);
+                            else k_panel_rp_mfma<4, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg);
+                        }
+                        break;
+                }
+            } else if (c2 < n) {
+                launch_rp<T>(x, false, a, ld, c0, c2, n, cur, rp, ldr, md, r64, dg);
+            }
             if (form == GBGPU_FORM_MFMA && c2 < n && c1 - c0 == PB && dg.nd <= 2) {
                 update(x, cur, c0, c0, n, c1, c2 - c1);  // [c0, c1) and [c2, n)
             } else if (form == GBGPU_FORM_MFMA && c2 < n && c1 - c0 == PB) {
