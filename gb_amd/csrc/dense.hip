@@ -1547,7 +1547,8 @@ __device__ __forceinline__ void tile_digits(int8_t (*Td)[PB][PB], int8_t (*Pd)[P
 //      it in rp, from where the side stream writes the pivot rows once every
 //      block has read them);
 //   2. the multipliers of its rows: L[r][t] = A[r][ecol t] (zero for this
-//      panel's pivot rows), their digit planes and the active flags, as
+//      panel's pivot rows), as their digit planes (the plain multipliers
+//      only the plain form reads are not written) and the active flags, as
 //      gather_body;
 //   3. A[r][c1, c2) -= L[r] Rp for its active rows, on the matrix cores.
 // The digits go through LDS: T and the pivot rows' entries (operands of 1),
@@ -1556,7 +1557,7 @@ template <int ND, typename T>
 __global__ __launch_bounds__(256) void k_panel_narrow(T *a, int64_t ld, int32_t m, int32_t c0,
                                                       int32_t c1, int32_t c2,
                                                       const int32_t *rowpiv, const Panel *pan,
-                                                      uint32_t *L, int32_t *active, uint32_t *rp,
+                                                      int32_t *active, uint32_t *rp,
                                                       int64_t ldr, Mod md, Digits dg) {
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
@@ -1654,7 +1655,6 @@ __global__ __launch_bounds__(256) void k_panel_narrow(T *a, int64_t ld, int32_t 
         const uint32_t d4 = digits_n<ND>(v, p, dg.hi);
         const bool act = __any(v != 0u);
         if (r < m) {
-            L[(int64_t)r * PB + lane] = v;
 #pragma unroll
             for (int i = 0; i < ND; ++i)
                 dg.L[i * dg.lplane + (int64_t)r * PB + lane] = (int8_t)((d4 >> (8 * i)) & 0xFFu);
@@ -1926,12 +1926,12 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
         if (fused) {
             const unsigned g = (unsigned)((m + PB - 1) / PB);
             switch (dg.nd) {
-                case 1: k_panel_narrow<1, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, L, active, rp, ldr, md, dg); break;
-                case 2: k_panel_narrow<2, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, L, active, rp, ldr, md, dg); break;
+                case 1: k_panel_narrow<1, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg); break;
+                case 2: k_panel_narrow<2, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg); break;
                 default:
                     if constexpr (sizeof(T) == 4) {
-                        if (dg.nd == 3) k_panel_narrow<3, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, L, active, rp, ldr, md, dg);
-                        else k_panel_narrow<4, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, L, active, rp, ldr, md, dg);
+                        if (dg.nd == 3) k_panel_narrow<3, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg);
+                        else k_panel_narrow<4, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg);
                     }
                     break;
             }
@@ -1999,7 +1999,10 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
             if (hipStreamWaitEvent(s, ctx->lag_ev[it % LAG], 0) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
-            k_prune<T><<<grid_of(m, 4), 256, 0, s>>>(a, ld, n, c1, cand, st, active);
+            // most panels prune nothing (every block returns at once), so
+            // the grid is kept to a block per CU
+            const int pg = grid_of(m, 4) < ctx->num_cu ? grid_of(m, 4) : ctx->num_cu;
+            k_prune<T><<<pg, 256, 0, s>>>(a, ld, n, c1, cand, st, active);
         } else {
             update(s, cur, c0, c0, n);
         }

@@ -254,23 +254,28 @@ __device__ __forceinline__ uint32_t digits_combine(const v4i *acc, int e, const 
 
 // C0 + 2^8 C1 + 2^16 C2 (nd = 2, up to 256 pivot slots: |C0|, |C2| <= 2^22,
 // |C1| <= 2^23) modulo p for 256 <= p <= 65521, as a residue in
-// [-(p/2 + 2), p/2 + 2], with single-precision quotients and 24-bit
+// [-(p/2 + 1), p/2 + 1], with single-precision quotients and 24-bit
 // multiplies (all full rate) instead of a 64-bit reduction.  Horner in base
 // 256: a = C2, then a = 256 a' + C1, then a = 256 a' + C0, where a' is a
 // reduced by q = rint(a / p).  Every a is below 2^24 in size, so it is exact
-// in single precision: |a'| <= p/2 + 2 <= 32762 gives |256 a'| <= 8387072.
-// The computed quotient is within 2^-23 |a / p| <= 2 / p of a / p, so
-// |a - q p| <= p/2 + 2.  The 24-bit products q p are exact (|q| < 2^17).
+// in single precision: |a'| <= p/2 + 1 <= 32761 gives |256 a'| <= 8386816.
+// The quotient is a (1/p) rounded to the nearest integer in one step: a
+// fused multiply-add with 1.5 2^23, whose sum has a unit in the last place
+// of one, leaves q in the low bits of its representation.  1/p is within
+// 2^-24 of its value relative, so a (1/p) is within |a / p| 2^-24 < 1 / p
+// of a / p and |a - q p| <= p/2 + 1.  The 24-bit products q p are exact
+// (|q| < 2^17).
+__device__ __forceinline__ int32_t rint_quot(int32_t a, float pinv) {
+    return __float_as_int(__builtin_fmaf((float)a, pinv, 12582912.0f)) - 0x4B400000;
+}
+
 __device__ __forceinline__ int32_t digits_balanced2(const v4i *acc, int e, int32_t p, float pinv) {
     int32_t a = acc[2][e];
-    int32_t q = (int32_t)__builtin_rintf((float)a * pinv);
-    a -= __mul24(q, p);
+    a -= __mul24(rint_quot(a, pinv), p);
     a = a * 256 + acc[1][e];
-    q = (int32_t)__builtin_rintf((float)a * pinv);
-    a -= __mul24(q, p);
+    a -= __mul24(rint_quot(a, pinv), p);
     a = a * 256 + acc[0][e];
-    q = (int32_t)__builtin_rintf((float)a * pinv);
-    return a - __mul24(q, p);
+    return a - __mul24(rint_quot(a, pinv), p);
 }
 
 // (x - combined digit products) mod p for x in [0, p): the single-precision
@@ -279,11 +284,10 @@ template <int ND>
 __device__ __forceinline__ uint32_t digits_sub(uint32_t x, const v4i *acc, int e, const Mod &md,
                                                const Digits &dg) {
     if constexpr (ND == 2) {
-        const int32_t ip = (int32_t)md.p;
-        int32_t r = (int32_t)x - digits_balanced2(acc, e, ip, dg.pinvf);
-        r += r < 0 ? ip : 0;
-        r -= r >= ip ? ip : 0;
-        return (uint32_t)r;
+        // x - b lies in [-(p/2 + 1), 3p/2 + 1]: of it, it + p and it - p
+        // as unsigned words the smallest is the residue
+        const uint32_t u = x - (uint32_t)digits_balanced2(acc, e, (int32_t)md.p, dg.pinvf);
+        return min(min(u, u + md.p), u - md.p);
     } else {
         const uint32_t u = digits_combine<ND>(acc, e, md, dg);
         return x >= u ? x - u : x + md.p - u;
