@@ -1999,9 +1999,9 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
             if (hipStreamWaitEvent(s, ctx->lag_ev[it % LAG], 0) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
-            // most panels prune nothing (every block returns at once), so
-            // the grid is kept to a block per CU
-            const int pg = grid_of(m, 4) < ctx->num_cu ? grid_of(m, 4) : ctx->num_cu;
+            // most panels prune nothing (every block returns at once): the
+            // grid stops at four blocks per CU
+            const int pg = grid_of(m, 4) < 4 * ctx->num_cu ? grid_of(m, 4) : 4 * ctx->num_cu;
             k_prune<T><<<pg, 256, 0, s>>>(a, ld, n, c1, cand, st, active);
         } else {
             update(s, cur, c0, c0, n);
