@@ -24,7 +24,9 @@ is timed beside it.  The JSON line also carries:
     (plain and MFMA forms, and the same run with the oracle's linear
     algebra on the host's cores), the full la_option 1 step on the same
     matrix in both forms, the dense 4096 x 8192 block elimination in
-    modular GOPS, and the 64-matrix batch.
+    modular GOPS, and the 64-matrix batch; each result checked against the
+    oracle's digest of it (tests/golden/full_size_digests.json) outside the
+    timed region, as `*_digest_ok`.
 Every libgbgpu call synchronises its stream before it returns, so the host
 clock around a call covers its device work.
 """
@@ -54,6 +56,19 @@ PMC_TRAFFIC_BYTES = 8068585 * 1024 * 2.0 + 940417 * 1024 * 1.0
 
 # the CPU share of one GPU on the benchmark host: worker pools never exceed it
 CPU_SHARE = 16
+
+
+def full_size_digests():
+    """The oracle's SHA-256 digests of the full-size results
+    (tests/golden/full_size_digests.json, written on the CPU by
+    tests/golden/make_digests.py): each extra's result is checked against
+    them outside its timed region (`*_digest_ok`), or None without them."""
+    path = os.path.join(ROOT, "tests", "golden", "full_size_digests.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except OSError:
+        return None
 
 
 def cpu_threads() -> int:
@@ -122,8 +137,10 @@ def f4_runs(nvars: int, ctx, with_cpu: bool, forms=("mfma", "plain"), cpu_vars: 
     with the GPU run on it beside)."""
     import numpy as np
     from gb_amd import native, synth
+    from tests import digests
     res = {"system": f"{nvars} dense quadratics in {nvars} variables mod 65521, DRL, seed 1"}
     s = synth.dense_quadratic_system(nvars, 65521, seed=1)
+    fix = (full_size_digests() or {}).get("c5", {}).get(f"vars{nvars}")
     runs = {}
     for form, name in ((native.FORM_MFMA, "mfma"), (native.FORM_PLAIN, "plain")):
         if name not in forms:
@@ -153,6 +170,8 @@ def f4_runs(nvars: int, ctx, with_cpu: bool, forms=("mfma", "plain"), cpu_vars: 
                      "basis_elements": len(b.lens), "max_matrix": [st["max_rows"], st["max_cols"]],
                      "max_nnz": st["max_nnz"], "rows_reduced": st["rows_total"],
                      "rows_per_s_la": st["rows_total"] / (st["la_ms"] / 1e3)}
+        if fix:
+            res[name]["basis_digest_ok"] = digests.basis(b) == fix["basis"]
     a = runs.get("mfma") or runs.get("plain")
     if len(runs) == 2:
         b2 = runs["plain"]
@@ -312,6 +331,8 @@ def main(argv=None) -> int:
                                "gpu_vs_cpu": rows_per_s / (nrl / tc)}
 
     if not args.no_extra:
+        from tests import digests
+        fix = full_size_digests() or {}
         extra = {}
         if not args.no_f4:
             extra["f4"] = f4_runs(args.f4_vars, ctx, not args.no_cpu, tuple(args.f4_forms.split(",")),
@@ -324,6 +345,8 @@ def main(argv=None) -> int:
             extra[f"la_option1_{name}_ms"] = 1000.0 * (time.perf_counter() - t0)
             extra[f"la_option1_{name}_dense_ms"] = ctx.phase_ms(1)
             extra["la_option1_new_pivots"] = npiv
+            if "c2" in fix:
+                extra[f"la_option1_{name}_digest_ok"] = digests.sparse_rows(d2.rows()) == fix["c2"]["la_option1"]
             d2.free()
         # dense block elimination (4096 x 8192, full rank and rank 3000)
         for dp, rank in ((251, None), (P, None), (2147483629, None), (P, 3000)):
@@ -340,6 +363,10 @@ def main(argv=None) -> int:
                 extra[f"{tag}_{name}_ms"] = 1000.0 * td
                 extra[f"{tag}_{name}_gops"] = dense_ops(4096, 8192, r) / td / 1e9
                 extra[f"{tag}_rank"] = r
+                key = tag[len("dense_4096x8192_"):]
+                if key in fix.get("c3", {}):
+                    rows, lead = dd.result()
+                    extra[f"{tag}_{name}_digest_ok"] = digests.dense(rows, lead) == fix["c3"][key]["rref"]
                 dd.free()
         # batched multimodular launch: 64 F4-shaped matrices of uneven sizes
         # (rows lognormal in [500, 20000], columns 1.5 x rows, density
@@ -363,6 +390,10 @@ def main(argv=None) -> int:
             extra[f"batch64_{name}_graph_ms"] = sorted(gs)[1]
             extra[f"batch64_{name}_rows_per_s"] = nrl_b / tb
             extra[f"batch64_{name}_tuned"] = bt.info()
+            if "c4" in fix:
+                extra[f"batch64_{name}_digest_ok"] = all(
+                    digests.sparse_rows(bt.rows(i)) == x["la_option1"]
+                    for i, x in enumerate(fix["c4"]["matrices"]))
         if bt is not None:
             bt.free()
         singles = [ctx.upload(mm) for mm in mats]
