@@ -824,23 +824,31 @@ __global__ __launch_bounds__(256) void k_prune_compact(int32_t *cand, ListState 
 
 // both steps in one launch: every block writes its rows' flags, and the last
 // block to finish compacts the list (the flags read through to L2)
+// block b of nb prune blocks (256 threads each)
 template <typename T>
-__global__ __launch_bounds__(256) void k_prune(const T *a, int64_t ld, int32_t n, int32_t c1,
-                                               int32_t *cand, ListState *st, int32_t *alive) {
+__device__ __forceinline__ void prune_body(const T *a, int64_t ld, int32_t n, int32_t c1,
+                                           int32_t *cand, ListState *st, int32_t *alive,
+                                           int32_t b, int32_t nb) {
     __shared__ int32_t last;
     const int32_t nzero = __builtin_amdgcn_readfirstlane(st->nzero);
     const int32_t done = __builtin_amdgcn_readfirstlane(st->done);
     if (nzero < PB || done) return;  // the same in every block: st changes below, after all checked
-    prune_flags_body(a, ld, n, c1, cand, st, alive,
-                     blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), gridDim.x * (blockDim.x >> 6));
+    prune_flags_body(a, ld, n, c1, cand, st, alive, b * (blockDim.x >> 6) + (threadIdx.x >> 6),
+                     nb * (blockDim.x >> 6));
     __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(&st->pcount, 1) == (int32_t)gridDim.x - 1;
+    if (threadIdx.x == 0) last = atomicAdd(&st->pcount, 1) == nb - 1;
     __syncthreads();
     if (!last) return;
     __threadfence();
     prune_compact_body(cand, st, alive, true);
     if (threadIdx.x == 0) st->pcount = 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_prune(const T *a, int64_t ld, int32_t n, int32_t c1,
+                                               int32_t *cand, ListState *st, int32_t *alive) {
+    prune_body(a, ld, n, c1, cand, st, alive, (int32_t)blockIdx.x, (int32_t)gridDim.x);
 }
 
 // ---------------------------------------------------------------- rp
@@ -1552,13 +1560,23 @@ __device__ __forceinline__ void tile_digits(int8_t (*Td)[PB][PB], int8_t (*Pd)[P
 //      gather_body;
 //   3. A[r][c1, c2) -= L[r] Rp for its active rows, on the matrix cores.
 // The digits go through LDS: T and the pivot rows' entries (operands of 1),
-// Rp^T and L (operands of 3).
+// Rp^T and L (operands of 3).  The blocks from gn on are the previous
+// panel's prune of the candidate list over the columns [c0, n) (prune_body;
+// the list is not read here, and a row this launch updates has a nonzero
+// multiplier in [c0, c1), which it does not write, so the prune's verdict
+// does not depend on the order of the two).
 template <int ND, typename T>
-__global__ __launch_bounds__(256) void k_panel_narrow(T *a, int64_t ld, int32_t m, int32_t c0,
-                                                      int32_t c1, int32_t c2,
+__global__ __launch_bounds__(256) void k_panel_narrow(T *a, int64_t ld, int32_t m, int32_t n,
+                                                      int32_t c0, int32_t c1, int32_t c2,
                                                       const int32_t *rowpiv, const Panel *pan,
                                                       int32_t *active, uint32_t *rp,
-                                                      int64_t ldr, Mod md, Digits dg) {
+                                                      int64_t ldr, Mod md, Digits dg,
+                                                      int32_t gn, int32_t *cand, ListState *st,
+                                                      int32_t *alive) {
+    if ((int32_t)blockIdx.x >= gn) {
+        prune_body(a, ld, n, c0, cand, st, alive, (int32_t)blockIdx.x - gn, (int32_t)gridDim.x - gn);
+        return;
+    }
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
     __shared__ __align__(16) int8_t Td[ND][PB][PB];  // T: [digit][t][s]
@@ -1772,31 +1790,34 @@ static int grid_of(int64_t n, int bs) {
 
 // element counts of the scratch arrays of an m x n echelon form, in the
 // order of DenseWs; the digit planes are empty in the plain form
-static void dense_ws_counts(int32_t m, int32_t n, int form, uint32_t p, size_t cnt[10],
-                            size_t elt[10]) {
+constexpr int NWS = 11;  // pieces of DenseWs
+
+static void dense_ws_counts(int32_t m, int32_t n, int form, uint32_t p, size_t cnt[NWS],
+                            size_t elt[NWS]) {
     const int64_t ldr = ((int64_t)n + 63) & ~63ll;
     const int nd = form == GBGPU_FORM_MFMA ? digits_for(p) : 0;
-    const size_t c[10] = {(size_t)m, (size_t)n, (size_t)m, (size_t)m, (size_t)m * PB,
-                          (size_t)PB * ldr, (size_t)nd * m * PB, (size_t)nd * ldr * PB, 2, 1};
-    const size_t e[10] = {4, 4, 4, 4, 4, 4, 1, 1, sizeof(Panel), sizeof(ListState)};
-    for (int i = 0; i < 10; ++i) {
+    const size_t c[NWS] = {(size_t)m, (size_t)n, (size_t)m, (size_t)m, (size_t)m * PB,
+                           (size_t)PB * ldr, (size_t)nd * m * PB, (size_t)nd * ldr * PB, 2, 1,
+                           (size_t)m};
+    const size_t e[NWS] = {4, 4, 4, 4, 4, 4, 1, 1, sizeof(Panel), sizeof(ListState), 4};
+    for (int i = 0; i < NWS; ++i) {
         cnt[i] = c[i];
         elt[i] = e[i];
     }
 }
 
 size_t dense_ws_bytes(int32_t m, int32_t n, int form, uint32_t p) {
-    size_t cnt[10], elt[10], tot = 0;
+    size_t cnt[NWS], elt[NWS], tot = 0;
     dense_ws_counts(m, n, form, p, cnt, elt);
-    for (int i = 0; i < 10; ++i) tot += (cnt[i] * elt[i] + 255) & ~(size_t)255;
+    for (int i = 0; i < NWS; ++i) tot += (cnt[i] * elt[i] + 255) & ~(size_t)255;
     return tot;
 }
 
 void dense_ws_carve(uint8_t *base, int32_t m, int32_t n, int form, uint32_t p, DenseWs *ws) {
-    size_t cnt[10], elt[10], off = 0;
+    size_t cnt[NWS], elt[NWS], off = 0;
     dense_ws_counts(m, n, form, p, cnt, elt);
-    void *ptr[10];
-    for (int i = 0; i < 10; ++i) {
+    void *ptr[NWS];
+    for (int i = 0; i < NWS; ++i) {
         ptr[i] = cnt[i] ? (void *)(base + off) : nullptr;
         off += (cnt[i] * elt[i] + 255) & ~(size_t)255;
     }
@@ -1810,6 +1831,7 @@ void dense_ws_carve(uint8_t *base, int32_t m, int32_t n, int form, uint32_t p, D
     ws->dr = (int8_t *)ptr[7];
     ws->pan = (Panel *)ptr[8];
     ws->st = (ListState *)ptr[9];
+    ws->alive = (int32_t *)ptr[10];
 }
 
 int dense_rref(gbgpu_dense *d, int form) {
@@ -1840,6 +1862,7 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
     const Mod md = Mod::make(d->p);
     const bool big = d->p >= (1u << 17);
     int32_t *rowpiv = ws.rowpiv, *colpiv = ws.colpiv, *active = ws.active, *cand = ws.cand;
+    int32_t *alive = ws.alive;
     uint32_t *L = ws.L, *rp = ws.rp;
     int8_t *dl = ws.dl, *dr = ws.dr;
     Panel *pan = ws.pan;
@@ -1901,6 +1924,10 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
     if (m > 0 && n > 0)
         select(s, 0, pan);
     constexpr int LAG = (int)(sizeof(ctx->lag_ev) / sizeof(ctx->lag_ev[0]));
+    // most panels prune nothing (every block returns at once): the prune's
+    // grid stops at four blocks per CU
+    const int pg = grid_of(m, 4) < 4 * ctx->num_cu ? grid_of(m, 4) : 4 * ctx->num_cu;
+    bool prune_pending = false;
     int it = 0;
     for (int32_t c0 = 0; c0 < n && m > 0; c0 += PB, ++it) {
         if (it >= LAG) {
@@ -1924,17 +1951,20 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
         // select is on its way
         const bool fused = form == GBGPU_FORM_MFMA && c1 < n;
         if (fused) {
-            const unsigned g = (unsigned)((m + PB - 1) / PB);
+            // with the last panel's prune, if one is pending, in extra blocks
+            const int32_t gn = (m + PB - 1) / PB;
+            const unsigned g = (unsigned)(gn + (prune_pending ? pg : 0));
             switch (dg.nd) {
-                case 1: k_panel_narrow<1, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg); break;
-                case 2: k_panel_narrow<2, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg); break;
+                case 1: k_panel_narrow<1, T><<<g, 256, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive); break;
+                case 2: k_panel_narrow<2, T><<<g, 256, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive); break;
                 default:
                     if constexpr (sizeof(T) == 4) {
-                        if (dg.nd == 3) k_panel_narrow<3, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg);
-                        else k_panel_narrow<4, T><<<g, 256, 0, s>>>(a, ld, m, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg);
+                        if (dg.nd == 3) k_panel_narrow<3, T><<<g, 256, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive);
+                        else k_panel_narrow<4, T><<<g, 256, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive);
                     }
                     break;
             }
+            prune_pending = false;
         } else {
             launch_rp<T>(s, true, a, ld, c0, c0, c2, cur, rp, ldr, md, r64, dg);
             k_panel_gather<T><<<grid_of(m, 4), 256, 0, s>>>(a, ld, m, c0, bw, rowpiv, cur, L, active,
@@ -1999,10 +2029,11 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
             if (hipStreamWaitEvent(s, ctx->lag_ev[it % LAG], 0) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
-            // most panels prune nothing (every block returns at once): the
-            // grid stops at four blocks per CU
-            const int pg = grid_of(m, 4) < 4 * ctx->num_cu ? grid_of(m, 4) : 4 * ctx->num_cu;
-            k_prune<T><<<pg, 256, 0, s>>>(a, ld, n, c1, cand, st, active);
+            // the prune: in the MFMA form inside the next panel's
+            // k_panel_narrow (nothing reads the list before the select after
+            // it), else its own launch
+            if (form == GBGPU_FORM_MFMA) prune_pending = true;
+            else k_prune<T><<<pg, 256, 0, s>>>(a, ld, n, c1, cand, st, alive);
         } else {
             update(s, cur, c0, c0, n);
         }

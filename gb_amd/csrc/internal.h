@@ -128,6 +128,7 @@ struct DenseWs {
     int8_t *dl, *dr;                           // digit planes (MFMA form)
     Panel *pan;                                // two panel records
     ListState *st;
+    int32_t *alive;                            // m: the prune's flags
 };
 // bytes of that scratch for an m x n block (pieces 256-byte aligned)
 size_t dense_ws_bytes(int32_t m, int32_t n, int form, uint32_t p);
