@@ -1717,16 +1717,15 @@ __global__ __launch_bounds__(256) void k_panel_narrow(T *a, int64_t ld, int32_t 
 // writes the pivot rows there from rp instead: every block of that launch
 // has read them by now.
 template <int ND, typename T>
-__global__ __launch_bounds__(256) void k_panel_rp_mfma(T *a, int64_t ld, int32_t c0, int32_t jhi,
-                                                       int32_t wb, const Panel *pan,
-                                                       const uint32_t *rp, int64_t ldr, Mod md,
-                                                       Digits dg) {
+__device__ __forceinline__ void rp_mfma_body(T *a, int64_t ld, int32_t c0, int32_t jhi, int32_t wb,
+                                             const Panel *pan, const uint32_t *rp, int64_t ldr,
+                                             const Mod &md, const Digits &dg, int32_t blk) {
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const uint32_t p = md.p;
-    const int32_t q0 = c0 + (int32_t)blockIdx.x * PB;
+    const int32_t q0 = c0 + blk * PB;
     if (q0 == wb) {
         for (int i = tid; i < kk * PB; i += 256) {
             const int t = i >> 6, j = i & 63;
@@ -1779,6 +1778,14 @@ __global__ __launch_bounds__(256) void k_panel_rp_mfma(T *a, int64_t ld, int32_t
                 *(uint32_t *)(dg.R + i * dg.rplane + (int64_t)(col - c0) * 64 + 16 * wv + 4 * lk) = wd[i];
         }
     }
+}
+
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void k_panel_rp_mfma(T *a, int64_t ld, int32_t c0, int32_t jhi,
+                                                       int32_t wb, const Panel *pan,
+                                                       const uint32_t *rp, int64_t ldr, Mod md,
+                                                       Digits dg) {
+    rp_mfma_body<ND, T>(a, ld, c0, jhi, wb, pan, rp, ldr, md, dg, (int32_t)blockIdx.x);
 }
 
 static int grid_of(int64_t n, int bs) {
@@ -2199,6 +2206,17 @@ __global__ __launch_bounds__(SEL_T) void k_batch_select(const DenseDesc *D, int3
                    nullptr);
 }
 
+// the MFMA form's Rp of every matrix: a block per 64 columns, on the matrix
+// cores (the plain Rp only the plain form's update reads is not written)
+template <int ND>
+__global__ __launch_bounds__(256) void k_batch_rp_mfma(const DenseDesc *D, const int32_t *pre,
+                                                       int32_t nmat, int32_t c0) {
+    const int i = tab_find(pre, nmat, blockIdx.x);
+    const DenseDesc &d = D[i];
+    rp_mfma_body<ND, uint32_t>(d.a, d.ld, c0, d.n, INT_MIN, d.pan, d.rp, d.ldr, d.md, d.dg,
+                               (int32_t)blockIdx.x - pre[i]);
+}
+
 __global__ __launch_bounds__(256) void k_batch_rp(const DenseDesc *D, const int32_t *pre,
                                                   int32_t nmat, int32_t c0) {
     const int i = tab_find(pre, nmat, blockIdx.x);
@@ -2347,15 +2365,24 @@ int batch_dense_step(gbgpu_ctx *ctx, const DenseDesc *D, int32_t nmat, const int
                           : (big ? (const void *)k_batch_update_plain<true>
                                  : (const void *)k_batch_update_plain<false>);
     if (!upd) return set_error(ctx, GBGPU_EINVAL, "no batch update kernel of that shape");
+    const void *rpk = form != GBGPU_FORM_MFMA ? (const void *)k_batch_rp
+                      : nd == 1 ? (const void *)k_batch_rp_mfma<1>
+                      : nd == 2 ? (const void *)k_batch_rp_mfma<2>
+                      : nd == 3 ? (const void *)k_batch_rp_mfma<3>
+                                : (const void *)k_batch_rp_mfma<4>;
     if ((rc = check_launch(ctx, (const void *)k_batch_select, nmat, SEL_T, "batch select")) ||
-        (nrp > 0 && (rc = check_launch(ctx, (const void *)k_batch_rp, nrp, 256, "batch rp"))) ||
+        (nrp > 0 && (rc = check_launch(ctx, rpk, nrp, 256, "batch rp"))) ||
         (nga > 0 && (rc = check_launch(ctx, (const void *)k_batch_gather, nga, 256, "batch gather"))) ||
         (nup > 0 && (rc = check_launch(ctx, upd, nup, 256, "batch update"))) ||
         (npr > 0 && (rc = check_launch(ctx, (const void *)k_batch_prune_flags, npr, 256, "batch prune"))) ||
         (npr > 0 && (rc = check_launch(ctx, (const void *)k_batch_prune_compact, nmat, 256, "batch prune"))))
         return rc;
     k_batch_select<<<nmat, SEL_T, 0, s>>>(D, c0);
-    if (nrp > 0) k_batch_rp<<<nrp, 256, 0, s>>>(D, pre, nmat, c0);
+    if (nrp > 0) {
+        int32_t nm = nmat, cc = c0;
+        void *args[] = {(void *)&D, (void *)&pre, (void *)&nm, (void *)&cc};
+        GBGPU_HIP(ctx, hipLaunchKernel(rpk, dim3((unsigned)nrp), dim3(256), args, 0, s));
+    }
     if (nga > 0) k_batch_gather<<<nga, 256, 0, s>>>(D, pre + (nmat + 1), nmat, c0);
     if (nup > 0) {
         const int32_t *pu = pre + 2 * (nmat + 1);
