@@ -773,16 +773,16 @@ __device__ __forceinline__ void prune_flags_body(const T *a, int64_t ld, int32_t
 
 __device__ __forceinline__ void prune_compact_body(int32_t *cand, ListState *st,
                                                    const int32_t *alive, bool coherent = false) {
-    __shared__ int32_t wsum[4];
+    __shared__ int32_t wsum[16];
     const int32_t nzero = __builtin_amdgcn_readfirstlane(st->nzero);
     const int32_t done = __builtin_amdgcn_readfirstlane(st->done);
     if (nzero < PB || done) return;
     const int32_t head = __builtin_amdgcn_readfirstlane(st->head);
     const int32_t count = __builtin_amdgcn_readfirstlane(st->count);
-    const int tid = threadIdx.x, wv = tid >> 6;
+    const int tid = threadIdx.x, wv = tid >> 6, nw = (int)(blockDim.x >> 6);
     int32_t out = head;
     // stable in-place compaction: writes never pass the reads of later chunks
-    for (int32_t base = head; base < count; base += 256) {
+    for (int32_t base = head; base < count; base += (int32_t)blockDim.x) {
         const int32_t i = base + tid;
         const int f = i < count ? (coherent ? __hip_atomic_load(&alive[i - head], __ATOMIC_RELAXED,
                                                                   __HIP_MEMORY_SCOPE_AGENT)
@@ -793,7 +793,7 @@ __device__ __forceinline__ void prune_compact_body(int32_t *cand, ListState *st,
         wsum[wv] = __shfl(incl, 63, 64);  // every lane stores the wave total
         __syncthreads();
         int off = 0, tot = 0;
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < nw; ++q) {
             const int ws = wsum[q];
             if (q < wv) off += ws;
             tot += ws;
@@ -1502,17 +1502,17 @@ __global__ void k_dense_init(int32_t m, int32_t n, int32_t *rowpiv, int32_t *col
 // (Pd[i][j][s]); erow in LDS.  A thread builds four consecutive s of one row
 // t and of one column j at a time (a wave reads 64 consecutive columns of a
 // pivot row); every load is in flight before the first is used.
-template <int ND, typename T>
+template <int ND, typename T, int NT>
 __device__ __forceinline__ void tile_digits(int8_t (*Td)[PB][PB], int8_t (*Pd)[PB][PB],
                                             const Panel *pan, const T *a, int64_t ld,
                                             const int32_t *erow, int32_t q0, int32_t qe, int kk,
                                             uint32_t p, uint32_t hi) {
     const int tid = threadIdx.x;
-    constexpr int IT = PB * PB / 4 / 256;  // four-element pieces per thread
+    constexpr int IT = PB * PB / 4 / NT;  // four-element pieces per thread (NT threads)
     uint32_t tv[IT][4], pv[IT][4];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-        const int u = tid + 256 * it;
+        const int u = tid + NT * it;
         // T holds zeros past kk in both indices
 #pragma unroll
         for (int b = 0; b < 4; ++b) tv[it][b] = pan->T[(u >> 4) * PB + (u & 15) * 4 + b];
@@ -1523,7 +1523,7 @@ __device__ __forceinline__ void tile_digits(int8_t (*Td)[PB][PB], int8_t (*Pd)[P
     }
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-        const int u = tid + 256 * it;
+        const int u = tid + NT * it;
         uint32_t wt[ND] = {}, wp[ND] = {};
         const int j = u & 63, s4 = (u >> 6) * 4;
 #pragma unroll
@@ -1565,14 +1565,16 @@ __device__ __forceinline__ void tile_digits(int8_t (*Td)[PB][PB], int8_t (*Pd)[P
 // the list is not read here, and a row this launch updates has a nonzero
 // multiplier in [c0, c1), which it does not write, so the prune's verdict
 // does not depend on the order of the two).
+constexpr int NAR_T = 512;  // threads of a k_panel_narrow block (eight waves)
+
 template <int ND, typename T>
-__global__ __launch_bounds__(256) void k_panel_narrow(T *a, int64_t ld, int32_t m, int32_t n,
-                                                      int32_t c0, int32_t c1, int32_t c2,
-                                                      const int32_t *rowpiv, const Panel *pan,
-                                                      int32_t *active, uint32_t *rp,
-                                                      int64_t ldr, Mod md, Digits dg,
-                                                      int32_t gn, int32_t *cand, ListState *st,
-                                                      int32_t *alive) {
+__global__ __launch_bounds__(NAR_T) void k_panel_narrow(T *a, int64_t ld, int32_t m, int32_t n,
+                                                        int32_t c0, int32_t c1, int32_t c2,
+                                                        const int32_t *rowpiv, const Panel *pan,
+                                                        int32_t *active, uint32_t *rp,
+                                                        int64_t ldr, Mod md, Digits dg,
+                                                        int32_t gn, int32_t *cand, ListState *st,
+                                                        int32_t *alive) {
     if ((int32_t)blockIdx.x >= gn) {
         prune_body(a, ld, n, c0, cand, st, alive, (int32_t)blockIdx.x - gn, (int32_t)gridDim.x - gn);
         return;
@@ -1583,29 +1585,33 @@ __global__ __launch_bounds__(256) void k_panel_narrow(T *a, int64_t ld, int32_t 
     __shared__ __align__(16) int8_t Pd[ND][PB][PB];  // pivot rows, transposed: [digit][j][s]
     __shared__ __align__(16) int8_t Rd[ND][PB][PB];  // Rp, transposed: [digit][j][t]
     __shared__ __align__(16) int8_t Ld[ND][PB][PB];  // multipliers: [digit][row][t]
-    __shared__ int32_t erow_s[PB], ecol_s[PB];
+    __shared__ int32_t erow_s[PB], ecol_s[PB], act_s[PB];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const uint32_t p = md.p;
     const int32_t w2 = c2 - c1;  // columns of the next panel (1 .. 64)
     const int32_t r0 = (int32_t)blockIdx.x * PB;
+    // wave wv: the rows 8 wv .. 8 wv + 7 of the block in step 2; in steps
+    // 1b and 3 the 16 x 16 tiles of row (or t) block tb and column blocks
+    // cb0, cb0 + 1
+    const int tb = wv & 3, cb0 = 2 * (wv >> 2);
     // loads that need nothing computed here go out first: the pivot
     // columns of this wave's rows and this lane's entries of step 3 (rows
-    // 16 wv + 4 (lane / 16) + e, columns c1 + 16 cb + lane % 16)
-    int32_t rpv[16];
+    // 16 tb + 4 (lane / 16) + e, columns c1 + 16 cb + lane % 16)
+    int32_t rpv[8];
 #pragma unroll
-    for (int h = 0; h < 16; ++h) {
-        const int32_t r = r0 + 16 * wv + h;
+    for (int h = 0; h < 8; ++h) {
+        const int32_t r = r0 + 8 * wv + h;
         rpv[h] = r < m ? rowpiv[r] : 0;
     }
-    uint32_t xv[4][4];
+    uint32_t xv[4][2];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        const int32_t r = r0 + 16 * wv + 4 * lk + e;
+        const int32_t r = r0 + 16 * tb + 4 * lk + e;
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-            const int32_t j = 16 * cb + lr;
-            xv[e][cb] = (r < m && j < w2) ? (uint32_t)a[(int64_t)r * ld + c1 + j] : 0u;
+        for (int q = 0; q < 2; ++q) {
+            const int32_t j = 16 * (cb0 + q) + lr;
+            xv[e][q] = (r < m && j < w2) ? (uint32_t)a[(int64_t)r * ld + c1 + j] : 0u;
         }
     }
     if (tid < PB) {
@@ -1617,37 +1623,36 @@ __global__ __launch_bounds__(256) void k_panel_narrow(T *a, int64_t ld, int32_t 
     // rows (a thread: column j, four consecutive s; a wave reads 64
     // consecutive columns of one row); every load in flight before the
     // first is used
-    tile_digits<ND, T>(Td, Pd, pan, a, ld, erow_s, c1, c1 + w2, kk, p, dg.hi);
-    // 2a. this wave's rows of the block (16), their multipliers' loads in
-    // flight together
-    uint32_t lv[16];
+    tile_digits<ND, T, NAR_T>(Td, Pd, pan, a, ld, erow_s, c1, c1 + w2, kk, p, dg.hi);
+    // 2a. this wave's rows' multipliers, their loads in flight together
+    uint32_t lv[8];
 #pragma unroll
-    for (int h = 0; h < 16; ++h) {
-        const int32_t r = r0 + 16 * wv + h;
+    for (int h = 0; h < 8; ++h) {
+        const int32_t r = r0 + 8 * wv + h;
         const bool on = r < m && !(rpv[h] >= c0 && rpv[h] < c1);
         lv[h] = (on && lane < kk) ? (uint32_t)a[(int64_t)r * ld + ecol_s[lane]] : 0u;
     }
     __syncthreads();
-    // 1b. Rp^T: wave wv takes rows t of 16 wv .. (the transform's), all 64
-    // columns j; result register e of lane l: t = 16 wv + 4 (l / 16) + e,
+    // 1b. Rp^T; result register e of lane l: t = 16 tb + 4 (l / 16) + e,
     // j = 16 cb + l % 16
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
+    for (int q = 0; q < 2; ++q) {
+        const int cb = cb0 + q;
         v4i acc[2 * ND - 1];
 #pragma unroll
-        for (int q = 0; q < 2 * ND - 1; ++q) acc[q] = v4i{0, 0, 0, 0};
+        for (int u = 0; u < 2 * ND - 1; ++u) acc[u] = v4i{0, 0, 0, 0};
 #pragma unroll
         for (int i = 0; i < ND; ++i)
 #pragma unroll
             for (int jd = 0; jd < ND; ++jd)
                 acc[i + jd] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                    *(const v4i *)&Td[i][16 * wv + lr][16 * lk],
+                    *(const v4i *)&Td[i][16 * tb + lr][16 * lk],
                     *(const v4i *)&Pd[jd][16 * cb + lr][16 * lk], acc[i + jd], 0, 0, 0);
         const int j = 16 * cb + lr;
         uint32_t wd[ND] = {};
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int t = 16 * wv + 4 * lk + e;
+            const int t = 16 * tb + 4 * lk + e;
             uint32_t r;
             if constexpr (ND == 2) {
                 const int32_t v = digits_balanced2(acc, e, (int32_t)p, dg.pinvf);
@@ -1662,16 +1667,15 @@ __global__ __launch_bounds__(256) void k_panel_narrow(T *a, int64_t ld, int32_t 
             for (int i = 0; i < ND; ++i) wd[i] |= ((d4 >> (8 * i)) & 0xFFu) << (8 * e);
         }
 #pragma unroll
-        for (int i = 0; i < ND; ++i) *(uint32_t *)&Rd[i][j][16 * wv + 4 * lk] = wd[i];
+        for (int i = 0; i < ND; ++i) *(uint32_t *)&Rd[i][j][16 * tb + 4 * lk] = wd[i];
     }
-    // 2b. the multipliers (plain, digit planes, LDS) and the active flags
-    bool xa[4] = {false, false, false, false};
+    // 2b. the multipliers (digit planes, LDS) and the active flags
 #pragma unroll
-    for (int h = 0; h < 16; ++h) {
-        const int32_t r = r0 + 16 * wv + h;
+    for (int h = 0; h < 8; ++h) {
+        const int32_t r = r0 + 8 * wv + h;
         const uint32_t v = lv[h];
         const uint32_t d4 = digits_n<ND>(v, p, dg.hi);
-        const bool act = __any(v != 0u);
+        const bool act = __any(v != 0u) && r < m;
         if (r < m) {
 #pragma unroll
             for (int i = 0; i < ND; ++i)
@@ -1679,32 +1683,30 @@ __global__ __launch_bounds__(256) void k_panel_narrow(T *a, int64_t ld, int32_t 
             active[r] = act ? 1 : 0;
         }
 #pragma unroll
-        for (int i = 0; i < ND; ++i) Ld[i][16 * wv + h][lane] = (int8_t)((d4 >> (8 * i)) & 0xFFu);
-        // this lane's output rows of step 3 are 16 wv + 4 (lane / 16) + e
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-            if (h == 4 * lk + e) xa[e] = act && r < m;
+        for (int i = 0; i < ND; ++i) Ld[i][8 * wv + h][lane] = (int8_t)((d4 >> (8 * i)) & 0xFFu);
+        if (lane == 0) act_s[8 * wv + h] = act ? 1 : 0;
     }
-    // 3. A[r][c1, c2) -= L[r] Rp for this wave's active rows
+    // 3. A[r][c1, c2) -= L[r] Rp for the active rows
     __syncthreads();
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
+    for (int q = 0; q < 2; ++q) {
+        const int cb = cb0 + q;
         v4i acc[2 * ND - 1];
 #pragma unroll
-        for (int q = 0; q < 2 * ND - 1; ++q) acc[q] = v4i{0, 0, 0, 0};
+        for (int u = 0; u < 2 * ND - 1; ++u) acc[u] = v4i{0, 0, 0, 0};
 #pragma unroll
         for (int i = 0; i < ND; ++i)
 #pragma unroll
             for (int jd = 0; jd < ND; ++jd)
                 acc[i + jd] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                    *(const v4i *)&Ld[i][16 * wv + lr][16 * lk],
+                    *(const v4i *)&Ld[i][16 * tb + lr][16 * lk],
                     *(const v4i *)&Rd[jd][16 * cb + lr][16 * lk], acc[i + jd], 0, 0, 0);
         const int32_t j = 16 * cb + lr;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int32_t r = r0 + 16 * wv + 4 * lk + e;
-            if (xa[e] && j < w2)
-                a[(int64_t)r * ld + c1 + j] = (T)digits_sub<ND>(xv[e][cb], acc, e, md, dg);
+            const int32_t r = r0 + 16 * tb + 4 * lk + e;
+            if (act_s[16 * tb + 4 * lk + e] && j < w2)
+                a[(int64_t)r * ld + c1 + j] = (T)digits_sub<ND>(xv[e][q], acc, e, md, dg);
         }
     }
 }
@@ -1738,7 +1740,7 @@ __device__ __forceinline__ void rp_mfma_body(T *a, int64_t ld, int32_t c0, int32
     __shared__ int32_t erow_s[PB];
     if (tid < PB) erow_s[tid] = tid < kk ? pan->erow[tid] : 0;
     __syncthreads();
-    tile_digits<ND, T>(Td, Pd, pan, a, ld, erow_s, q0, jhi, kk, p, dg.hi);
+    tile_digits<ND, T, 256>(Td, Pd, pan, a, ld, erow_s, q0, jhi, kk, p, dg.hi);
     __syncthreads();  // every read of the pivot rows is done before they are written
     // wave wv: rows t of 16 wv .., all 64 columns; register e of lane l:
     // t = 16 wv + 4 (l / 16) + e, column 16 cb + l % 16
@@ -1962,12 +1964,12 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
             const int32_t gn = (m + PB - 1) / PB;
             const unsigned g = (unsigned)(gn + (prune_pending ? pg : 0));
             switch (dg.nd) {
-                case 1: k_panel_narrow<1, T><<<g, 256, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive); break;
-                case 2: k_panel_narrow<2, T><<<g, 256, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive); break;
+                case 1: k_panel_narrow<1, T><<<g, NAR_T, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive); break;
+                case 2: k_panel_narrow<2, T><<<g, NAR_T, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive); break;
                 default:
                     if constexpr (sizeof(T) == 4) {
-                        if (dg.nd == 3) k_panel_narrow<3, T><<<g, 256, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive);
-                        else k_panel_narrow<4, T><<<g, 256, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive);
+                        if (dg.nd == 3) k_panel_narrow<3, T><<<g, NAR_T, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive);
+                        else k_panel_narrow<4, T><<<g, NAR_T, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive);
                     }
                     break;
             }
