@@ -139,6 +139,16 @@ inline void flush_later(gbgpu_ctx *ctx) {
     ctx->later.clear();
 }
 
+// in[n - 1] + out[n - 1] (the sum of an exclusive scan's input) and a flag
+// word into the context's mapped host block
+template <typename I>
+__global__ void k_scan_total(const I *in, const I *out, int64_t n, const int32_t *flag,
+                             int64_t *dst) {
+    volatile int64_t *d = dst;
+    d[0] = n > 0 ? (int64_t)in[n - 1] + (int64_t)out[n - 1] : 0;
+    d[1] = flag ? *flag : 0;
+}
+
 // exclusive prefix sum of n values into out (not aliasing in); with total
 // given, waits for the stream once and returns the sum of the values in
 // *total, and a device word dflag (if given) in *hflag; with total null it

@@ -2132,17 +2132,24 @@ int dense_output(gbgpu_dense *d, const int32_t *colpiv) {
         return rc;
     }
     if (n > 0) k_out_flags<<<grid_of(n, 256), 256, 0, s>>>(n, colpiv, flag);
-    int64_t total = 0;
-    if ((rc = exclusive_scan_i32(ctx, flag, rank, n, &total))) { dfree(flag); dfree(rank); return rc; }
-    d->rank = (int32_t)total;
-    if ((rc = dalloc(ctx, &d->orow, (size_t)total)) || (rc = dalloc(ctx, &d->olead, (size_t)total))) {
+    // the lists sized for a full rank, so that the rank is read in the one
+    // wait for the stream, with the lists written
+    if ((rc = exclusive_scan_i32(ctx, flag, rank, n, nullptr)) ||
+        (rc = dalloc(ctx, &d->orow, (size_t)n)) || (rc = dalloc(ctx, &d->olead, (size_t)n))) {
+        (void)hipStreamSynchronize(s);
         dfree(flag); dfree(rank); return rc;
     }
-    if (n > 0) k_out_list<<<grid_of(n, 256), 256, 0, s>>>(n, colpiv, flag, rank, d->orow, d->olead);
-    hipError_t e = hipStreamSynchronize(s);
+    if (n > 0) {
+        k_out_list<<<grid_of(n, 256), 256, 0, s>>>(n, colpiv, flag, rank, d->orow, d->olead);
+        k_scan_total<int32_t><<<1, 1, 0, s>>>(flag, rank, n, nullptr, ctx->dscan);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    flush_later(ctx);
     dfree(flag);
     dfree(rank);
     if (e != hipSuccess) return check_hip(ctx, e, "dense output");
+    d->rank = n > 0 ? (int32_t)ctx->hscan[0] : 0;
     return GBGPU_OK;
 }
 
