@@ -1443,9 +1443,11 @@ void launch_update_i8(int nd, int kc, hipStream_t s, T *a, int64_t ld, int32_t m
         }
         return;
     }
-    dim3 grid((unsigned)((ce - cb + 63) / 64), (unsigned)((m + 63) / 64));
-    if (nd == 1) k_update_mfma<1, 1, 2, 2, T><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg);
-    else if (nd == 2 || W16) k_update_mfma<2, 1, 2, 2, T><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg);
+    // one and two digits: 32 x 64 block tiles, as in the panel loop
+    const int rows = (nd <= 2 || W16) ? 32 : 64;
+    dim3 grid((unsigned)((ce - cb + 63) / 64), (unsigned)((m + rows - 1) / rows));
+    if (nd == 1) k_update_mfma<1, 1, 1, 2, T><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg);
+    else if (nd == 2 || W16) k_update_mfma<2, 1, 1, 2, T><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg);
     else if constexpr (!W16) {
         if (nd == 3) k_update_mfma<3><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg);
         else k_update_mfma<4><<<grid, 256, 0, s>>>(a, ld, m, cb, ce, c0, pan, active, md, dg);
@@ -1892,10 +1894,13 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
                       int32_t gb = INT_MAX, int32_t gsz = 0) {
         if (cb >= ce) return;
         if (form == GBGPU_FORM_MFMA) {
-            dim3 g((unsigned)((ce - cb - gsz + 63) / 64), (unsigned)((m + 63) / 64));
+            // one and two digits: 32 x 64 block tiles (16 x 32 per wave:
+            // fewer registers, more waves in flight); three and four: 64 x 64
+            const int rows = dg.nd <= 2 ? 32 : 64;
+            dim3 g((unsigned)((ce - cb - gsz + 63) / 64), (unsigned)((m + rows - 1) / rows));
             switch (dg.nd) {
-                case 1: k_update_mfma<1, 1, 2, 2, T><<<g, 256, 0, s>>>(a, ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz); break;
-                case 2: k_update_mfma<2, 1, 2, 2, T><<<g, 256, 0, s>>>(a, ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz); break;
+                case 1: k_update_mfma<1, 1, 1, 2, T><<<g, 256, 0, s>>>(a, ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz); break;
+                case 2: k_update_mfma<2, 1, 1, 2, T><<<g, 256, 0, s>>>(a, ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz); break;
                 default:
                     if constexpr (sizeof(T) == 4) {
                         if (dg.nd == 3) k_update_mfma<3><<<g, 256, 0, s>>>(a, ld, m, cb, ce, c0, pp, active, md, dg, gb, gsz);
