@@ -181,6 +181,10 @@ int gbgpu_init(int device, gbgpu_ctx **out) {
     *ctx->hdone = 0;
     ctx->hscan = (volatile int64_t *)((char *)hflag + 64);
     *ctx->hscan = 0;
+    ctx->hprog = (volatile int32_t *)((char *)hflag + 128);
+    *ctx->hprog = 0;
+    ctx->herr = (volatile int32_t *)((char *)hflag + 192);
+    *ctx->herr = 0;
     void *dflag = nullptr;
     if (hipHostGetDevicePointer(&dflag, hflag, 0) != hipSuccess) {
         (void)hipGetLastError();
@@ -189,6 +193,17 @@ int gbgpu_init(int device, gbgpu_ctx **out) {
     }
     ctx->ddone = (int32_t *)dflag;
     ctx->dscan = (int64_t *)((char *)dflag + 64);
+    ctx->dprog = (int32_t *)((char *)dflag + 128);
+    ctx->derr = (int32_t *)((char *)dflag + 192);
+    // zero before any stream of the context runs (hipMemset may return
+    // before it has run, and the streams do not wait for the null stream)
+    if (hipMalloc(&ctx->dsync, 512) != hipSuccess ||
+        hipMemsetAsync(ctx->dsync, 0, 512, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        gbgpu_finalize(ctx);
+        return GBGPU_ENOMEM;
+    }
     *out = ctx;
     return GBGPU_OK;
 }
@@ -206,6 +221,7 @@ void gbgpu_finalize(gbgpu_ctx *ctx) {
     for (auto &e : ctx->lag_ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->hdone) (void)hipHostFree((void *)ctx->hdone);
+    if (ctx->dsync) (void)hipFree(ctx->dsync);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -85,6 +85,15 @@ struct gbgpu_ctx {
     // a scan's total, written by the device into the same mapped block
     volatile int64_t *hscan = nullptr;
     int64_t *dscan = nullptr;
+    // the dense panel loop's MFMA form orders its two streams through device
+    // words instead of events (dsync, 512 bytes, zero at start; values of a
+    // sequence that grows by one per panel, sync_seq the next free one); the
+    // side stream's last panel done and a wait that ran out of its bound
+    // reach the host through the same mapped block
+    int32_t *dsync = nullptr;
+    int32_t sync_seq = 1;
+    volatile int32_t *hprog = nullptr, *herr = nullptr;
+    int32_t *dprog = nullptr, *derr = nullptr;
     // buffers freed while queued work of this context may still read them:
     // they go back to the pool at the next wait for the stream
     std::vector<void *> later;

@@ -24,9 +24,13 @@
 // multiply-add), MFMA form on the FP64 matrix cores with the exactness
 // bound k (p-1)^2 < 2^53 (p < 2^17) or, for larger p, 16-bit operand
 // splitting (four exact products recombined in integers).
+#include <chrono>
 #include <climits>
+#include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 
 #include "internal.h"
@@ -51,6 +55,59 @@ struct ListState {
     int32_t nzero;  // candidates found zero on their panel since the last prune
     int32_t pcount; // blocks of a prune that have written their flags
 };
+
+// Ordering of the panel loop's two streams through device words (the MFMA
+// form).  A marker and a wait between the streams cost several
+// microseconds of dispatch each, twice a panel, on the loop's critical
+// path; instead a kernel posts a word of the loop's sequence (s0 + panel)
+// and one thread of the kernel that depends on it waits for it:
+//   w[0]   the k_panel_narrow launch of panel j has begun, so the select of
+//          panel j before it on the main stream is complete;
+//   w[32]  the select of panel j has begun, so the k_panel_narrow launch of
+//          panel j - 1 before it is complete;
+//   w[64]  the side stream's work of panel j is complete;
+//   w[96]  a wait ran out of its bound: the s0 of that loop.
+// Only single blocks wait -- a one-block launch on the side stream, one
+// block of its Rp launch and a select -- and what each waits for never
+// waits on the side stream's later work, nor for the device to have room
+// for more than one waiting block, so the waits end; each is still bounded, and
+// running out of the bound marks the loop failed (herr) and makes every
+// later wait of the loop return at once.  Releases and acquires are at agent
+// scope: the words and the data they publish cross the XCDs' L2s.
+struct LoopSync {
+    int32_t *w = nullptr;      // nullptr: the loop orders its streams by events
+    int32_t *herr = nullptr;   // mapped host word, 1 once a wait ran out
+    int32_t *hprog = nullptr;  // mapped host word, the last w[64] posted
+    int32_t s0 = 0;
+};
+constexpr int LS_NAR = 0, LS_SEL = 32, LS_SIDE = 64, LS_ABORT = 96;
+constexpr uint32_t LS_SPIN = 1u << 22;  // polls of a wait (about a microsecond each)
+
+__device__ __forceinline__ bool seq_reached(int32_t v, int32_t want) {
+    return (int32_t)((uint32_t)v - (uint32_t)want) >= 0;
+}
+
+__device__ __forceinline__ void ls_post(const LoopSync &ls, int at, int32_t v) {
+    __hip_atomic_store(ls.w + at, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// one thread: until w[at] reaches want (polled by coherent loads that leave
+// the caches alone, one acquire once it has)
+__device__ void ls_wait(const LoopSync &ls, int at, int32_t want) {
+    for (uint32_t i = 0;; ++i) {
+        if (seq_reached(__hip_atomic_load(ls.w + at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), want)) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            return;
+        }
+        if (seq_reached(__hip_atomic_load(ls.w + LS_ABORT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), ls.s0)) return;
+        if (i >= LS_SPIN) {
+            __hip_atomic_store(ls.w + LS_ABORT, ls.s0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (ls.herr) __hip_atomic_store(ls.herr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
 
 __device__ uint32_t inv_mod(uint32_t a, uint32_t p) {
     // extended Euclid on 32-bit operands (tools.c:83-110 computes the same
@@ -735,9 +792,16 @@ __global__ __launch_bounds__(SEL_T) void k_panel_select(const T *a, int64_t ld, 
                                                         int32_t c0, int32_t *cand,
                                                         int32_t *rowpiv, int32_t *colpiv,
                                                         Panel *pan, ListState *st, Mod md,
-                                                        Digits dg, int32_t *hdone) {
+                                                        Digits dg, int32_t *hdone,
+                                                        LoopSync ls = LoopSync(), int32_t j = 0,
+                                                        int32_t wside = -1) {
     __shared__ SelShared sh;
+    // with ls.w: panel j's select has begun (the launch before it on the
+    // main stream is complete); at the end, the main stream's next launch
+    // (after this one) waits for the side stream's work of panel wside
+    if (ls.w && threadIdx.x == 0) ls_post(ls, LS_SEL, ls.s0 + j);
     select_body<P16, T>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg, hdone);
+    if (ls.w && wside >= 0 && threadIdx.x == 0) ls_wait(ls, LS_SIDE, ls.s0 + wside);
 }
 
 // ---------------------------------------------------------------- prune
@@ -1576,7 +1640,11 @@ __global__ __launch_bounds__(NAR_T) void k_panel_narrow(T *a, int64_t ld, int32_
                                                         int32_t *active, uint32_t *rp,
                                                         int64_t ldr, Mod md, Digits dg,
                                                         int32_t gn, int32_t *cand, ListState *st,
-                                                        int32_t *alive) {
+                                                        int32_t *alive, LoopSync ls = LoopSync(),
+                                                        int32_t jp = 0) {
+    // with ls.w: panel jp's launch has begun, so the select before it is
+    // complete (the side stream's Rp of panel jp waits for this)
+    if (ls.w && blockIdx.x == 0 && threadIdx.x == 0) ls_post(ls, LS_NAR, ls.s0 + jp);
     if ((int32_t)blockIdx.x >= gn) {
         prune_body(a, ld, n, c0, cand, st, alive, (int32_t)blockIdx.x - gn, (int32_t)gridDim.x - gn);
         return;
@@ -1792,6 +1860,54 @@ __global__ __launch_bounds__(256) void k_panel_rp_mfma(T *a, int64_t ld, int32_t
     rp_mfma_body<ND, T>(a, ld, c0, jhi, wb, pan, rp, ldr, md, dg, (int32_t)blockIdx.x);
 }
 
+// the side stream's first launch of panel j in the loop ordered by device
+// words: one thread waits until panel j's k_panel_narrow launch has begun,
+// so the select of panel j that the Rp launch after this one reads is
+// complete
+__global__ void k_side_wait(LoopSync ls, int32_t j) {
+    if (threadIdx.x == 0) ls_wait(ls, LS_NAR, ls.s0 + j);
+}
+
+// k_panel_rp_mfma for the loop ordered by device words (panel j, after
+// k_side_wait): the block at column wb (the next panel's, whose pivot rows
+// k_side_done writes) has no Rp to compute and instead waits until the next
+// select has begun, so that this launch ends only after panel j's
+// k_panel_narrow launch, whose multipliers the update after it reads.
+// Nothing this launch writes is read by that launch: the pivot rows'
+// entries outside [wb, wb + 64).
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void k_panel_rp_look(T *a, int64_t ld, int32_t c0, int32_t jhi,
+                                                       int32_t wb, const Panel *pan,
+                                                       const uint32_t *rp, int64_t ldr, Mod md,
+                                                       Digits dg, LoopSync ls, int32_t j) {
+    if (c0 + (int32_t)blockIdx.x * PB == wb) {
+        if (threadIdx.x == 0) ls_wait(ls, LS_SEL, ls.s0 + j + 1);
+        return;
+    }
+    rp_mfma_body<ND, T>(a, ld, c0, jhi, -1, pan, rp, ldr, md, dg, (int32_t)blockIdx.x);
+}
+
+// the end of the side stream's work of panel j: the pivot rows' entries of
+// the next panel's columns [c1, c2) from rp (every reader of the old ones,
+// panel j's k_panel_narrow launch, is complete), then w[64] and the host's
+// copy of it
+template <typename T>
+__global__ __launch_bounds__(256) void k_side_done(T *a, int64_t ld, int32_t c0, int32_t c1,
+                                                   int32_t c2, const Panel *pan, const uint32_t *rp,
+                                                   int64_t ldr, LoopSync ls, int32_t j) {
+    const int kk = pan->k;
+    for (int i = threadIdx.x; i < kk * PB; i += 256) {
+        const int t = i >> 6, jj = i & 63;
+        if (c1 + jj < c2) a[(int64_t)pan->erow[t] * ld + c1 + jj] = (T)rp[(int64_t)t * ldr + (c1 + jj - c0)];
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ls_post(ls, LS_SIDE, ls.s0 + j);
+        if (ls.hprog) __hip_atomic_store(ls.hprog, ls.s0 + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 static int grid_of(int64_t n, int bs) {
     int64_t g = (n + bs - 1) / bs;
     if (g < 1) g = 1;
@@ -1925,18 +2041,50 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
     // between two slots; the candidate list is touched by the select and the
     // prune only, both on the main stream.
     hipStream_t x = ctx->side;
-    auto select = [&](hipStream_t sx, int32_t cs, Panel *pp) {
+    // the MFMA form orders the two streams through device words (unless
+    // GBGPU_DENSE_SYNC=events asks for events, e.g. where a tool runs the
+    // kernels one at a time: a wait for a kernel queued behind it would run
+    // out of its bound); this loop's words are s0 .. s0 + panels
+    LoopSync ls;
+    if (form == GBGPU_FORM_MFMA) {
+        const char *e = getenv("GBGPU_DENSE_SYNC");
+        if (!(e && strcmp(e, "events") == 0)) {
+            ls.w = ctx->dsync;
+            ls.herr = ctx->derr;
+            ls.hprog = ctx->dprog;
+            ls.s0 = ctx->sync_seq;
+            ctx->sync_seq += n / PB + 4;
+            *ctx->herr = 0;
+        }
+    }
+    auto select = [&](hipStream_t sx, int32_t cs, Panel *pp, int32_t j, int32_t wside) {
         if (d->p < (1u << 16))
-            k_panel_select<1, T><<<1, SEL_T, 0, sx>>>(a, ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg, ctx->ddone);
+            k_panel_select<1, T><<<1, SEL_T, 0, sx>>>(a, ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg, ctx->ddone, ls, j, wside);
         else
-            k_panel_select<0, T><<<1, SEL_T, 0, sx>>>(a, ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg, ctx->ddone);
+            k_panel_select<0, T><<<1, SEL_T, 0, sx>>>(a, ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg, ctx->ddone, ls, j, wside);
+    };
+    // the host stays at most LAG panels ahead: with the device words it
+    // reads the side stream's last panel done from mapped memory
+    auto wait_side = [&](int32_t j) -> int {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint32_t i = 0; (int32_t)((uint32_t)*ctx->hprog - (uint32_t)(ls.s0 + j)) < 0; ++i) {
+            if ((i & 4095u) == 4095u) {
+                const hipError_t e = hipStreamQuery(x);
+                if (e != hipSuccess && e != hipErrorNotReady) return check_hip(ctx, e, "dense panel lag");
+                if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(60))
+                    return set_error(ctx, GBGPU_EDEVICE, "dense panel loop: the side stream made no progress for 60 s");
+            }
+            std::this_thread::yield();
+        }
+        return GBGPU_OK;
     };
     // a select that finds no candidate left sets the flag; every entry point
     // has synchronised its stream before it returned, so no earlier loop's
     // select writes it after this
     *ctx->hdone = 0;
     if (m > 0 && n > 0)
-        select(s, 0, pan);
+        select(s, 0, pan, 0, -1);
+    bool side_queued = false;
     constexpr int LAG = (int)(sizeof(ctx->lag_ev) / sizeof(ctx->lag_ev[0]));
     // most panels prune nothing (every block returns at once): the prune's
     // grid stops at four blocks per CU
@@ -1949,7 +2097,10 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
             // device still has the ones after it queued; every panel but the
             // last records it); stop once a select found nothing left: the
             // panels from there on are empty
-            if (hipEventSynchronize(ctx->lag_ev[it % LAG]) != hipSuccess) {
+            if (ls.w) {
+                const int rc = wait_side(it - LAG);
+                if (rc) return rc;
+            } else if (hipEventSynchronize(ctx->lag_ev[it % LAG]) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense panel lag");
             }
             if (*ctx->hdone) break;
@@ -1969,12 +2120,12 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
             const int32_t gn = (m + PB - 1) / PB;
             const unsigned g = (unsigned)(gn + (prune_pending ? pg : 0));
             switch (dg.nd) {
-                case 1: k_panel_narrow<1, T><<<g, NAR_T, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive); break;
-                case 2: k_panel_narrow<2, T><<<g, NAR_T, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive); break;
+                case 1: k_panel_narrow<1, T><<<g, NAR_T, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive, ls, it); break;
+                case 2: k_panel_narrow<2, T><<<g, NAR_T, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive, ls, it); break;
                 default:
                     if constexpr (sizeof(T) == 4) {
-                        if (dg.nd == 3) k_panel_narrow<3, T><<<g, NAR_T, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive);
-                        else k_panel_narrow<4, T><<<g, NAR_T, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive);
+                        if (dg.nd == 3) k_panel_narrow<3, T><<<g, NAR_T, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive, ls, it);
+                        else k_panel_narrow<4, T><<<g, NAR_T, 0, s>>>(a, ld, m, n, c0, c1, c2, rowpiv, cur, active, rp, ldr, md, dg, gn, cand, st, alive, ls, it);
                     }
                     break;
             }
@@ -1986,17 +2137,35 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
         }
         if (c1 < n) {
             if (!fused) update(s, cur, c0, c1, c2);
-            if (hipEventRecord(ctx->sync_ev[0], s) != hipSuccess) {
+            if (!ls.w && hipEventRecord(ctx->sync_ev[0], s) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
             // the select reads the columns [c1, c2) of candidate rows, which
             // the side stream's work does not touch; queued first, it takes
-            // its CU before the side stream's blocks fill the device
-            select(s, c1, nxt);
-            if (hipStreamWaitEvent(x, ctx->sync_ev[0], 0) != hipSuccess) {
+            // its CU before the side stream's blocks fill the device.  With
+            // the device words it holds the main stream until the side
+            // stream's work of this panel is complete.
+            select(s, c1, nxt, it + 1, it);
+            if (!ls.w && hipStreamWaitEvent(x, ctx->sync_ev[0], 0) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
-            if (fused) {
+            if (fused && ls.w) {
+                // Rp of [c0, c1) and [c2, n) once the select of this panel is
+                // complete; the launch ends after this panel's
+                // k_panel_narrow launch (its multipliers)
+                k_side_wait<<<1, 64, 0, x>>>(ls, it);
+                const unsigned g = (unsigned)((n - c0 + PB - 1) / PB);
+                switch (dg.nd) {
+                    case 1: k_panel_rp_look<1, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg, ls, it); break;
+                    case 2: k_panel_rp_look<2, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg, ls, it); break;
+                    default:
+                        if constexpr (sizeof(T) == 4) {
+                            if (dg.nd == 3) k_panel_rp_look<3, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg, ls, it);
+                            else k_panel_rp_look<4, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg, ls, it);
+                        }
+                        break;
+                }
+            } else if (fused) {
                 // Rp of [c0, c1) and [c2, n) on the matrix cores, the pivot
                 // rows of [c1, c2) from rp
                 const unsigned g = (unsigned)((n - c0 + PB - 1) / PB);
@@ -2035,13 +2204,20 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
                 update(x, cur, c0, c0, c1);
                 update(x, cur, c0, c2, n);
             }
-            // the event that orders the side stream's work before the prune
-            // is also the one the host waits on LAG panels later
-            if (hipEventRecord(ctx->lag_ev[it % LAG], x) != hipSuccess) {
-                return check_hip(ctx, hipGetLastError(), "dense look-ahead");
-            }
-            if (hipStreamWaitEvent(s, ctx->lag_ev[it % LAG], 0) != hipSuccess) {
-                return check_hip(ctx, hipGetLastError(), "dense look-ahead");
+            if (ls.w) {
+                // the pivot rows of [c1, c2), then the words that release the
+                // select's wait and the host
+                k_side_done<T><<<1, 256, 0, x>>>(a, ld, c0, c1, c2, cur, rp, ldr, ls, it);
+                side_queued = true;
+            } else {
+                // the event that orders the side stream's work before the
+                // prune is also the one the host waits on LAG panels later
+                if (hipEventRecord(ctx->lag_ev[it % LAG], x) != hipSuccess) {
+                    return check_hip(ctx, hipGetLastError(), "dense look-ahead");
+                }
+                if (hipStreamWaitEvent(s, ctx->lag_ev[it % LAG], 0) != hipSuccess) {
+                    return check_hip(ctx, hipGetLastError(), "dense look-ahead");
+                }
             }
             // the prune: in the MFMA form inside the next panel's
             // k_panel_narrow (nothing reads the list before the select after
@@ -2053,6 +2229,12 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { return check_hip(ctx, e, "dense panel launch"); }
+    }
+    // the side stream's last kernel has posted its word but may not have
+    // ended: the main stream waits for it once
+    if (side_queued) {
+        if (hipEventRecord(ctx->lag_ev[0], x) != hipSuccess || hipStreamWaitEvent(s, ctx->lag_ev[0], 0) != hipSuccess)
+            return check_hip(ctx, hipGetLastError(), "dense look-ahead");
     }
     (void)hipEventRecord(ctx->ev[3], s);
     return GBGPU_OK;
@@ -2109,6 +2291,14 @@ __global__ void k_to_u32(uint32_t *a, int64_t ld, int32_t m, int32_t n) {
     }
 }
 
+int loop_sync_check(gbgpu_ctx *ctx) {
+    if (!*ctx->herr) return GBGPU_OK;
+    *ctx->herr = 0;
+    return set_error(ctx, GBGPU_EDEVICE,
+                     "dense panel loop: a wait between its two streams ran out of its bound "
+                     "(kernels run one at a time? GBGPU_DENSE_SYNC=events orders them by events)");
+}
+
 int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws) {
     // the MFMA form keeps residues of p <= 65521 in 16 bits while it runs,
     // which halves the bytes every update moves
@@ -2154,6 +2344,7 @@ int dense_output(gbgpu_dense *d, const int32_t *colpiv) {
     dfree(flag);
     dfree(rank);
     if (e != hipSuccess) return check_hip(ctx, e, "dense output");
+    if ((rc = loop_sync_check(ctx))) return rc;
     d->rank = n > 0 ? (int32_t)ctx->hscan[0] : 0;
     return GBGPU_OK;
 }

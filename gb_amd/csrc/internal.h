@@ -137,6 +137,9 @@ void dense_ws_carve(uint8_t *base, int32_t m, int32_t n, int form, uint32_t p, D
 // the panel loop on scratch ws, no allocation: leaves ws.colpiv[c] = the
 // pivot row of column c (-1: none); times itself with ctx->ev[2], ev[3]
 int dense_rref_ws(gbgpu_dense *d, int form, const DenseWs &ws);
+// after a wait for the main stream: GBGPU_EDEVICE if a wait between the dense
+// panel loop's streams ran out of its bound since the loop began
+int loop_sync_check(gbgpu_ctx *ctx);
 // d->rank, d->orow, d->olead from the column -> pivot row map of an
 // eliminated block (decreasing lead order)
 int dense_output(gbgpu_dense *d, const int32_t *colpiv);

@@ -147,6 +147,7 @@ int gbgpu_dense_rref_noalloc(gbgpu_ctx *ctx, uint32_t *a, int32_t m, int32_t n, 
     GBGPU_HIP(ctx, hipMemcpyAsync(lead, olead, (size_t)mn * 4, hipMemcpyDeviceToDevice, s));
     GBGPU_HIP(ctx, hipMemcpyAsync(&hr, count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     GBGPU_HIP(ctx, hipStreamSynchronize(s));
+    if ((rc = loop_sync_check(ctx))) return rc;
     float ms = 0;
     if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]) == hipSuccess) ctx->phase_ms[1] = ms;
     *rank = hr;
