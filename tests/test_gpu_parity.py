@@ -128,6 +128,29 @@ def test_dense_rref_structured(gpu_lib, p, form):
     assert np.array_equal(got, exp)
 
 
+@pytest.mark.parametrize("p", [251, 65521, 16777213, 2147483629])
+@pytest.mark.parametrize("shape,rank", [((330, 200), None), ((256, 520), 100), ((700, 900), None)])
+def test_dense_rref_event_order(gpu_lib, monkeypatch, p, shape, rank):
+    """The MFMA form's panel loop with its two streams ordered by events
+    (GBGPU_DENSE_SYNC=events, read at every call) instead of device words:
+    the same echelon form as the oracle's, and calls in the default order
+    before and after it still match."""
+    a = synth.dense_block(shape[0], shape[1], p, seed=shape[0] + 7, rank=rank)
+    exp, elead = oracle.dense_rref(a, p, nthreads=8)
+    for mode in ("words", "events", "words"):
+        if mode == "events":
+            monkeypatch.setenv("GBGPU_DENSE_SYNC", "events")
+        else:
+            monkeypatch.delenv("GBGPU_DENSE_SYNC", raising=False)
+        d = gpu_lib.dense(a, p)
+        r = d.rref(FORM_MFMA)
+        got, lead = d.result()
+        d.free()
+        assert r == len(exp), mode
+        assert np.array_equal(lead, elead), mode
+        assert np.array_equal(got, exp), mode
+
+
 @pytest.mark.parametrize("p", [101, 65521, 2147483647])
 def test_interreduce(gpu_lib, p):
     rng = np.random.default_rng(p % 97)
