@@ -59,21 +59,28 @@ struct ListState {
 // Ordering of the panel loop's two streams through device words (the MFMA
 // form).  A marker and a wait between the streams cost several
 // microseconds of dispatch each, twice a panel, on the loop's critical
-// path; instead a kernel posts a word of the loop's sequence (s0 + panel)
-// and one thread of the kernel that depends on it waits for it:
-//   w[0]   the k_panel_narrow launch of panel j has begun, so the select of
-//          panel j before it on the main stream is complete;
-//   w[32]  the select of panel j has begun, so the k_panel_narrow launch of
-//          panel j - 1 before it is complete;
-//   w[64]  the side stream's work of panel j is complete;
-//   w[96]  a wait ran out of its bound: the s0 of that loop.
-// Only single blocks wait -- a one-block launch on the side stream, one
-// block of its Rp launch and a select -- and what each waits for never
+// path; instead a kernel posts a word and one thread of the kernel that
+// depends on it waits for it:
+//   w[0]    the k_panel_narrow launch of panel j has begun, so the select of
+//           panel j before it on the main stream is complete (s0 + j);
+//   w[32]   the select of panel j has begun, so the k_panel_narrow launch
+//           of panel j - 1 before it is complete (s0 + j);
+//   w[64]   the side stream's work of panel j is complete (s0 + j);
+//   w[96]   a wait ran out of its bound: the s0 of that loop.
+// Every wait is for a kernel queued by the host before the waiting one: if
+// the two streams share a hardware queue, or a tool runs the kernels one at
+// a time in the order they were queued, what a wait needs has run by then.
+// The side stream's update waits (through its Rp launch) for the next
+// select to have begun, so that the select holds its CU before the
+// update's blocks fill the device.  Only single blocks wait -- one-block
+// launches on either stream and one block of the side stream's Rp launch
+// -- and what each waits for never
 // waits on the side stream's later work, nor for the device to have room
-// for more than one waiting block, so the waits end; each is still bounded, and
-// running out of the bound marks the loop failed (herr) and makes every
-// later wait of the loop return at once.  Releases and acquires are at agent
-// scope: the words and the data they publish cross the XCDs' L2s.
+// for more than one waiting block, so the waits end; each is still
+// bounded, and running out of the bound marks the loop failed (herr) and
+// makes every later wait of the loop return at once.  Releases and
+// acquires are at agent scope: the words and the data they publish cross
+// the XCDs' L2s.
 struct LoopSync {
     int32_t *w = nullptr;      // nullptr: the loop orders its streams by events
     int32_t *herr = nullptr;   // mapped host word, 1 once a wait ran out
@@ -793,15 +800,11 @@ __global__ __launch_bounds__(SEL_T) void k_panel_select(const T *a, int64_t ld, 
                                                         int32_t *rowpiv, int32_t *colpiv,
                                                         Panel *pan, ListState *st, Mod md,
                                                         Digits dg, int32_t *hdone,
-                                                        LoopSync ls = LoopSync(), int32_t j = 0,
-                                                        int32_t wside = -1) {
+                                                        LoopSync ls = LoopSync(), int32_t j = 0) {
     __shared__ SelShared sh;
-    // with ls.w: panel j's select has begun (the launch before it on the
-    // main stream is complete); at the end, the main stream's next launch
-    // (after this one) waits for the side stream's work of panel wside
+    // with ls.w: panel j's select has begun
     if (ls.w && threadIdx.x == 0) ls_post(ls, LS_SEL, ls.s0 + j);
     select_body<P16, T>(sh, a, ld, n, c0, cand, rowpiv, colpiv, pan, st, md, dg, hdone);
-    if (ls.w && wside >= 0 && threadIdx.x == 0) ls_wait(ls, LS_SIDE, ls.s0 + wside);
 }
 
 // ---------------------------------------------------------------- prune
@@ -1634,21 +1637,10 @@ __device__ __forceinline__ void tile_digits(int8_t (*Td)[PB][PB], int8_t (*Pd)[P
 constexpr int NAR_T = 512;  // threads of a k_panel_narrow block (eight waves)
 
 template <int ND, typename T>
-__global__ __launch_bounds__(NAR_T) void k_panel_narrow(T *a, int64_t ld, int32_t m, int32_t n,
-                                                        int32_t c0, int32_t c1, int32_t c2,
-                                                        const int32_t *rowpiv, const Panel *pan,
-                                                        int32_t *active, uint32_t *rp,
-                                                        int64_t ldr, Mod md, Digits dg,
-                                                        int32_t gn, int32_t *cand, ListState *st,
-                                                        int32_t *alive, LoopSync ls = LoopSync(),
-                                                        int32_t jp = 0) {
-    // with ls.w: panel jp's launch has begun, so the select before it is
-    // complete (the side stream's Rp of panel jp waits for this)
-    if (ls.w && blockIdx.x == 0 && threadIdx.x == 0) ls_post(ls, LS_NAR, ls.s0 + jp);
-    if ((int32_t)blockIdx.x >= gn) {
-        prune_body(a, ld, n, c0, cand, st, alive, (int32_t)blockIdx.x - gn, (int32_t)gridDim.x - gn);
-        return;
-    }
+__device__ __forceinline__ void narrow_body(T *a, int64_t ld, int32_t m, int32_t c0, int32_t c1,
+                                            int32_t c2, const int32_t *rowpiv, const Panel *pan,
+                                            int32_t *active, uint32_t *rp, int64_t ldr,
+                                            const Mod &md, const Digits &dg) {
     const int kk = __builtin_amdgcn_readfirstlane(pan->k);
     if (kk == 0) return;
     __shared__ __align__(16) int8_t Td[ND][PB][PB];  // T: [digit][t][s]
@@ -1781,6 +1773,31 @@ __global__ __launch_bounds__(NAR_T) void k_panel_narrow(T *a, int64_t ld, int32_
     }
 }
 
+template <int ND, typename T>
+__global__ __launch_bounds__(NAR_T) void k_panel_narrow(T *a, int64_t ld, int32_t m, int32_t n,
+                                                        int32_t c0, int32_t c1, int32_t c2,
+                                                        const int32_t *rowpiv, const Panel *pan,
+                                                        int32_t *active, uint32_t *rp,
+                                                        int64_t ldr, Mod md, Digits dg,
+                                                        int32_t gn, int32_t *cand, ListState *st,
+                                                        int32_t *alive, LoopSync ls = LoopSync(),
+                                                        int32_t jp = 0) {
+    // with ls.w: panel jp's launch has begun, so the select before it is
+    // complete (the side stream's Rp of panel jp waits for this)
+    if (ls.w && blockIdx.x == 0 && threadIdx.x == 0) ls_post(ls, LS_NAR, ls.s0 + jp);
+    if ((int32_t)blockIdx.x >= gn) {
+        prune_body(a, ld, n, c0, cand, st, alive, (int32_t)blockIdx.x - gn, (int32_t)gridDim.x - gn);
+        return;
+    }
+    narrow_body<ND, T>(a, ld, m, c0, c1, c2, rowpiv, pan, active, rp, ldr, md, dg);
+}
+
+// queued on the main stream after panel j's side stream work: one thread
+// waits until that work is complete (the next panel's launch reads it)
+__global__ void k_main_wait(LoopSync ls, int32_t j) {
+    if (threadIdx.x == 0) ls_wait(ls, LS_SIDE, ls.s0 + j);
+}
+
 // Rp of the columns [c0, jhi) in the MFMA form, a block per 64 columns
 // [c0 + 64 b, c0 + 64 b + 64): T times the pivot rows' entries there, on
 // the matrix cores in signed base-256 digits; writes the pivot rows (in
@@ -1868,20 +1885,23 @@ __global__ void k_side_wait(LoopSync ls, int32_t j) {
     if (threadIdx.x == 0) ls_wait(ls, LS_NAR, ls.s0 + j);
 }
 
-// k_panel_rp_mfma for the loop ordered by device words (panel j, after
+// k_panel_rp_mfma for the loop ordered by device words (after
 // k_side_wait): the block at column wb (the next panel's, whose pivot rows
-// k_side_done writes) has no Rp to compute and instead waits until the next
-// select has begun, so that this launch ends only after panel j's
-// k_panel_narrow launch, whose multipliers the update after it reads.
-// Nothing this launch writes is read by that launch: the pivot rows'
-// entries outside [wb, wb + 64).
+// k_side_done writes) has no Rp to compute and instead waits until the
+// select of the next panel has begun (w[32] reaching nwant), so that this
+// launch ends only after this panel's k_panel_narrow launch (before that
+// select on the main stream): its multipliers, which the update after this
+// launch reads, are written and its prune's reads of the rows that the
+// update writes are done.  Nothing this
+// launch writes is read by that launch: the pivot rows' entries outside
+// [wb, wb + 64).
 template <int ND, typename T>
 __global__ __launch_bounds__(256) void k_panel_rp_look(T *a, int64_t ld, int32_t c0, int32_t jhi,
                                                        int32_t wb, const Panel *pan,
                                                        const uint32_t *rp, int64_t ldr, Mod md,
-                                                       Digits dg, LoopSync ls, int32_t j) {
+                                                       Digits dg, LoopSync ls, int32_t nwant) {
     if (c0 + (int32_t)blockIdx.x * PB == wb) {
-        if (threadIdx.x == 0) ls_wait(ls, LS_SEL, ls.s0 + j + 1);
+        if (threadIdx.x == 0) ls_wait(ls, LS_SEL, nwant);
         return;
     }
     rp_mfma_body<ND, T>(a, ld, c0, jhi, -1, pan, rp, ldr, md, dg, (int32_t)blockIdx.x);
@@ -2001,6 +2021,21 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
     dg.R = dr;
     dg.lplane = (int64_t)m * PB;
     dg.rplane = ldr * PB;
+    // the MFMA form orders the two streams through device words (unless
+    // GBGPU_DENSE_SYNC=events asks for events); this loop's sequence words
+    // are s0 .. s0 + panels
+    LoopSync ls;
+    if (form == GBGPU_FORM_MFMA) {
+        const char *e = getenv("GBGPU_DENSE_SYNC");
+        if (!(e && strcmp(e, "events") == 0)) {
+            ls.w = ctx->dsync;
+            ls.herr = ctx->derr;
+            ls.hprog = ctx->dprog;
+            ls.s0 = ctx->sync_seq;
+            ctx->sync_seq += n / PB + 4;
+            *ctx->herr = 0;
+        }
+    }
     k_dense_init<<<grid_of(m > n ? m : n, 256), 256, 0, s>>>(m, n, rowpiv, colpiv, cand, pan, st);
     const uint32_t r64 = (uint32_t)((((unsigned __int128)1) << 64) % d->p);
     // A[r][cb, ce) -= L[r] * Rp for the rows of the current panel's update
@@ -2041,27 +2076,11 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
     // between two slots; the candidate list is touched by the select and the
     // prune only, both on the main stream.
     hipStream_t x = ctx->side;
-    // the MFMA form orders the two streams through device words (unless
-    // GBGPU_DENSE_SYNC=events asks for events, e.g. where a tool runs the
-    // kernels one at a time: a wait for a kernel queued behind it would run
-    // out of its bound); this loop's words are s0 .. s0 + panels
-    LoopSync ls;
-    if (form == GBGPU_FORM_MFMA) {
-        const char *e = getenv("GBGPU_DENSE_SYNC");
-        if (!(e && strcmp(e, "events") == 0)) {
-            ls.w = ctx->dsync;
-            ls.herr = ctx->derr;
-            ls.hprog = ctx->dprog;
-            ls.s0 = ctx->sync_seq;
-            ctx->sync_seq += n / PB + 4;
-            *ctx->herr = 0;
-        }
-    }
-    auto select = [&](hipStream_t sx, int32_t cs, Panel *pp, int32_t j, int32_t wside) {
+    auto select = [&](hipStream_t sx, int32_t cs, Panel *pp, int32_t j) {
         if (d->p < (1u << 16))
-            k_panel_select<1, T><<<1, SEL_T, 0, sx>>>(a, ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg, ctx->ddone, ls, j, wside);
+            k_panel_select<1, T><<<1, SEL_T, 0, sx>>>(a, ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg, ctx->ddone, ls, j);
         else
-            k_panel_select<0, T><<<1, SEL_T, 0, sx>>>(a, ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg, ctx->ddone, ls, j, wside);
+            k_panel_select<0, T><<<1, SEL_T, 0, sx>>>(a, ld, n, cs, cand, rowpiv, colpiv, pp, st, md, dg, ctx->ddone, ls, j);
     };
     // the host stays at most LAG panels ahead: with the device words it
     // reads the side stream's last panel done from mapped memory
@@ -2083,7 +2102,7 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
     // select writes it after this
     *ctx->hdone = 0;
     if (m > 0 && n > 0)
-        select(s, 0, pan, 0, -1);
+        select(s, 0, pan, 0);
     bool side_queued = false;
     constexpr int LAG = (int)(sizeof(ctx->lag_ev) / sizeof(ctx->lag_ev[0]));
     // most panels prune nothing (every block returns at once): the prune's
@@ -2142,10 +2161,8 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
             }
             // the select reads the columns [c1, c2) of candidate rows, which
             // the side stream's work does not touch; queued first, it takes
-            // its CU before the side stream's blocks fill the device.  With
-            // the device words it holds the main stream until the side
-            // stream's work of this panel is complete.
-            select(s, c1, nxt, it + 1, it);
+            // its CU before the side stream's blocks fill the device
+            select(s, c1, nxt, it + 1);
             if (!ls.w && hipStreamWaitEvent(x, ctx->sync_ev[0], 0) != hipSuccess) {
                 return check_hip(ctx, hipGetLastError(), "dense look-ahead");
             }
@@ -2155,13 +2172,14 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
                 // k_panel_narrow launch (its multipliers)
                 k_side_wait<<<1, 64, 0, x>>>(ls, it);
                 const unsigned g = (unsigned)((n - c0 + PB - 1) / PB);
+                const int32_t nwant = ls.s0 + it + 1;
                 switch (dg.nd) {
-                    case 1: k_panel_rp_look<1, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg, ls, it); break;
-                    case 2: k_panel_rp_look<2, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg, ls, it); break;
+                    case 1: k_panel_rp_look<1, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg, ls, nwant); break;
+                    case 2: k_panel_rp_look<2, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg, ls, nwant); break;
                     default:
                         if constexpr (sizeof(T) == 4) {
-                            if (dg.nd == 3) k_panel_rp_look<3, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg, ls, it);
-                            else k_panel_rp_look<4, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg, ls, it);
+                            if (dg.nd == 3) k_panel_rp_look<3, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg, ls, nwant);
+                            else k_panel_rp_look<4, T><<<g, 256, 0, x>>>(a, ld, c0, n, c1, cur, rp, ldr, md, dg, ls, nwant);
                         }
                         break;
                 }
@@ -2206,9 +2224,10 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
             }
             if (ls.w) {
                 // the pivot rows of [c1, c2), then the words that release the
-                // select's wait and the host
+                // main stream's wait (queued after this) and the host
                 k_side_done<T><<<1, 256, 0, x>>>(a, ld, c0, c1, c2, cur, rp, ldr, ls, it);
                 side_queued = true;
+                k_main_wait<<<1, 64, 0, s>>>(ls, it);
             } else {
                 // the event that orders the side stream's work before the
                 // prune is also the one the host waits on LAG panels later
