@@ -128,6 +128,30 @@ def test_dense_rref_structured(gpu_lib, p, form):
     assert np.array_equal(got, exp)
 
 
+def test_dense_rref_many_contexts():
+    """Six contexts alive at once (twelve streams, more than the process's
+    hardware queues, so some streams share one): the MFMA panel loop of each,
+    ordered by device words whose waits are all for work queued before the
+    waiting kernel, finishes and matches the oracle."""
+    from gb_amd import native
+    p = 65521
+    a = synth.dense_block(300, 700, p, seed=11)
+    exp, elead = oracle.dense_rref(a, p, nthreads=8)
+    ctxs = [native.Context(0) for _ in range(6)]
+    try:
+        for k, c in enumerate(ctxs):
+            d = c.dense(a, p)
+            r = d.rref(FORM_MFMA)
+            got, lead = d.result()
+            d.free()
+            assert r == len(exp), k
+            assert np.array_equal(lead, elead), k
+            assert np.array_equal(got, exp), k
+    finally:
+        for c in ctxs:
+            c.close()
+
+
 @pytest.mark.parametrize("p", [251, 65521, 16777213, 2147483629])
 @pytest.mark.parametrize("shape,rank", [((330, 200), None), ((256, 520), 100), ((700, 900), None)])
 def test_dense_rref_event_order(gpu_lib, monkeypatch, p, shape, rank):
