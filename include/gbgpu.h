@@ -192,6 +192,10 @@ int gbgpu_la_ff(gbgpu_ctx *ctx, const gbgpu_csr_t *m, int la_option, int form,
 int gbgpu_dense_upload(gbgpu_ctx *ctx, int32_t m, int32_t n, uint32_t p,
                        const uint32_t *host, gbgpu_dense **out);
 void gbgpu_dense_free(gbgpu_dense *d);
+/* The MFMA form's panel loop orders its two streams through device words
+ * (every wait is for work queued before it, bounded; a wait that runs out
+ * returns GBGPU_EDEVICE); GBGPU_DENSE_SYNC=events in the environment, read at
+ * each call, orders them by HIP events instead. */
 int gbgpu_dense_rref(gbgpu_ctx *ctx, gbgpu_dense *d, int form, int32_t *rank);
 /* rows: rank x n full-width RREF rows in decreasing lead order; lead: rank
  * lead columns.  Either may be NULL. */
