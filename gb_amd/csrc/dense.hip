@@ -2200,12 +2200,16 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
             } else if (c2 < n) {
                 launch_rp<T>(x, false, a, ld, c0, c2, n, cur, rp, ldr, md, r64, dg);
             }
-            if (form == GBGPU_FORM_MFMA && c2 < n && c1 - c0 == PB && dg.nd <= 2) {
+            // one and two digits: 32 x 64 tiles up to 6144 rows, the strip
+            // walk beyond (la_option 1's 8000 x 18000 dense part: 27.1 ->
+            // 25.0 ms; the 4096 x 8192 block the same either way)
+            if (form == GBGPU_FORM_MFMA && c2 < n && c1 - c0 == PB && dg.nd <= 2 && m < 6144) {
                 update(x, cur, c0, c0, n, c1, c2 - c1);  // [c0, c1) and [c2, n)
             } else if (form == GBGPU_FORM_MFMA && c2 < n && c1 - c0 == PB) {
-                // [c0, c1) and [c2, n) for three and four digits: blocks that
-                // walk 512 columns with the Rp digits staged in LDS (measured
-                // faster than 64 x 64 tiles there, slower for one or two)
+                // [c0, c1) and [c2, n) for three and four digits and for tall
+                // blocks: blocks that walk 512 columns with the Rp digits
+                // staged in LDS (measured faster than 64 x 64 tiles for three
+                // and four digits)
                 const int rows = dg.nd <= 2 ? 128 : 64;
                 dim3 g((unsigned)((n - c0 - (c2 - c1) + 511) / 512), (unsigned)((m + rows - 1) / rows));
                 switch (dg.nd) {

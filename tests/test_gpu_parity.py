@@ -128,6 +128,21 @@ def test_dense_rref_structured(gpu_lib, p, form):
     assert np.array_equal(got, exp)
 
 
+@pytest.mark.parametrize("p", [251, 65521])
+def test_dense_rref_tall_strip_update(gpu_lib, p):
+    """6400 rows: the MFMA form's side-stream update for one and two digits
+    walks 512-column strips (k_update_wide) instead of 32 x 64 tiles."""
+    a = synth.dense_block(6400, 800, p, seed=21)
+    exp, elead = oracle.dense_rref(a, p, nthreads=8)
+    d = gpu_lib.dense(a, p)
+    r = d.rref(FORM_MFMA)
+    got, lead = d.result()
+    d.free()
+    assert r == len(exp)
+    assert np.array_equal(lead, elead)
+    assert np.array_equal(got, exp)
+
+
 def test_dense_rref_many_contexts():
     """Six contexts alive at once (twelve streams, more than the process's
     hardware queues, so some streams share one): the MFMA panel loop of each,
