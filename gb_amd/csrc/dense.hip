@@ -68,8 +68,10 @@ struct ListState {
 //   w[64]   the side stream's work of panel j is complete (s0 + j);
 //   w[96]   a wait ran out of its bound: the s0 of that loop.
 // Every wait is for a kernel queued by the host before the waiting one: if
-// the two streams share a hardware queue, or a tool runs the kernels one at
-// a time in the order they were queued, what a wait needs has run by then.
+// the two streams share a hardware queue, which runs them in that order,
+// what a wait needs has run by then.  A tool that runs kernels one at a
+// time in another order (rocprofv3 --pmc does) makes the waits run out:
+// such runs take GBGPU_DENSE_SYNC=events.
 // The side stream's update waits (through its Rp launch) for the next
 // select to have begun, so that the select holds its CU before the
 // update's blocks fill the device.  Only single blocks wait -- one-block

@@ -195,7 +195,8 @@ void gbgpu_dense_free(gbgpu_dense *d);
 /* The MFMA form's panel loop orders its two streams through device words
  * (every wait is for work queued before it, bounded; a wait that runs out
  * returns GBGPU_EDEVICE); GBGPU_DENSE_SYNC=events in the environment, read at
- * each call, orders them by HIP events instead. */
+ * each call, orders them by HIP events instead, as runs under a tool that
+ * serialises kernels across queues (rocprofv3 --pmc) need. */
 int gbgpu_dense_rref(gbgpu_ctx *ctx, gbgpu_dense *d, int form, int32_t *rank);
 /* rows: rank x n full-width RREF rows in decreasing lead order; lead: rank
  * lead columns.  Either may be NULL. */
