@@ -2212,11 +2212,13 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
                 // blocks: blocks that walk 512 columns with the Rp digits
                 // staged in LDS (measured faster than 64 x 64 tiles for three
                 // and four digits)
-                const int rows = dg.nd <= 2 ? 128 : 64;
+                // (64-row strips: 128 measured 6 % slower on la_option 1's
+                // dense part)
+                const int rows = 64;
                 dim3 g((unsigned)((n - c0 - (c2 - c1) + 511) / 512), (unsigned)((m + rows - 1) / rows));
                 switch (dg.nd) {
-                    case 1: k_update_wide<1, 2, T, 1><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1); break;
-                    case 2: k_update_wide<2, 2, T, 1><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1); break;
+                    case 1: k_update_wide<1, 1, T, 1><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1); break;
+                    case 2: k_update_wide<2, 1, T, 1><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1); break;
                     default:
                         if constexpr (sizeof(T) == 4) {
                             if (dg.nd == 3) k_update_wide<3, 1, T, 1><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1);
