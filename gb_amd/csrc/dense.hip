@@ -1482,14 +1482,14 @@ __device__ __forceinline__ void update_wide_body(T *a, int64_t ld, int32_t m, in
     }
 }
 
-template <int ND, int RS, typename T = uint32_t, int KC = 4>
+template <int ND, int RS, typename T = uint32_t, int KC = 4, int NT = WT_NT>
 __global__ __launch_bounds__(256) void k_update_wide(T *a, int64_t ld, int32_t m,
                                                      int32_t cb, int32_t ce, int32_t c0,
                                                      const Panel *pan, const int32_t *active,
                                                      Mod md, Digits dg, int32_t gb = INT_MAX,
                                                      int32_t gsz = 0) {
-    update_wide_body<ND, RS, T, KC>(a, ld, m, cb, ce, c0, pan, active, md, dg, gb, gsz,
-                                    (int32_t)blockIdx.x, (int32_t)blockIdx.y);
+    update_wide_body<ND, RS, T, KC, NT>(a, ld, m, cb, ce, c0, pan, active, md, dg, gb, gsz,
+                                        (int32_t)blockIdx.x, (int32_t)blockIdx.y);
 }
 
 template <typename T>
@@ -2209,16 +2209,18 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
                 update(x, cur, c0, c0, n, c1, c2 - c1);  // [c0, c1) and [c2, n)
             } else if (form == GBGPU_FORM_MFMA && c2 < n && c1 - c0 == PB) {
                 // [c0, c1) and [c2, n) for three and four digits and for tall
-                // blocks: blocks that walk 512 columns with the Rp digits
-                // staged in LDS (measured faster than 64 x 64 tiles for three
-                // and four digits)
-                // (64-row strips: 128 measured 6 % slower on la_option 1's
-                // dense part)
+                // blocks: 64-row blocks that walk column strips with the Rp
+                // digits staged in LDS (measured faster than 64 x 64 tiles
+                // for three and four digits; 128-row blocks measured 6 %
+                // slower on la_option 1's dense part)
                 const int rows = 64;
-                dim3 g((unsigned)((n - c0 - (c2 - c1) + 511) / 512), (unsigned)((m + rows - 1) / rows));
+                // one and two digits: 256-column strips (8 tiles; 512 and
+                // 1024 measured 2 and 8 % slower there); three and four: 512
+                const int sw = dg.nd <= 2 ? 256 : 512;
+                dim3 g((unsigned)((n - c0 - (c2 - c1) + sw - 1) / sw), (unsigned)((m + rows - 1) / rows));
                 switch (dg.nd) {
-                    case 1: k_update_wide<1, 1, T, 1><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1); break;
-                    case 2: k_update_wide<2, 1, T, 1><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1); break;
+                    case 1: k_update_wide<1, 1, T, 1, 8><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1); break;
+                    case 2: k_update_wide<2, 1, T, 1, 8><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1); break;
                     default:
                         if constexpr (sizeof(T) == 4) {
                             if (dg.nd == 3) k_update_wide<3, 1, T, 1><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1);
