@@ -2202,10 +2202,10 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
             } else if (c2 < n) {
                 launch_rp<T>(x, false, a, ld, c0, c2, n, cur, rp, ldr, md, r64, dg);
             }
-            // one and two digits: 32 x 64 tiles up to 6144 rows, the strip
-            // walk beyond (la_option 1's 8000 x 18000 dense part: 27.1 ->
-            // 25.0 ms; the 4096 x 8192 block the same either way)
-            if (form == GBGPU_FORM_MFMA && c2 < n && c1 - c0 == PB && dg.nd <= 2 && m < 6144) {
+            // one and two digits: 32 x 64 tiles below 4096 rows, the strip
+            // walk from there on (la_option 1's 8000 x 18000 dense part:
+            // 27.1 -> 23.0 ms; the 4096 x 8192 block 1-2 % faster)
+            if (form == GBGPU_FORM_MFMA && c2 < n && c1 - c0 == PB && dg.nd <= 2 && m < 4096) {
                 update(x, cur, c0, c0, n, c1, c2 - c1);  // [c0, c1) and [c2, n)
             } else if (form == GBGPU_FORM_MFMA && c2 < n && c1 - c0 == PB) {
                 // [c0, c1) and [c2, n) for three and four digits and for tall

@@ -130,8 +130,9 @@ def test_dense_rref_structured(gpu_lib, p, form):
 
 @pytest.mark.parametrize("p", [251, 65521])
 def test_dense_rref_tall_strip_update(gpu_lib, p):
-    """6400 rows: the MFMA form's side-stream update for one and two digits
-    walks 512-column strips (k_update_wide) instead of 32 x 64 tiles."""
+    """6400 rows (4096 or more): the MFMA form's side-stream update for one
+    and two digits walks 256-column strips (k_update_wide) instead of 32 x 64
+    tiles."""
     a = synth.dense_block(6400, 800, p, seed=21)
     exp, elead = oracle.dense_rref(a, p, nthreads=8)
     d = gpu_lib.dense(a, p)
