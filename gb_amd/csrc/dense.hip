@@ -2214,17 +2214,18 @@ static int panel_loop(gbgpu_dense *d, T *a, int64_t ld, int form, const DenseWs 
                 // for three and four digits; 128-row blocks measured 6 %
                 // slower on la_option 1's dense part)
                 const int rows = 64;
-                // one and two digits: 256-column strips (8 tiles; 512 and
-                // 1024 measured 2 and 8 % slower there); three and four: 512
-                const int sw = dg.nd <= 2 ? 256 : 512;
+                // 256-column strips (8 tiles): 512 measured 2 % slower for
+                // two digits on la_option 1's dense part and 4-7 % slower for
+                // three and four on the 4096 x 8192 block, 1024 8 % slower
+                const int sw = 256;
                 dim3 g((unsigned)((n - c0 - (c2 - c1) + sw - 1) / sw), (unsigned)((m + rows - 1) / rows));
                 switch (dg.nd) {
                     case 1: k_update_wide<1, 1, T, 1, 8><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1); break;
                     case 2: k_update_wide<2, 1, T, 1, 8><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1); break;
                     default:
                         if constexpr (sizeof(T) == 4) {
-                            if (dg.nd == 3) k_update_wide<3, 1, T, 1><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1);
-                            else k_update_wide<4, 1, T, 1><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1);
+                            if (dg.nd == 3) k_update_wide<3, 1, T, 1, 8><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1);
+                            else k_update_wide<4, 1, T, 1, 8><<<g, 256, 0, x>>>(a, ld, m, c0, n, c0, cur, active, md, dg, c1, c2 - c1);
                         }
                         break;
                 }
